@@ -1,0 +1,331 @@
+"""FP64 CPU oracle for PNFFT's single-process NFFT path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the CPU-baseline leg of ``bench.py`` -- never by the product package.
+
+The C part (``pnfft_oracle.c``) restates the reference's windows, matrix D,
+matrix B (on the ghost-padded grid) and the NDFT; this module composes them
+into ``trafo``/``adj`` the way ``api/api-basic.c:170-196`` and ``:307-333`` do,
+with the FFT stage (PFFT in the reference, ``kernel/ndft-parallel.c:1524-1546``)
+taken by scipy.fft (pocketfft) on the host.
+"""
+import ctypes
+import os
+
+import numpy as np
+import scipy.fft
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+# flag values of api/pnfft.h:302-390
+PNFFT_FAST_GAUSSIAN = 1 << 1
+PNFFT_WINDOW_KAISER_BESSEL = 0
+PNFFT_WINDOW_GAUSSIAN = 1 << 13
+PNFFT_WINDOW_BSPLINE = 1 << 14
+PNFFT_WINDOW_SINC_POWER = 1 << 15
+PNFFT_WINDOW_BESSEL_I0 = 1 << 16
+PNFFT_COMPUTE_F = 1 << 0
+PNFFT_COMPUTE_GRAD_F = 1 << 1
+PNFFT_COMPUTE_ACCUMULATED = 1 << 4
+
+W_KB, W_GAUSS, W_BSPLINE, W_SINCPOW, W_BESSEL_I0 = range(5)
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the Makefile beside this file."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        lp = ctypes.POINTER(ctypes.c_long)
+        L.orc_window_id.argtypes = [ctypes.c_uint]
+        L.orc_window_id.restype = ctypes.c_int
+        L.orc_window_param_b.argtypes = [lp, lp, ctypes.c_int, ctypes.c_uint, dp]
+        L.orc_bessel_i0.argtypes = [ctypes.c_double]
+        L.orc_bessel_i0.restype = ctypes.c_double
+        L.orc_sinc.argtypes = [ctypes.c_double]
+        L.orc_sinc.restype = ctypes.c_double
+        L.orc_bspline.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_bspline.restype = ctypes.c_double
+        L.orc_inv_phi_hat.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.c_double, ctypes.c_int]
+        L.orc_inv_phi_hat.restype = ctypes.c_double
+        L.orc_psi_axis.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_double, dp, dp]
+        L.orc_psi.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        L.orc_psi.restype = ctypes.c_double
+        L.orc_dpsi.argtypes = L.orc_psi.argtypes
+        L.orc_dpsi.restype = ctypes.c_double
+        L.orc_ndft_trafo_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
+        L.orc_ndft_adj_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
+        L.orc_ndft_trafo_c2r.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
+        L.orc_inv_phi_hat_tables.argtypes = [lp, lp, ctypes.c_int, ctypes.c_uint, dp]
+        L.orc_D.argtypes = [lp, dp, dp, dp, ctypes.c_int]
+        L.orc_B_trafo.argtypes = [lp, lp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_long, dp, dp,
+                                  ctypes.c_int, dp, dp, ctypes.c_uint]
+        L.orc_B_adj.argtypes = [lp, lp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_long, dp, dp, dp,
+                                ctypes.c_int, dp, ctypes.c_uint]
+        L.orc_sort_nodes.argtypes = [lp, ctypes.c_int, ctypes.c_long, dp, lp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+def _lp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
+
+
+def _longs(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.int64).reshape(3))
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# plan-level parameters
+# ---------------------------------------------------------------------------
+def fft_output_size(n, x_max, m):
+    """api/api-guru.c:195-218: no_t = min(n_t, 2*(floor(n_t x_max_t) + m + 2))."""
+    return [min(int(nt), 2 * (int(np.floor(nt * xm)) + m + 2)) for nt, xm in zip(n, x_max)]
+
+
+def window_id(flags):
+    return lib().orc_window_id(flags)
+
+
+def window_b(N, n, m, flags):
+    b = np.zeros(3)
+    lib().orc_window_param_b(_lp(_longs(N)), _lp(_longs(n)), m, flags, _dp(b))
+    return b
+
+
+def bessel_i0(x):
+    return lib().orc_bessel_i0(float(x))
+
+
+def sinc(x):
+    return lib().orc_sinc(float(x))
+
+
+def bspline(k, x):
+    return lib().orc_bspline(int(k), float(x))
+
+
+def inv_phi_hat(window, k, n, b, m):
+    return lib().orc_inv_phi_hat(window, int(k), int(n), float(b), int(m))
+
+
+def inv_phi_hat_tables(N, n, m, flags):
+    N = _longs(N)
+    tab = np.zeros(int(N.sum()))
+    lib().orc_inv_phi_hat_tables(_lp(N), _lp(_longs(n)), m, flags, _dp(tab))
+    return tab
+
+
+def psi_axis(window, n, b, m, nx, want_dpsi=True):
+    c = 2 * m + 1
+    psi = np.zeros(c)
+    dpsi = np.zeros(c) if want_dpsi else None
+    lib().orc_psi_axis(window, int(n), float(b), int(m), float(nx), _dp(psi), _dp(dpsi))
+    return (psi, dpsi) if want_dpsi else psi
+
+
+def psi(window, n, b, m, x):
+    return lib().orc_psi(window, int(n), float(b), int(m), float(x))
+
+
+def dpsi(window, n, b, m, x):
+    return lib().orc_dpsi(window, int(n), float(b), int(m), float(x))
+
+
+# ---------------------------------------------------------------------------
+# NDFT
+# ---------------------------------------------------------------------------
+def ndft_trafo(N, x, fhat, compute_grad=False, compute_f=True):
+    """Direct sums of kernel/ndft-parallel.c:377-613 (c2c). Returns (f, grad)."""
+    N = _longs(N)
+    x = _f64(x).reshape(-1)
+    M = x.size // 3
+    fhat = _c128(fhat).reshape(-1)
+    assert fhat.size == int(np.prod(N))
+    f = np.zeros(M, np.complex128)
+    grad = np.zeros(3 * M, np.complex128) if compute_grad else None
+    fl = (PNFFT_COMPUTE_F if compute_f else 0) | (PNFFT_COMPUTE_GRAD_F if compute_grad else 0)
+    lib().orc_ndft_trafo_c2c(_lp(N), M, _dp(x), _dp(fhat.view(np.float64)), _dp(f.view(np.float64)),
+                             _dp(grad.view(np.float64)) if grad is not None else None, fl)
+    return (f if compute_f else None), grad
+
+
+def ndft_adj(N, x, f=None, grad=None, fhat_in=None):
+    """Direct adjoint sums of kernel/ndft-parallel.c:617-722 (adds onto fhat_in)."""
+    N = _longs(N)
+    x = _f64(x).reshape(-1)
+    M = x.size // 3
+    fhat = np.zeros(int(np.prod(N)), np.complex128) if fhat_in is None else _c128(fhat_in).reshape(-1).copy()
+    fl = 0
+    fa = ga = None
+    if f is not None:
+        fa = _c128(f).reshape(-1)
+        fl |= PNFFT_COMPUTE_F
+    if grad is not None:
+        ga = _c128(grad).reshape(-1)
+        fl |= PNFFT_COMPUTE_GRAD_F
+    lib().orc_ndft_adj_c2c(_lp(N), M, _dp(x), _dp(fa.view(np.float64)) if fa is not None else None,
+                           _dp(ga.view(np.float64)) if ga is not None else None,
+                           _dp(fhat.view(np.float64)), fl)
+    return fhat
+
+
+def ndft_trafo_c2r(N, x, fhat_half, compute_grad=False):
+    N = _longs(N)
+    x = _f64(x).reshape(-1)
+    M = x.size // 3
+    fh = _c128(fhat_half).reshape(-1)
+    assert fh.size == int(N[0] * N[1] * (N[2] // 2 + 1))
+    f = np.zeros(M)
+    grad = np.zeros(3 * M) if compute_grad else None
+    fl = PNFFT_COMPUTE_F | (PNFFT_COMPUTE_GRAD_F if compute_grad else 0)
+    lib().orc_ndft_trafo_c2r(_lp(N), M, _dp(x), _dp(fh.view(np.float64)), _dp(f), _dp(grad), fl)
+    return f, grad
+
+
+# ---------------------------------------------------------------------------
+# Fast NFFT (D -> F -> ghost cells -> B), c2c
+# ---------------------------------------------------------------------------
+class Plan:
+    """Parameters of one oracle plan (api/api-guru.c:110-164, kernel/ndft-parallel.c:869-1069)."""
+
+    def __init__(self, N, n=None, m=6, flags=0, x_max=(0.5, 0.5, 0.5), workers=-1):
+        self.N = [int(v) for v in N]
+        self.n = [2 * v for v in self.N] if n is None else [int(v) for v in n]
+        self.m = int(m)
+        self.flags = int(flags)
+        self.x_max = list(x_max)
+        self.no = fft_output_size(self.n, self.x_max, self.m)
+        self.window = window_id(self.flags)
+        self.b = window_b(self.N, self.n, self.m, self.flags)
+        self.tab = inv_phi_hat_tables(self.N, self.n, self.m, self.flags)
+        self.workers = workers
+
+    # -- D (kernel/matrix_D.c:227-280)
+    def D(self, fhat):
+        out = np.empty(int(np.prod(self.N)), np.complex128)
+        src = _c128(fhat).reshape(-1)
+        lib().orc_D(_lp(_longs(self.N)), _dp(self.tab), _dp(src.view(np.float64)), _dp(out.view(np.float64)), 0)
+        return out
+
+    def DT_accumulate(self, g1, fhat):
+        src = _c128(g1).reshape(-1)
+        lib().orc_D(_lp(_longs(self.N)), _dp(self.tab), _dp(src.view(np.float64)), _dp(fhat.view(np.float64)), 1)
+
+    # -- F: pruned, shifted, unnormalized forward FFT N -> no (doc/manual.tex:178-243)
+    def F(self, g1):
+        N, n, no = self.N, self.n, self.no
+        A = np.zeros(n, np.complex128)
+        idx = [np.arange(-Nt // 2, Nt // 2) % nt for Nt, nt in zip(N, n)]
+        A[np.ix_(*idx)] = np.asarray(g1).reshape(N)
+        G = scipy.fft.fftn(A, workers=self.workers)
+        oidx = [np.arange(-nt // 2, nt // 2) % nft for nt, nft in zip(no, n)]
+        return G[np.ix_(*oidx)]
+
+    # -- F^H: backward FFT no -> N
+    def FH(self, g2):
+        N, n, no = self.N, self.n, self.no
+        A = np.zeros(n, np.complex128)
+        oidx = [np.arange(-nt // 2, nt // 2) % nft for nt, nft in zip(no, n)]
+        A[np.ix_(*oidx)] = g2
+        G = scipy.fft.ifftn(A, workers=self.workers) * float(np.prod(n))
+        idx = [np.arange(-Nt // 2, Nt // 2) % nt for Nt, nt in zip(N, n)]
+        return G[np.ix_(*idx)].reshape(-1)
+
+    # -- ghost cells (PX(exchange)/PX(reduce), kernel/ndft-parallel.c:2558,2679):
+    # m planes below, m above, periodic over the FFT output of size no.
+    def ghost_exchange(self, g2):
+        m = self.m
+        return np.pad(g2, [(m, m)] * 3, mode="wrap")
+
+    def ghost_reduce(self, gpad):
+        m = self.m
+        arr = gpad
+        # fold every padded index i of each axis back to (i - m) mod no, axis by axis
+        for t in range(3):
+            nt = self.no[t]
+            moved = np.moveaxis(arr, t, 0)
+            res = np.zeros((nt,) + moved.shape[1:], arr.dtype)
+            np.add.at(res, (np.arange(moved.shape[0]) - m) % nt, moved)
+            arr = np.moveaxis(res, 0, t)
+        return arr
+
+    def trafo(self, x, fhat, compute_f=True, compute_grad=False):
+        """pnfft_trafo (api/api-basic.c:199-244) on host arrays; returns (f, grad)."""
+        x = _f64(x).reshape(-1)
+        M = x.size // 3
+        g2 = self.F(self.D(fhat))
+        gpad = np.ascontiguousarray(self.ghost_exchange(g2))
+        f = np.zeros(M, np.complex128) if compute_f else None
+        grad = np.zeros(3 * M, np.complex128) if compute_grad else None
+        fl = (PNFFT_COMPUTE_F if compute_f else 0) | (PNFFT_COMPUTE_GRAD_F if compute_grad else 0)
+        lib().orc_B_trafo(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
+                          _dp(x), _dp(gpad.view(np.float64)), 0,
+                          _dp(f.view(np.float64)) if f is not None else None,
+                          _dp(grad.view(np.float64)) if grad is not None else None, fl)
+        return f, grad
+
+    def adj(self, x, f=None, grad=None, fhat_in=None):
+        """pnfft_adj (api/api-basic.c:344-378); adds onto fhat_in if given (ACCUMULATED)."""
+        x = _f64(x).reshape(-1)
+        M = x.size // 3
+        ng = [v + 2 * self.m for v in self.no]
+        gpad = np.zeros(ng, np.complex128)
+        fl = 0
+        fa = ga = None
+        if f is not None:
+            fa = _c128(f).reshape(-1)
+            fl |= PNFFT_COMPUTE_F
+        if grad is not None:
+            ga = _c128(grad).reshape(-1)
+            fl |= PNFFT_COMPUTE_GRAD_F
+        lib().orc_B_adj(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
+                        _dp(x), _dp(fa.view(np.float64)) if fa is not None else None,
+                        _dp(ga.view(np.float64)) if ga is not None else None, 0,
+                        _dp(gpad.view(np.float64)), fl)
+        g2 = self.ghost_reduce(gpad)
+        g1 = self.FH(g2)
+        fhat = np.zeros(int(np.prod(self.N)), np.complex128) if fhat_in is None else _c128(fhat_in).reshape(-1).copy()
+        self.DT_accumulate(g1, fhat)
+        return fhat
+
+
+def sort_nodes(n, m, x):
+    """Reference node sort (kernel/ndft-parallel.c:2121-2159): returns (keys, perm)."""
+    x = _f64(x).reshape(-1)
+    M = x.size // 3
+    out = np.zeros(2 * max(M, 1), np.int64)
+    lib().orc_sort_nodes(_lp(_longs(n)), int(m), M, _dp(x), _lp(out))
+    out = out[: 2 * M].reshape(M, 2)
+    return out[:, 0].copy(), out[:, 1].copy()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a).reshape(-1)
+    b = np.asarray(b).reshape(-1)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))

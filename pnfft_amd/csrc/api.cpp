@@ -1,0 +1,1027 @@
+// api.cpp -- the C interface (include/pnfft.h) on one MI355X.
+//
+// Plan and node objects mirror plan_s / nodes_s of the reference
+// (kernel/ipnfft.h:160-256): the same parameters, the same ownership rules for
+// f_hat / x / f / grad_f, and the same stage order in trafo / adj
+// (api/api-basic.c:170-378).  The arrays behind the getters may live in host
+// memory (the reference's model) or in device memory (set_* with a device
+// pointer, or init_nodes_device); host arrays are staged through device
+// mirrors on every call, device arrays are used in place.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "device_common.hpp"
+#include "fft.hpp"
+#include "host_math.hpp"
+#include "ndft.hpp"
+#include "nfft_b.hpp"
+#include "window_math.hpp"
+
+// flag values (api/pnfft.h:302-390)
+namespace fl {
+constexpr unsigned PRE_PHI_HAT = 1u << 0;
+constexpr unsigned MALLOC_F_HAT = 1u << 6;
+constexpr unsigned INTERLACED = 1u << 8;
+constexpr unsigned TRANSPOSED_F_HAT = 1u << 11;
+constexpr unsigned DIFF_IK = 1u << 12;
+constexpr unsigned WINDOW_BESSEL_I0 = 1u << 16;
+constexpr unsigned USE_FK_GAUSSIAN_T = 1u << 17;
+constexpr unsigned REAL_F = 1u << 4;
+constexpr unsigned PRE_GRAD_PSI = 1u << 2;
+constexpr unsigned PRE_HESSIAN_PSI = 1u << 3;
+constexpr unsigned PRE_PSI = 1u << 1;
+constexpr unsigned MALLOC_X = 1u << 0;
+constexpr unsigned MALLOC_F = 1u << 1;
+constexpr unsigned MALLOC_GRAD_F = 1u << 2;
+constexpr unsigned MALLOC_HESSIAN_F = 1u << 3;
+constexpr unsigned COMPUTE_F = 1u << 0;
+constexpr unsigned COMPUTE_GRAD_F = 1u << 1;
+constexpr unsigned COMPUTE_HESSIAN_F = 1u << 2;
+constexpr unsigned COMPUTE_DIRECT = 1u << 3;
+constexpr unsigned COMPUTE_ACCUMULATED = 1u << 4;
+constexpr unsigned OMIT_DECONV = 1u << 5;
+constexpr unsigned OMIT_FFT = 1u << 6;
+constexpr unsigned OMIT_CONV = 1u << 7;
+}  // namespace fl
+
+enum TimerSlot {
+  T_ITER = 0, T_WHOLE = 1, T_LOOP_B = 2, T_SORT = 3, T_GCELLS = 4, T_B = 5, T_F = 6, T_D = 7, T_SHIFT_IN = 8,
+  T_SHIFT_OUT = 9, T_LEN = 10
+};
+
+namespace pnfft_amd {
+
+// ---------------------------------------------------------------------------
+// errors and memory registry
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static void set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  fprintf(stderr, "!!! Error in PNFFT: %s !!!\n", buf);
+}
+
+enum MemKind : int { MEM_PINNED = 1, MEM_DEVICE = 2, MEM_HEAP = 3 };
+static std::mutex g_mem_mu;
+static std::unordered_map<void*, int>& mem_registry() {
+  static std::unordered_map<void*, int> m;
+  return m;
+}
+
+static void* host_alloc(size_t bytes) {
+  if (bytes == 0) return nullptr;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    p = aligned_alloc(64, (bytes + 63) / 64 * 64);
+    if (!p) return nullptr;
+    std::lock_guard<std::mutex> g(g_mem_mu);
+    mem_registry()[p] = MEM_HEAP;
+    return p;
+  }
+  std::lock_guard<std::mutex> g(g_mem_mu);
+  mem_registry()[p] = MEM_PINNED;
+  return p;
+}
+
+static void* device_alloc(size_t bytes) {
+  if (bytes == 0) return nullptr;
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device allocation of %zu bytes failed", bytes);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(g_mem_mu);
+  mem_registry()[p] = MEM_DEVICE;
+  return p;
+}
+
+static void mem_free(void* p) {
+  if (!p) return;
+  int kind = 0;
+  {
+    std::lock_guard<std::mutex> g(g_mem_mu);
+    auto it = mem_registry().find(p);
+    if (it != mem_registry().end()) {
+      kind = it->second;
+      mem_registry().erase(it);
+    }
+  }
+  if (kind == MEM_PINNED) (void)hipHostFree(p);
+  else if (kind == MEM_DEVICE) (void)hipFree(p);
+  else if (kind == MEM_HEAP) free(p);
+  else set_error("free of a pointer that was not allocated by the library");
+}
+
+// internal device buffers (not registered: freed directly)
+template <class T>
+static T* dalloc(size_t count) {
+  if (count == 0) return nullptr;
+  void* p = nullptr;
+  if (hipMalloc(&p, sizeof(T) * count) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return static_cast<T*>(p);
+}
+template <class T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+static bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice;
+}
+
+// ---------------------------------------------------------------------------
+// plan and nodes
+// ---------------------------------------------------------------------------
+struct EventTimer {
+  hipEvent_t ev[8] = {};
+  bool ok = false;
+  void init() {
+    ok = true;
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) ok = false;
+  }
+  void release() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  void rec(int i, hipStream_t s) {
+    if (ok) (void)hipEventRecord(ev[i], s);
+  }
+  double ms(int a, int b) {
+    float t = 0;
+    if (!ok || hipEventElapsedTime(&t, ev[a], ev[b]) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0.0;
+    }
+    return t;
+  }
+};
+
+template <class R>
+struct PlanT {
+  int d = 3;
+  long N[3] = {0, 0, 0}, n[3] = {0, 0, 0}, no[3] = {0, 0, 0};
+  int m = 6;
+  double x_max[3] = {0.5, 0.5, 0.5};
+  double sigma[3] = {2, 2, 2};
+  double b[3] = {0, 0, 0};
+  unsigned pnfft_flags = 0, pfft_flags = 0;
+  int win = WIN_KB;
+  long N_total = 0, n_total = 0;
+
+  void* f_hat = nullptr;  // user-visible pointer (host or device)
+  bool own_f_hat = false;
+
+  hipStream_t stream = nullptr;
+  Fft3d<R> fft;
+  Cplx<R>* d_fhat = nullptr;
+  Cplx<R>* d_grid = nullptr;
+  Cplx<R>* d_buf1 = nullptr;
+  Cplx<R>* d_buf2 = nullptr;
+  double* d_tab = nullptr;
+  std::vector<double> h_tab;
+  GridGeom geom{};
+  TileLayout tiles{};
+  EventTimer evt;
+  double timer_trafo[T_LEN] = {0};
+  double timer_adj[T_LEN] = {0};
+};
+
+template <class R>
+struct NodesT {
+  long M = 0;
+  R* x = nullptr;
+  R* f = nullptr;
+  R* grad_f = nullptr;
+  R* hessian_f = nullptr;
+  bool device_arrays = false;  // created by init_nodes_device
+  unsigned precompute_flags = 0;
+
+  // device mirrors of host arrays
+  R* d_x = nullptr;
+  Cplx<R>* d_f = nullptr;
+  Cplx<R>* d_grad = nullptr;
+  R* d_gx = nullptr;
+
+  // binning
+  int* d_perm = nullptr;
+  unsigned* d_keys = nullptr;
+  int* d_tile_start = nullptr;
+  long tile_cap = 0;
+  SortWork sw;
+  bool bins_valid = false;
+  GridGeom bin_geom{};
+  TileLayout bin_tiles{};
+};
+
+template <class R>
+static void tables_compute(PlanT<R>* P) {
+  P->h_tab.resize(P->N[0] + P->N[1] + P->N[2]);
+  long off = 0;
+  for (int t = 0; t < 3; t++)
+    for (long i = 0; i < P->N[t]; i++)
+      P->h_tab[off++] = host::inv_phi_hat(P->win, i - P->N[t] / 2, P->n[t], P->b[t], P->m);
+  if (P->d_tab)
+    (void)hipMemcpy(P->d_tab, P->h_tab.data(), sizeof(double) * P->h_tab.size(), hipMemcpyHostToDevice);
+}
+
+// Tile layout for binning: tiles of 8 cells per axis (fewer if the grid is small).
+static TileLayout choose_tiles(const long* n) {
+  TileLayout T{};
+  for (int t = 0; t < 3; t++) {
+    int ts = 8;
+    while (ts > 1 && n[t] % ts) ts >>= 1;
+    T.ts[t] = ts;
+    T.ntile[t] = (int)((n[t] + ts - 1) / ts);
+  }
+  return T;
+}
+
+template <class R>
+static void plan_destroy(PlanT<R>* P) {
+  if (!P) return;
+  if (P->stream) (void)hipStreamSynchronize(P->stream);
+  P->fft.release();
+  dfree(P->d_fhat);
+  dfree(P->d_grid);
+  dfree(P->d_buf1);
+  dfree(P->d_buf2);
+  dfree(P->d_tab);
+  P->evt.release();
+  if (P->stream) (void)hipStreamDestroy(P->stream);
+  delete P;
+}
+
+// api/api-guru.c:110-164 + kernel/ndft-parallel.c:869-1069
+template <class R>
+static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,
+                             unsigned pnfft_flags, unsigned pfft_flags, bool c2r) {
+  if (d != 3) {
+    set_error("d != 3 not yet implemented");
+    return nullptr;
+  }
+  if (c2r) {
+    set_error("c2r plans are not supported by this build");
+    return nullptr;
+  }
+  const unsigned unsupported = fl::INTERLACED | fl::DIFF_IK | fl::TRANSPOSED_F_HAT | fl::WINDOW_BESSEL_I0 |
+                               fl::USE_FK_GAUSSIAN_T;
+  if (pnfft_flags & unsupported) {
+    set_error("plan flags 0x%x (interlacing, ik-differentiation, transposed f_hat, Bessel-I0 or truncated "
+              "Gaussian window) are not supported by this build",
+              pnfft_flags & unsupported);
+    return nullptr;
+  }
+  if (m < 1 || m > kMaxM) {
+    set_error("window cutoff m=%d outside [1, %d]", m, kMaxM);
+    return nullptr;
+  }
+  for (int t = 0; t < 3; t++) {
+    if (N[t] < 1 || n[t] < N[t]) {
+      set_error("need 1 <= N[%d]=%td <= n[%d]=%td", t, N[t], t, n[t]);
+      return nullptr;
+    }
+    if (!fft_length_supported((int)n[t], sizeof(R) == 8)) {
+      set_error("oversampled length n[%d]=%td must be a power of two <= %d", t, n[t], sizeof(R) == 8 ? 4096 : 8192);
+      return nullptr;
+    }
+    if (n[t] < 2 * m + 1) {
+      set_error("oversampled length n[%d]=%td is shorter than the stencil 2m+1=%d", t, n[t], 2 * m + 1);
+      return nullptr;
+    }
+  }
+  // precomputation implications of api/api-guru.c:152-155
+  if (pnfft_flags & fl::PRE_HESSIAN_PSI) pnfft_flags |= fl::PRE_GRAD_PSI;
+  if (pnfft_flags & fl::PRE_GRAD_PSI) pnfft_flags |= fl::PRE_PSI;
+
+  auto* P = new PlanT<R>();
+  P->m = m;
+  P->pnfft_flags = pnfft_flags;
+  P->pfft_flags = pfft_flags;
+  P->win = host::window_from_flags(pnfft_flags);
+  P->N_total = P->n_total = 1;
+  for (int t = 0; t < 3; t++) {
+    P->N[t] = N[t];
+    P->n[t] = n[t];
+    P->x_max[t] = (double)x_max[t];
+    // fft_output_size, api/api-guru.c:195-218
+    const long c = (long)std::lrint(std::floor((double)n[t] * (double)x_max[t]));
+    P->no[t] = std::min<long>(n[t], 2 * (c + m + 2));
+    P->sigma[t] = (double)n[t] / (double)N[t];
+    P->b[t] = host::window_b(P->win, m, P->sigma[t]);
+    P->N_total *= N[t];
+    P->n_total *= n[t];
+  }
+  P->geom.m = m;
+  P->geom.win = P->win;
+  for (int t = 0; t < 3; t++) {
+    P->geom.n[t] = (int)n[t];
+    P->geom.b[t] = P->b[t];
+  }
+  P->tiles = choose_tiles(P->n);
+
+  bool ok = hipStreamCreate(&P->stream) == hipSuccess;
+  int Ni[3] = {(int)N[0], (int)N[1], (int)N[2]}, ni[3] = {(int)n[0], (int)n[1], (int)n[2]};
+  if (ok) ok = P->fft.init(Ni, ni) == kOk;
+  if (ok) {
+    P->d_fhat = dalloc<Cplx<R>>(P->N_total);
+    P->d_grid = dalloc<Cplx<R>>(P->n_total);
+    P->d_buf1 = dalloc<Cplx<R>>(P->fft.buf1_elems());
+    P->d_buf2 = dalloc<Cplx<R>>(P->fft.buf2_elems());
+    P->d_tab = dalloc<double>(N[0] + N[1] + N[2]);
+    ok = P->d_fhat && P->d_grid && P->d_buf1 && P->d_buf2 && P->d_tab;
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    set_error("device setup failed (memory or HIP runtime)");
+    plan_destroy(P);
+    return nullptr;
+  }
+  P->evt.init();
+  tables_compute(P);
+  if (pnfft_flags & fl::MALLOC_F_HAT) {
+    P->f_hat = host_alloc(sizeof(Cplx<R>) * P->N_total);
+    P->own_f_hat = true;
+  }
+  return P;
+}
+
+template <class R>
+static void nodes_release_device(NodesT<R>* D) {
+  dfree(D->d_x);
+  dfree(D->d_f);
+  dfree(D->d_grad);
+  dfree(D->d_gx);
+  dfree(D->d_perm);
+  dfree(D->d_keys);
+  dfree(D->d_tile_start);
+  dfree(D->sw.keys[0]);
+  dfree(D->sw.keys[1]);
+  dfree(D->sw.vals[0]);
+  dfree(D->sw.vals[1]);
+  dfree(D->sw.hist);
+  dfree(D->sw.partial);
+  D->sw.cap = D->sw.hist_cap = D->sw.partial_cap = 0;
+  D->tile_cap = 0;
+  D->bins_valid = false;
+}
+
+template <class R>
+static NodesT<R>* nodes_create(long M, unsigned malloc_flags, bool device) {
+  auto* D = new NodesT<R>();
+  D->M = M;
+  D->device_arrays = device;
+  auto alloc = [&](size_t bytes) -> R* {
+    if (M <= 0) return nullptr;  // kernel/ndft-parallel.c:1491
+    return static_cast<R*>(device ? device_alloc(bytes) : host_alloc(bytes));
+  };
+  if (malloc_flags & fl::MALLOC_X) D->x = alloc(sizeof(R) * 3 * M);
+  if (malloc_flags & fl::MALLOC_F) D->f = alloc(sizeof(R) * 2 * M);
+  if (malloc_flags & fl::MALLOC_GRAD_F) D->grad_f = alloc(sizeof(R) * 6 * M);
+  if (malloc_flags & fl::MALLOC_HESSIAN_F) D->hessian_f = alloc(sizeof(R) * 12 * M);
+  return D;
+}
+
+// Resolve an array for the kernels: device pointers are used in place, host
+// arrays get a device mirror (allocated on demand, uploaded when `upload`).
+template <class T>
+static T* stage_in(void* user, T*& mirror, size_t count, bool upload, hipStream_t s, bool* is_host) {
+  *is_host = false;
+  if (!user) return nullptr;
+  if (is_device_ptr(user)) return static_cast<T*>(user);
+  *is_host = true;
+  if (!mirror) mirror = dalloc<T>(count);
+  if (!mirror) return nullptr;
+  if (upload) (void)hipMemcpyAsync(mirror, user, sizeof(T) * count, hipMemcpyHostToDevice, s);
+  return mirror;
+}
+
+template <class R>
+static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
+  const long M = D->M;
+  if (D->sw.cap < M) {
+    dfree(D->sw.keys[0]);
+    dfree(D->sw.keys[1]);
+    dfree(D->sw.vals[0]);
+    dfree(D->sw.vals[1]);
+    dfree(D->d_perm);
+    dfree(D->d_keys);
+    D->sw.keys[0] = dalloc<unsigned>(M);
+    D->sw.keys[1] = dalloc<unsigned>(M);
+    D->sw.vals[0] = dalloc<int>(M);
+    D->sw.vals[1] = dalloc<int>(M);
+    D->d_perm = dalloc<int>(M);
+    D->d_keys = dalloc<unsigned>(M);
+    if (!D->sw.keys[0] || !D->sw.keys[1] || !D->sw.vals[0] || !D->sw.vals[1] || !D->d_perm || !D->d_keys)
+      return false;
+    D->sw.cap = M;
+  }
+  const long nblk = (M + 4095) / 4096;
+  const long hist = 256 * nblk;
+  if (D->sw.hist_cap < hist) {
+    dfree(D->sw.hist);
+    D->sw.hist = dalloc<unsigned>(hist);
+    if (!D->sw.hist) return false;
+    D->sw.hist_cap = hist;
+  }
+  const long parts = (hist + 4095) / 4096;
+  if (D->sw.partial_cap < parts) {
+    dfree(D->sw.partial);
+    D->sw.partial = dalloc<unsigned>(parts);
+    if (!D->sw.partial) return false;
+    D->sw.partial_cap = parts;
+  }
+  if (D->tile_cap < T.ntiles() + 1) {
+    dfree(D->d_tile_start);
+    D->d_tile_start = dalloc<int>(T.ntiles() + 1);
+    if (!D->d_tile_start) return false;
+    D->tile_cap = T.ntiles() + 1;
+  }
+  return true;
+}
+
+static bool same_geom(const GridGeom& a, const GridGeom& b) {
+  return a.n[0] == b.n[0] && a.n[1] == b.n[1] && a.n[2] == b.n[2] && a.m == b.m;
+}
+
+// Bin the nodes of D for plan P (on P's stream).
+template <class R>
+static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx) {
+  if (!ensure_sort_capacity(D, P->tiles)) {
+    set_error("device allocation for node binning failed");
+    return kHipError;
+  }
+  int st = bin_nodes<R>(dx, D->M, P->geom, P->tiles, D->sw, D->d_perm, D->d_keys, D->d_tile_start, P->stream);
+  if (st != kOk) {
+    set_error("node binning failed (%d)", st);
+    return st;
+  }
+  D->bin_geom = P->geom;
+  D->bin_tiles = P->tiles;
+  D->bins_valid = true;
+  return kOk;
+}
+
+template <class R>
+static void finish_timers(PlanT<R>* P, double* timer, bool had_sort, bool had_f, bool had_b) {
+  if (had_sort) timer[T_SORT] += 1e-3 * P->evt.ms(0, 1);
+  if (had_f) {
+    const double t = 1e-3 * P->evt.ms(2, 3);
+    timer[T_F] += t;
+  }
+  if (had_b) {
+    const double t = 1e-3 * P->evt.ms(4, 5);
+    timer[T_B] += t;
+    timer[T_LOOP_B] += t;
+  }
+}
+
+// api/api-basic.c:199-244
+template <class R>
+static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
+  if (!P) return;
+  if (!D && !(cf & fl::OMIT_CONV)) return;
+  auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = P->stream;
+  const long M = D ? D->M : 0;
+  const bool want_f = cf & fl::COMPUTE_F, want_g = cf & fl::COMPUTE_GRAD_F;
+  if (cf & fl::COMPUTE_HESSIAN_F) set_error("PNFFT_COMPUTE_HESSIAN_F is not supported by this build");
+  const bool accumulate = (cf & fl::COMPUTE_ACCUMULATED) != 0;
+  bool host_x = false, host_f = false, host_g = false, host_fh = false;
+
+  R* dx = D ? stage_in<R>(D->x, D->d_x, 3 * M, true, s, &host_x) : nullptr;
+  Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, true, s, &host_fh);
+  const bool direct = cf & fl::COMPUTE_DIRECT;
+  // outputs: uploaded only when accumulating (the NDFT overwrites them regardless, :400-414)
+  Cplx<R>* df = nullptr;
+  Cplx<R>* dg = nullptr;
+  if (D && want_f) df = stage_in<Cplx<R>>(D->f, D->d_f, M, accumulate && !direct, s, &host_f);
+  if (D && want_g) dg = stage_in<Cplx<R>>(D->grad_f, D->d_grad, 3 * M, accumulate && !direct, s, &host_g);
+  if ((want_f && M > 0 && !df) || (want_g && M > 0 && !dg) || (M > 0 && !dx && !(cf & fl::OMIT_CONV)) || !dfh) {
+    set_error("trafo: missing node or f_hat arrays");
+    return;
+  }
+
+  bool had_sort = false, had_f = false, had_b = false;
+  if (direct) {
+    NdftArgs<R> A{};
+    for (int t = 0; t < 3; t++) A.N[t] = (int)P->N[t];
+    A.M = M;
+    A.x = dx;
+    A.fhat = dfh;
+    A.f = df;
+    A.grad = dg;
+    A.want_f = want_f;
+    A.want_grad = want_g;
+    if (ndft_trafo<R>(A, s) != kOk) set_error("NDFT trafo launch failed");
+  } else {
+    // D and F (fused): grid <- F D f_hat
+    if (!(cf & fl::OMIT_FFT)) {
+      P->evt.rec(2, s);
+      const double* tab = (cf & fl::OMIT_DECONV) ? nullptr : P->d_tab;
+      if (fft3d_forward<R>(P->fft, dfh, tab, P->d_buf1, P->d_buf2, P->d_grid, s) != kOk)
+        set_error("FFT launch failed");
+      P->evt.rec(3, s);
+      had_f = true;
+    }
+    // B
+    if (!(cf & fl::OMIT_CONV) && M > 0 && (want_f || want_g)) {
+      const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x &&
+                         same_geom(D->bin_geom, P->geom);
+      if (!reuse) {
+        P->evt.rec(0, s);
+        if (bin(P, D, dx) != kOk) return;
+        P->evt.rec(1, s);
+        had_sort = true;
+      }
+      BArgs<R> A{};
+      A.g = P->geom;
+      A.M = M;
+      A.x = dx;
+      A.perm = D->d_perm;
+      A.grid = P->d_grid;
+      A.f = df;
+      A.grad = dg;
+      A.accumulate = accumulate;
+      A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
+      A.tile_start = D->d_tile_start;
+      int mode = (want_f && want_g) ? B_FGRAD : (want_f ? B_F : B_GRAD);
+      P->evt.rec(4, s);
+      if (gather<R>(A, mode, s) != kOk) set_error("gather launch failed");
+      P->evt.rec(5, s);
+      had_b = true;
+    }
+  }
+  if (host_f) (void)hipMemcpyAsync(D->f, df, sizeof(Cplx<R>) * M, hipMemcpyDeviceToHost, s);
+  if (host_g) (void)hipMemcpyAsync(D->grad_f, dg, sizeof(Cplx<R>) * 3 * M, hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error("trafo: HIP error %s", hipGetErrorString(hipGetLastError()));
+    return;
+  }
+  finish_timers(P, P->timer_trafo, had_sort, had_f, had_b);
+  P->timer_trafo[T_ITER] += 1;
+  P->timer_trafo[T_WHOLE] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// api/api-basic.c:344-378
+template <class R>
+static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
+  if (!P) return;
+  if (!D && !(cf & fl::OMIT_CONV)) return;
+  auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = P->stream;
+  const long M = D ? D->M : 0;
+  const bool use_f = cf & fl::COMPUTE_F, use_g = cf & fl::COMPUTE_GRAD_F;
+  const bool accumulate = (cf & fl::COMPUTE_ACCUMULATED) != 0;
+  bool host_x = false, host_f = false, host_g = false, host_fh = false;
+  R* dx = D ? stage_in<R>(D->x, D->d_x, 3 * M, true, s, &host_x) : nullptr;
+  Cplx<R>* df = (D && use_f) ? stage_in<Cplx<R>>(D->f, D->d_f, M, true, s, &host_f) : nullptr;
+  Cplx<R>* dg = (D && use_g) ? stage_in<Cplx<R>>(D->grad_f, D->d_grad, 3 * M, true, s, &host_g) : nullptr;
+  Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, accumulate, s, &host_fh);
+  if (!dfh || (M > 0 && !(cf & fl::OMIT_CONV) && (!dx || (use_f && !df) || (use_g && !dg)))) {
+    set_error("adj: missing node or f_hat arrays");
+    return;
+  }
+  // zero f_hat unless accumulating (:355-356)
+  if (!accumulate) fill_zero<R>(dfh, P->N_total, s);
+
+  bool had_sort = false, had_f = false, had_b = false;
+  if (cf & fl::COMPUTE_DIRECT) {
+    NdftArgs<R> A{};
+    for (int t = 0; t < 3; t++) A.N[t] = (int)P->N[t];
+    A.M = M;
+    A.x = dx;
+    A.fhat = dfh;
+    A.f = df;
+    A.grad = dg;
+    A.want_f = use_f;
+    A.want_grad = use_g;
+    if (ndft_adj<R>(A, s) != kOk) set_error("NDFT adj launch failed");
+  } else {
+    if (!(cf & fl::OMIT_CONV)) {
+      // B^T into the zeroed grid (:2629-2635)
+      fill_zero<R>(P->d_grid, P->n_total, s);
+      if (M > 0 && (use_f || use_g)) {
+        const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x &&
+                           same_geom(D->bin_geom, P->geom);
+        if (!reuse) {
+          P->evt.rec(0, s);
+          if (bin(P, D, dx) != kOk) return;
+          P->evt.rec(1, s);
+          had_sort = true;
+        }
+        BArgs<R> A{};
+        A.g = P->geom;
+        A.M = M;
+        A.x = dx;
+        A.perm = D->d_perm;
+        A.grid = P->d_grid;
+        A.f = df;
+        A.grad = dg;
+        A.tile_start = D->d_tile_start;
+        const int mode = (use_f && use_g) ? B_FGRAD : (use_f ? B_F : B_GRAD);
+        P->evt.rec(4, s);
+        if (scatter<R>(A, mode, s) != kOk) set_error("scatter launch failed");
+        P->evt.rec(5, s);
+        had_b = true;
+      }
+    }
+    // F^H and D^T (fused), f_hat += D^T F^H grid
+    if (!(cf & fl::OMIT_FFT) && !(cf & fl::OMIT_DECONV)) {
+      P->evt.rec(2, s);
+      if (fft3d_backward<R>(P->fft, P->d_grid, P->d_tab, P->d_buf1, P->d_buf2, dfh, 1, s) != kOk)
+        set_error("FFT launch failed");
+      P->evt.rec(3, s);
+      had_f = true;
+    }
+  }
+  if (host_fh) (void)hipMemcpyAsync(P->f_hat, dfh, sizeof(Cplx<R>) * P->N_total, hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error("adj: HIP error %s", hipGetErrorString(hipGetLastError()));
+    return;
+  }
+  finish_timers(P, P->timer_adj, had_sort, had_f, had_b);
+  P->timer_adj[T_ITER] += 1;
+  P->timer_adj[T_WHOLE] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// x-adjoint of f = trafo(f_hat): grad_x[3j+t] = Re(conj(w_j) df_j/dx_t)
+template <class R>
+static void vjp_impl(PlanT<R>* P, NodesT<R>* D, const Cplx<R>* w, R* gx) {
+  if (!P || !D) return;
+  hipStream_t s = P->stream;
+  const long M = D->M;
+  if (M == 0) return;
+  bool host_x = false, host_fh = false, host_w = false, host_gx = false;
+  R* dx = stage_in<R>(D->x, D->d_x, 3 * M, true, s, &host_x);
+  Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, true, s, &host_fh);
+  Cplx<R>* dw_m = nullptr;
+  Cplx<R>* dw = stage_in<Cplx<R>>((void*)w, D->d_f, M, true, s, &host_w);
+  (void)dw_m;
+  R* dgx = stage_in<R>(gx, D->d_gx, 3 * M, false, s, &host_gx);
+  if (!dx || !dfh || !dw || !dgx) {
+    set_error("trafo_grad_x_vjp: missing arrays");
+    return;
+  }
+  if (fft3d_forward<R>(P->fft, dfh, P->d_tab, P->d_buf1, P->d_buf2, P->d_grid, s) != kOk) {
+    set_error("FFT launch failed");
+    return;
+  }
+  const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x &&
+                     same_geom(D->bin_geom, P->geom);
+  if (!reuse && bin(P, D, dx) != kOk) return;
+  BArgs<R> A{};
+  A.g = P->geom;
+  A.M = M;
+  A.x = dx;
+  A.perm = D->d_perm;
+  A.grid = P->d_grid;
+  A.w = dw;
+  A.gx = dgx;
+  A.tile_start = D->d_tile_start;
+  if (gather<R>(A, B_VJP, s) != kOk) set_error("gather launch failed");
+  if (host_gx) (void)hipMemcpyAsync(gx, dgx, sizeof(R) * 3 * M, hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) set_error("vjp: HIP error %s", hipGetErrorString(hipGetLastError()));
+}
+
+// kernel/ndft-parallel.c:1144-1250: here the node precomputation is the
+// binning of the nodes (sort permutation and tile bounds) for the plan's grid.
+template <class R>
+static void precompute_impl(PlanT<R>* P, NodesT<R>* D, unsigned flags) {
+  if (!P || !D) return;
+  D->precompute_flags = flags;
+  if (!(flags & fl::PRE_PSI) || D->M == 0) return;
+  bool host_x = false;
+  R* dx = stage_in<R>(D->x, D->d_x, 3 * D->M, true, P->stream, &host_x);
+  if (!dx) return;
+  if (bin(P, D, dx) != kOk) return;
+  (void)hipStreamSynchronize(P->stream);
+  // host x is re-uploaded on every call, so its bins are not reused
+  if (host_x) D->bins_valid = false;
+}
+
+template <class R>
+static int node_bins_impl(PlanT<R>* P, NodesT<R>* D, int* order, unsigned* keys) {
+  if (!P || !D) return -1;
+  if (D->M == 0) return 0;
+  bool host_x = false;
+  R* dx = stage_in<R>(D->x, D->d_x, 3 * D->M, true, P->stream, &host_x);
+  if (!dx) return -1;
+  if (!(D->bins_valid && !host_x && same_geom(D->bin_geom, P->geom)))
+    if (bin(P, D, dx) != kOk) return -1;
+  if (order) (void)hipMemcpyAsync(order, D->d_perm, sizeof(int) * D->M, hipMemcpyDefault, P->stream);
+  if (keys) (void)hipMemcpyAsync(keys, D->d_keys, sizeof(unsigned) * D->M, hipMemcpyDefault, P->stream);
+  if (hipStreamSynchronize(P->stream) != hipSuccess) return -1;
+  if (host_x) D->bins_valid = false;
+  return 0;
+}
+
+// glibc-rand based test data (api/api-basic.c:681-704, api/api-adv.c:35-86)
+template <class R>
+static void init_x_3d_adv(const R* lo, const R* up, const R* x_max, long M, R* x) {
+  for (long j = 0; j < M; j++)
+    for (int t = 0; t < 3; t++) {
+      R v;
+      do {
+        v = (R)((double)rand() / RAND_MAX) * (up[t] - lo[t]) + lo[t];
+      } while ((v < lo[t]) || (v >= up[t]) || (v < -x_max[t]) || (v >= x_max[t]));
+      x[3 * j + t] = v;
+    }
+}
+
+}  // namespace pnfft_amd
+
+using namespace pnfft_amd;
+
+// ---------------------------------------------------------------------------
+// C interface, both precisions
+// ---------------------------------------------------------------------------
+#define PNFFT_DEFINE_IMPL(PREFIX, R)                                                                       \
+  typedef PlanT<R>* PREFIX##plan_t;                                                                         \
+  typedef NodesT<R>* PREFIX##nodes_t;                                                                       \
+  extern "C" {                                                                                             \
+  int PREFIX##create_procmesh_2d(MPI_Comm comm, int np0, int np1, MPI_Comm* cc) {                           \
+    (void)np0; (void)np1; if (cc) *cc = comm; return 0;                                                     \
+  }                                                                                                        \
+  int PREFIX##create_procmesh(int rnk, MPI_Comm comm, const int* np, MPI_Comm* cc) {                        \
+    (void)rnk; (void)np; if (cc) *cc = comm; return 0;                                                      \
+  }                                                                                                        \
+  static void PREFIX##local_sizes(const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,           \
+                                  ptrdiff_t* lN, ptrdiff_t* lNs, R* lo, R* up) {                            \
+    for (int t = 0; t < 3; t++) {                                                                          \
+      lN[t] = N[t]; lNs[t] = -N[t] / 2;                                                                     \
+      const long c = (long)std::lrint(std::floor((double)n[t] * (double)x_max[t]));                         \
+      const long no = std::min<long>(n[t], 2 * (c + m + 2));                                                \
+      /* node_borders, kernel/ndft-parallel.c:734-775 */                                                    \
+      double l = (double)(-no / 2) / (double)n[t], u = (double)(-no / 2 + no) / (double)n[t];              \
+      if (l < -x_max[t]) l = -x_max[t]; if (l > x_max[t]) l = x_max[t];                                     \
+      if (u < -x_max[t]) u = -x_max[t]; if (u > x_max[t]) u = x_max[t];                                     \
+      lo[t] = (R)l; up[t] = (R)u;                                                                           \
+    }                                                                                                      \
+  }                                                                                                        \
+  void PREFIX##local_size_guru(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,       \
+                               MPI_Comm cc, unsigned pf, ptrdiff_t* lN, ptrdiff_t* lNs, R* lo, R* up) {     \
+    (void)cc; (void)pf;                                                                                     \
+    if (d != 3) { set_error("d != 3 not yet implemented"); return; }                                        \
+    PREFIX##local_sizes(N, n, x_max, m, lN, lNs, lo, up);                                                   \
+  }                                                                                                        \
+  void PREFIX##local_size_guru_c2r(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,   \
+                                   MPI_Comm cc, unsigned pf, ptrdiff_t* lN, ptrdiff_t* lNs, R* lo, R* up) { \
+    PREFIX##local_size_guru(d, N, n, x_max, m, cc, pf, lN, lNs, lo, up);                                    \
+    if (d == 3) lN[2] = N[2] / 2 + 1;                                                                       \
+  }                                                                                                        \
+  void PREFIX##local_size_adv(int d, const ptrdiff_t* N, MPI_Comm cc, unsigned pf, ptrdiff_t* lN,          \
+                              ptrdiff_t* lNs, R* lo, R* up) {                                               \
+    ptrdiff_t n[3] = {2 * N[0], 2 * N[1], 2 * N[2]}; R xm[3] = {R(0.5), R(0.5), R(0.5)};                    \
+    PREFIX##local_size_guru(d, N, n, xm, 6, cc, pf, lN, lNs, lo, up);                                       \
+  }                                                                                                        \
+  void PREFIX##local_size_adv_c2r(int d, const ptrdiff_t* N, MPI_Comm cc, unsigned pf, ptrdiff_t* lN,      \
+                                  ptrdiff_t* lNs, R* lo, R* up) {                                           \
+    ptrdiff_t n[3] = {2 * N[0], 2 * N[1], 2 * N[2]}; R xm[3] = {R(0.5), R(0.5), R(0.5)};                    \
+    PREFIX##local_size_guru_c2r(d, N, n, xm, 6, cc, pf, lN, lNs, lo, up);                                   \
+  }                                                                                                        \
+  void PREFIX##local_size_3d(const ptrdiff_t* N, MPI_Comm cc, unsigned pf, ptrdiff_t* lN, ptrdiff_t* lNs,  \
+                             R* lo, R* up) { PREFIX##local_size_adv(3, N, cc, pf, lN, lNs, lo, up); }        \
+  void PREFIX##local_size_3d_c2r(const ptrdiff_t* N, MPI_Comm cc, unsigned pf, ptrdiff_t* lN,              \
+                                 ptrdiff_t* lNs, R* lo, R* up) {                                            \
+    PREFIX##local_size_adv_c2r(3, N, cc, pf, lN, lNs, lo, up);                                              \
+  }                                                                                                        \
+  void* PREFIX##init_guru(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,            \
+                          unsigned pnfft_flags, unsigned pfft_flags, MPI_Comm cc) {                         \
+    (void)cc; return plan_create<R>(d, N, n, x_max, m, pnfft_flags, pfft_flags, false);                      \
+  }                                                                                                        \
+  void* PREFIX##init_guru_c2r(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,        \
+                              unsigned pnfft_flags, unsigned pfft_flags, MPI_Comm cc) {                     \
+    (void)cc; return plan_create<R>(d, N, n, x_max, m, pnfft_flags, pfft_flags, true);                       \
+  }                                                                                                        \
+  void* PREFIX##init_adv(int d, const ptrdiff_t* N, unsigned pnfft_flags, unsigned pfft_flags,             \
+                         MPI_Comm cc) {                                                                     \
+    if (d != 3) { set_error("d != 3 not yet implemented"); return nullptr; }                                \
+    ptrdiff_t n[3] = {2 * N[0], 2 * N[1], 2 * N[2]}; R xm[3] = {R(0.5), R(0.5), R(0.5)};                    \
+    return PREFIX##init_guru(d, N, n, xm, 6, pnfft_flags, pfft_flags, cc);                                  \
+  }                                                                                                        \
+  void* PREFIX##init_adv_c2r(int d, const ptrdiff_t* N, unsigned pnfft_flags, unsigned pfft_flags,         \
+                             MPI_Comm cc) {                                                                 \
+    if (d != 3) { set_error("d != 3 not yet implemented"); return nullptr; }                                \
+    ptrdiff_t n[3] = {2 * N[0], 2 * N[1], 2 * N[2]}; R xm[3] = {R(0.5), R(0.5), R(0.5)};                    \
+    return PREFIX##init_guru_c2r(d, N, n, xm, 6, pnfft_flags, pfft_flags, cc);                              \
+  }                                                                                                        \
+  void* PREFIX##init_3d(const ptrdiff_t* N, MPI_Comm cc) {                                                  \
+    return PREFIX##init_adv(3, N, fl::MALLOC_F_HAT, 0, cc);                                                 \
+  }                                                                                                        \
+  void* PREFIX##init_3d_c2r(const ptrdiff_t* N, MPI_Comm cc) {                                              \
+    return PREFIX##init_adv_c2r(3, N, fl::MALLOC_F_HAT, 0, cc);                                             \
+  }                                                                                                        \
+  void* PREFIX##init_nodes(ptrdiff_t M, unsigned mf) { return nodes_create<R>(M, mf, false); }               \
+  void* PREFIX##init_nodes_device(ptrdiff_t M, unsigned mf) { return nodes_create<R>(M, mf, true); }         \
+  void PREFIX##free_nodes(void* h, unsigned ff) {                                                           \
+    auto* D = static_cast<NodesT<R>*>(h);                                                                   \
+    if (!D) return;                                                                                        \
+    (void)hipDeviceSynchronize();                                                                           \
+    if (ff & fl::MALLOC_GRAD_F) mem_free(D->grad_f);                                                        \
+    if (ff & fl::MALLOC_HESSIAN_F) mem_free(D->hessian_f);                                                  \
+    if (ff & fl::MALLOC_F) mem_free(D->f);                                                                  \
+    if (ff & fl::MALLOC_X) mem_free(D->x);                                                                  \
+    nodes_release_device(D);                                                                                \
+    delete D;                                                                                               \
+  }                                                                                                        \
+  void PREFIX##precompute_psi(void* p, void* h, unsigned pf) {                                              \
+    precompute_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), pf);                          \
+  }                                                                                                        \
+  void PREFIX##precompute_one_psi(void* p, void* h, unsigned pf) { PREFIX##precompute_psi(p, h, pf); }       \
+  void PREFIX##set_f(void* f, void* h) { if (h) static_cast<NodesT<R>*>(h)->f = (R*)f; }                    \
+  void PREFIX##set_grad_f(void* g, void* h) { if (h) static_cast<NodesT<R>*>(h)->grad_f = (R*)g; }          \
+  void PREFIX##set_hessian_f(void* g, void* h) { if (h) static_cast<NodesT<R>*>(h)->hessian_f = (R*)g; }    \
+  void PREFIX##set_f_real(R* f, void* h) { if (h) static_cast<NodesT<R>*>(h)->f = f; }                      \
+  void PREFIX##set_grad_f_real(R* g, void* h) { if (h) static_cast<NodesT<R>*>(h)->grad_f = g; }            \
+  void PREFIX##set_hessian_f_real(R* g, void* h) { if (h) static_cast<NodesT<R>*>(h)->hessian_f = g; }      \
+  void PREFIX##set_x(R* x, void* h) {                                                                       \
+    if (!h) return;                                                                                        \
+    auto* D = static_cast<NodesT<R>*>(h);                                                                   \
+    D->x = x; D->bins_valid = false;                                                                        \
+  }                                                                                                        \
+  void PREFIX##set_f_hat(void* fh, void* p) { if (p) static_cast<PlanT<R>*>(p)->f_hat = fh; }               \
+  void PREFIX##set_f_hat_real(R* fh, void* p) { if (p) static_cast<PlanT<R>*>(p)->f_hat = fh; }             \
+  void PREFIX##set_b(R b0, R b1, R b2, void* p) {                                                           \
+    auto* P = static_cast<PlanT<R>*>(p); if (!P) return;                                                    \
+    P->b[0] = b0; P->b[1] = b1; P->b[2] = b2;                                                               \
+    for (int t = 0; t < 3; t++) P->geom.b[t] = P->b[t];                                                     \
+    tables_compute(P);                                                                                      \
+  }                                                                                                        \
+  void* PREFIX##get_f(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->f : nullptr; }           \
+  void* PREFIX##get_grad_f(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->grad_f : nullptr; } \
+  void* PREFIX##get_hessian_f(const void* h) {                                                              \
+    return h ? static_cast<const NodesT<R>*>(h)->hessian_f : nullptr;                                       \
+  }                                                                                                        \
+  R* PREFIX##get_f_real(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->f : nullptr; }         \
+  R* PREFIX##get_grad_f_real(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->grad_f : nullptr; } \
+  R* PREFIX##get_hessian_f_real(const void* h) {                                                            \
+    return h ? static_cast<const NodesT<R>*>(h)->hessian_f : nullptr;                                       \
+  }                                                                                                        \
+  R* PREFIX##get_x(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->x : nullptr; }              \
+  void* PREFIX##get_f_hat(const void* p) { return p ? static_cast<const PlanT<R>*>(p)->f_hat : nullptr; }    \
+  R* PREFIX##get_f_hat_real(const void* p) {                                                                \
+    return p ? (R*)static_cast<const PlanT<R>*>(p)->f_hat : nullptr;                                        \
+  }                                                                                                        \
+  int PREFIX##get_d(const void* p) { return static_cast<const PlanT<R>*>(p)->d; }                           \
+  int PREFIX##get_m(const void* p) { return static_cast<const PlanT<R>*>(p)->m; }                           \
+  void PREFIX##get_x_max(const void* p, R* xm) {                                                            \
+    for (int t = 0; t < 3; t++) xm[t] = (R) static_cast<const PlanT<R>*>(p)->x_max[t];                      \
+  }                                                                                                        \
+  void PREFIX##get_N(const void* p, ptrdiff_t* N) {                                                         \
+    for (int t = 0; t < 3; t++) N[t] = static_cast<const PlanT<R>*>(p)->N[t];                               \
+  }                                                                                                        \
+  void PREFIX##get_n(const void* p, ptrdiff_t* n) {                                                         \
+    for (int t = 0; t < 3; t++) n[t] = static_cast<const PlanT<R>*>(p)->n[t];                               \
+  }                                                                                                        \
+  unsigned PREFIX##get_pnfft_flags(const void* p) { return static_cast<const PlanT<R>*>(p)->pnfft_flags; }  \
+  unsigned PREFIX##get_pfft_flags(const void* p) { return static_cast<const PlanT<R>*>(p)->pfft_flags; }    \
+  void PREFIX##get_b(const void* p, R* b0, R* b1, R* b2) {                                                  \
+    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
+    *b0 = (R)P->b[0]; *b1 = (R)P->b[1]; *b2 = (R)P->b[2];                                                   \
+  }                                                                                                        \
+  void PREFIX##finalize(void* p, unsigned ff) {                                                             \
+    auto* P = static_cast<PlanT<R>*>(p);                                                                    \
+    if (!P) return;                                                                                        \
+    if ((ff & fl::MALLOC_F_HAT) && P->f_hat) mem_free(P->f_hat);                                            \
+    plan_destroy(P);                                                                                        \
+  }                                                                                                        \
+  void PREFIX##trafo(void* p, void* h, unsigned cf) {                                                       \
+    trafo_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), cf);                               \
+  }                                                                                                        \
+  void PREFIX##adj(void* p, void* h, unsigned cf) {                                                         \
+    adj_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), cf);                                 \
+  }                                                                                                        \
+  void PREFIX##trafo_grad_x_vjp(void* p, void* h, const void* w, R* gx) {                                   \
+    vjp_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), (const Cplx<R>*)w, gx);              \
+  }                                                                                                        \
+  int PREFIX##node_bins(void* p, void* h, int* order, unsigned* keys) {                                      \
+    return node_bins_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), order, keys);           \
+  }                                                                                                        \
+  void PREFIX##init(void) {}                                                                                \
+  void PREFIX##cleanup(void) {}                                                                             \
+  void* PREFIX##malloc(size_t n) { return host_alloc(n); }                                                  \
+  R* PREFIX##alloc_real(size_t n) { return (R*)host_alloc(sizeof(R) * n); }                                  \
+  void* PREFIX##alloc_complex(size_t n) { return host_alloc(2 * sizeof(R) * n); }                            \
+  void PREFIX##free(void* ptr) { mem_free(ptr); }                                                           \
+  void* PREFIX##malloc_device(size_t n) { return device_alloc(n); }                                         \
+  void PREFIX##init_f_hat_3d(const ptrdiff_t* N, const ptrdiff_t* lN, const ptrdiff_t* lNs, unsigned pf,    \
+                             void* data) {                                                                  \
+    /* deterministic fill; the reference delegates to PFFT's init_input_complex_3d */                      \
+    (void)pf; R* d = (R*)data; long k = 0;                                                                  \
+    for (long k0 = lNs[0]; k0 < lNs[0] + lN[0]; k0++)                                                       \
+      for (long k1 = lNs[1]; k1 < lNs[1] + lN[1]; k1++)                                                     \
+        for (long k2 = lNs[2]; k2 < lNs[2] + lN[2]; k2++, k++) {                                            \
+          const long g = ((k0 + N[0] / 2) * N[1] + (k1 + N[1] / 2)) * N[2] + (k2 + N[2] / 2);               \
+          d[2 * k] = (R)(1000.0 / (2.0 * g + 1.0)); d[2 * k + 1] = (R)(1000.0 / (2.0 * g + 2.0));           \
+        }                                                                                                  \
+  }                                                                                                        \
+  void PREFIX##init_f(ptrdiff_t M, void* data) {                                                            \
+    R* d = (R*)data;                                                                                        \
+    for (long j = 0; j < M; j++) {                                                                          \
+      R re = (R)(100.0 * (R)rand() / RAND_MAX); R im = (R)(100.0 * (R)rand() / RAND_MAX);                   \
+      d[2 * j] = re; d[2 * j + 1] = im;                                                                     \
+    }                                                                                                      \
+  }                                                                                                        \
+  void PREFIX##init_x_3d_adv(const R* lo, const R* up, const R* xm, ptrdiff_t M, R* x) {                    \
+    init_x_3d_adv<R>(lo, up, xm, M, x);                                                                     \
+  }                                                                                                        \
+  void PREFIX##init_x_3d(const R* lo, const R* up, ptrdiff_t M, R* x) {                                     \
+    R xm[3] = {R(0.5), R(0.5), R(0.5)}; init_x_3d_adv<R>(lo, up, xm, M, x);                                  \
+  }                                                                                                        \
+  void PREFIX##zero_f_hat(void* p) {                                                                        \
+    auto* P = static_cast<PlanT<R>*>(p); if (!P || !P->f_hat) return;                                       \
+    if (is_device_ptr(P->f_hat)) {                                                                          \
+      fill_zero<R>((Cplx<R>*)P->f_hat, P->N_total, P->stream); (void)hipStreamSynchronize(P->stream);        \
+    } else memset(P->f_hat, 0, sizeof(Cplx<R>) * P->N_total);                                               \
+  }                                                                                                        \
+  R PREFIX##inv_phi_hat(const void* p, int dim, ptrdiff_t k) {                                               \
+    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
+    return (R)host::inv_phi_hat(P->win, k, P->n[dim], P->b[dim], P->m);                                     \
+  }                                                                                                        \
+  R PREFIX##phi_hat(const void* p, int dim, ptrdiff_t k) {                                                   \
+    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
+    return (R)host::phi_hat(P->win, k, P->n[dim], P->b[dim], P->m);                                         \
+  }                                                                                                        \
+  R PREFIX##psi(const void* p, int dim, R x) {                                                              \
+    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
+    return (R)host::psi(P->win, P->n[dim], P->b[dim], P->m, (double)x);                                     \
+  }                                                                                                        \
+  R PREFIX##dpsi(const void* p, int dim, R x) {                                                             \
+    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
+    return (R)host::dpsi(P->win, P->n[dim], P->b[dim], P->m, (double)x);                                    \
+  }                                                                                                        \
+  R PREFIX##ddpsi(const void* p, int dim, R x) {                                                            \
+    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
+    return (R)host::ddpsi(P->win, P->n[dim], P->b[dim], P->m, (double)x);                                   \
+  }                                                                                                        \
+  double* PREFIX##get_timer_trafo(void* p) {                                                                 \
+    double* t = (double*)malloc(sizeof(double) * T_LEN);                                                    \
+    memcpy(t, static_cast<PlanT<R>*>(p)->timer_trafo, sizeof(double) * T_LEN); return t;                    \
+  }                                                                                                        \
+  double* PREFIX##get_timer_adj(void* p) {                                                                   \
+    double* t = (double*)malloc(sizeof(double) * T_LEN);                                                    \
+    memcpy(t, static_cast<PlanT<R>*>(p)->timer_adj, sizeof(double) * T_LEN); return t;                      \
+  }                                                                                                        \
+  void PREFIX##timer_average(double* t) {                                                                    \
+    if (t[T_ITER] > 0) for (int i = 1; i < T_LEN; i++) t[i] /= t[T_ITER];                                   \
+  }                                                                                                        \
+  double* PREFIX##timer_copy(const double* o) {                                                              \
+    double* t = (double*)malloc(sizeof(double) * T_LEN); memcpy(t, o, sizeof(double) * T_LEN); return t;    \
+  }                                                                                                        \
+  double* PREFIX##timer_reduce_max(MPI_Comm c, double* t) { (void)c; return PREFIX##timer_copy(t); }         \
+  double* PREFIX##timer_add(const double* a, const double* b) {                                              \
+    double* t = (double*)malloc(sizeof(double) * T_LEN);                                                    \
+    for (int i = 0; i < T_LEN; i++) t[i] = a[i] + b[i]; return t;                                           \
+  }                                                                                                        \
+  void PREFIX##timer_free(double* t) { free(t); }                                                           \
+  void PREFIX##reset_timer(void* p) {                                                                       \
+    auto* P = static_cast<PlanT<R>*>(p);                                                                    \
+    for (int i = 0; i < T_LEN; i++) P->timer_trafo[i] = P->timer_adj[i] = 0;                                 \
+  }                                                                                                        \
+  static void PREFIX##print_timer(const double* tm, const char* name) {                                      \
+    double t[T_LEN]; memcpy(t, tm, sizeof(t));                                                              \
+    const double it = t[T_ITER] > 0 ? t[T_ITER] : 1;                                                        \
+    printf("%s: iterations %g, whole %.3e s, D %.3e s, F %.3e s, B %.3e s (loop_B %.3e s, sort %.3e s)\n",   \
+           name, t[T_ITER], t[T_WHOLE] / it, t[T_D] / it, t[T_F] / it, t[T_B] / it, t[T_LOOP_B] / it,       \
+           t[T_SORT] / it);                                                                                 \
+  }                                                                                                        \
+  void PREFIX##print_average_timer(const void* p, MPI_Comm c) {                                              \
+    (void)c; auto* P = static_cast<const PlanT<R>*>(p);                                                     \
+    PREFIX##print_timer(P->timer_trafo, "PNFFT trafo"); PREFIX##print_timer(P->timer_adj, "PNFFT adj");      \
+  }                                                                                                        \
+  void PREFIX##print_average_timer_adv(const void* p, MPI_Comm c) { PREFIX##print_average_timer(p, c); }     \
+  const char* PREFIX##last_error(void) { return g_last_error.empty() ? nullptr : g_last_error.c_str(); }     \
+  }
+
+typedef int MPI_Comm;
+PNFFT_DEFINE_IMPL(pnfft_, double)
+PNFFT_DEFINE_IMPL(pnfftf_, float)

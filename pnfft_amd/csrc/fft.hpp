@@ -1,0 +1,49 @@
+// fft.hpp -- host interface of the pruned 3-D FFT passes (fft.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device_common.hpp"
+
+namespace pnfft_amd {
+
+// Lengths handled by the LDS pencil kernel: powers of two up to 8192 (FP32)
+// or 4096 (FP64).
+bool fft_length_supported(int L, bool is_double);
+
+template <class R>
+struct FftAxis {
+  int L = 0;
+  int C = 0;          // pencils per block
+  int nst = 0;
+  int radix[16] = {0};
+  Cplx<R>* tw = nullptr;  // device twiddles e^{-2 pi i j / L}
+  int init(int L);
+  void release();
+};
+
+template <class R>
+struct Fft3d {
+  int N[3] = {0, 0, 0};
+  int n[3] = {0, 0, 0};
+  FftAxis<R> ax[3];
+  int init(const int* N, const int* n);
+  void release();
+  // element counts of the two intermediate buffers
+  size_t buf1_elems() const { return (size_t)N[0] * N[1] * n[2]; }
+  size_t buf2_elems() const { return (size_t)N[0] * n[1] * n[2]; }
+};
+
+// grid <- F D fhat  (tab = concatenated FP64 D factors, or null to skip D)
+template <class R>
+int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
+                  Cplx<R>* grid, hipStream_t s);
+
+// fhat (+)= D^T F^H grid
+template <class R>
+int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
+                   Cplx<R>* fhat, int accumulate, hipStream_t s);
+
+}  // namespace pnfft_amd

@@ -1,0 +1,113 @@
+// ndft.hip -- direct NDFT on the GPU (PNFFT_COMPUTE_DIRECT), the O(M N^3) sums
+// of PNX(trafo_A) / PNX(adj_A) (kernel/ndft-parallel.c:377-722), c2c.
+// Used for small problems and as an in-library cross-check; the exponentials
+// e^{-2 pi i k_t x_t} are formed per axis from sincos in the compute precision.
+#include "device_common.hpp"
+#include "ndft.hpp"
+
+namespace pnfft_amd {
+
+namespace {
+
+__device__ inline void sc(double a, double* s, double* c) { sincos(a, s, c); }
+__device__ inline void sc(float a, float* s, float* c) { sincosf(a, s, c); }
+
+template <class R>
+__global__ __launch_bounds__(256) void ndft_trafo_k(NdftArgs<R> A) {
+  const long j = (long)blockIdx.x * 256 + threadIdx.x;
+  if (j >= A.M) return;
+  const int N0 = A.N[0], N1 = A.N[1], N2 = A.N[2];
+  const R twopi = R(6.28318530717958647692);
+  const R x0 = A.x[3 * j], x1 = A.x[3 * j + 1], x2 = A.x[3 * j + 2];
+  Cplx<R> f = cmk<R>(0, 0), g0 = cmk<R>(0, 0), g1 = cmk<R>(0, 0), g2 = cmk<R>(0, 0);
+  for (int i0 = 0; i0 < N0; i0++) {
+    const R k0 = R(i0 - N0 / 2);
+    Cplx<R> s1 = cmk<R>(0, 0), s1k = cmk<R>(0, 0), s1z = cmk<R>(0, 0);
+    for (int i1 = 0; i1 < N1; i1++) {
+      const R k1 = R(i1 - N1 / 2);
+      const Cplx<R>* fh = A.fhat + ((long)i0 * N1 + i1) * N2;
+      Cplx<R> s2 = cmk<R>(0, 0), s2k = cmk<R>(0, 0);
+      for (int i2 = 0; i2 < N2; i2++) {
+        const R k2 = R(i2 - N2 / 2);
+        R sn, cs;
+        sc(-twopi * k2 * x2, &sn, &cs);
+        const Cplx<R> p = cmul(fh[i2], cmk<R>(cs, sn));
+        s2 = cadd(s2, p);
+        s2k = cadd(s2k, cscale(p, k2));
+      }
+      R sn, cs;
+      sc(-twopi * k1 * x1, &sn, &cs);
+      const Cplx<R> e1 = cmk<R>(cs, sn);
+      const Cplx<R> t = cmul(s2, e1);
+      s1 = cadd(s1, t);
+      s1k = cadd(s1k, cscale(t, k1));
+      s1z = cadd(s1z, cmul(s2k, e1));
+    }
+    R sn, cs;
+    sc(-twopi * k0 * x0, &sn, &cs);
+    const Cplx<R> e0 = cmk<R>(cs, sn);
+    const Cplx<R> t = cmul(s1, e0);
+    f = cadd(f, t);
+    g0 = cadd(g0, cscale(t, k0));
+    g1 = cadd(g1, cmul(s1k, e0));
+    g2 = cadd(g2, cmul(s1z, e0));
+  }
+  // the NDFT overwrites its outputs (kernel/ndft-parallel.c:400-414)
+  if (A.want_f) A.f[j] = f;
+  if (A.want_grad) {
+    // grad *= -2 pi i (:594-603)
+    A.grad[3 * j + 0] = cmk<R>(twopi * g0.y, -twopi * g0.x);
+    A.grad[3 * j + 1] = cmk<R>(twopi * g1.y, -twopi * g1.x);
+    A.grad[3 * j + 2] = cmk<R>(twopi * g2.y, -twopi * g2.x);
+  }
+}
+
+// one thread per frequency; adds onto f_hat (:649-650, :681, :699-707)
+template <class R>
+__global__ __launch_bounds__(256) void ndft_adj_k(NdftArgs<R> A) {
+  const long kidx = (long)blockIdx.x * 256 + threadIdx.x;
+  const int N0 = A.N[0], N1 = A.N[1], N2 = A.N[2];
+  if (kidx >= (long)N0 * N1 * N2) return;
+  const int i2 = (int)(kidx % N2), i1 = (int)((kidx / N2) % N1), i0 = (int)(kidx / ((long)N1 * N2));
+  const R k0 = R(i0 - N0 / 2), k1 = R(i1 - N1 / 2), k2 = R(i2 - N2 / 2);
+  const R twopi = R(6.28318530717958647692);
+  Cplx<R> acc = cmk<R>(0, 0);
+  for (long j = 0; j < A.M; j++) {
+    R sn, cs;
+    sc(twopi * (k0 * A.x[3 * j] + k1 * A.x[3 * j + 1] + k2 * A.x[3 * j + 2]), &sn, &cs);
+    Cplx<R> c = cmk<R>(0, 0);
+    if (A.want_f) c = A.f[j];
+    if (A.want_grad) {
+      const Cplx<R> g0 = A.grad[3 * j], g1 = A.grad[3 * j + 1], g2 = A.grad[3 * j + 2];
+      const R sr = k0 * g0.x + k1 * g1.x + k2 * g2.x;
+      const R si = k0 * g0.y + k1 * g1.y + k2 * g2.y;
+      c = cadd(c, cmk<R>(-twopi * si, twopi * sr));
+    }
+    acc = cadd(acc, cmul(c, cmk<R>(cs, sn)));
+  }
+  A.fhat[kidx] = cadd(A.fhat[kidx], acc);
+}
+
+}  // namespace
+
+template <class R>
+int ndft_trafo(const NdftArgs<R>& A, hipStream_t s) {
+  if (A.M == 0) return kOk;
+  hipLaunchKernelGGL((ndft_trafo_k<R>), dim3((unsigned)((A.M + 255) / 256)), dim3(256), 0, s, A);
+  return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
+template <class R>
+int ndft_adj(const NdftArgs<R>& A, hipStream_t s) {
+  const long K = (long)A.N[0] * A.N[1] * A.N[2];
+  if (K == 0) return kOk;
+  hipLaunchKernelGGL((ndft_adj_k<R>), dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, A);
+  return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
+template int ndft_trafo<float>(const NdftArgs<float>&, hipStream_t);
+template int ndft_trafo<double>(const NdftArgs<double>&, hipStream_t);
+template int ndft_adj<float>(const NdftArgs<float>&, hipStream_t);
+template int ndft_adj<double>(const NdftArgs<double>&, hipStream_t);
+
+}  // namespace pnfft_amd

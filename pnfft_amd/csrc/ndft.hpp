@@ -1,0 +1,28 @@
+// ndft.hpp -- host interface of the direct NDFT kernels (ndft.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace pnfft_amd {
+
+template <class R>
+struct NdftArgs {
+  int N[3];
+  long M;
+  const R* x;
+  Cplx<R>* fhat;
+  Cplx<R>* f;
+  Cplx<R>* grad;
+  int want_f;
+  int want_grad;
+};
+
+template <class R>
+int ndft_trafo(const NdftArgs<R>& A, hipStream_t s);
+
+template <class R>
+int ndft_adj(const NdftArgs<R>& A, hipStream_t s);
+
+}  // namespace pnfft_amd

@@ -1,0 +1,293 @@
+// nfft_b.hip -- matrix B of the NFFT (window gather for trafo, window scatter
+// for adj) and the binning of nodes by grid cell.
+//
+// Reference: PNX(trafo_B_ad) / PNX(adjoint_B_ad) and their node loops
+// (kernel/ndft-parallel.c:2521-3009) with the stencil kernels of
+// kernel/assign.c:478-1066.  The reference stores the FFT output shifted and
+// ghost-padded by m cells and exchanges the ghost layers (PX(exchange) /
+// PX(reduce), :2558, :2679); here the grid stays in natural DFT order and the
+// periodic wrap is applied to the stencil indices, so no ghost copy exists.
+//
+// Node n's stencil on axis t covers grid indices (floor(n_t x_t) - m + s) mod n_t,
+// s = 0..2m (cutoff 2m+1, :910, :2165-2174).  floor(n x) is taken in FP64 for
+// both precisions; the window is evaluated in the compute precision from the
+// FP64 fractional part.
+#include <cstdio>
+
+#include "device_common.hpp"
+#include "b_common.hpp"
+#include "nfft_b.hpp"
+#include "window_math.hpp"
+
+namespace pnfft_amd {
+
+namespace {
+
+constexpr int kB = 256;  // threads per block of the B kernels
+
+// ---------------------------------------------------------------------------
+// binning: keys, stable LSD radix sort, tile bounds
+// ---------------------------------------------------------------------------
+constexpr int kSortT = 256;
+constexpr int kSortIPB = 4096;  // items per block
+constexpr int kScanChunk = 4096;
+
+template <class R>
+__global__ __launch_bounds__(kB) void bin_keys_k(const R* __restrict__ x, long M, GridGeom g, TileLayout T,
+                                                 unsigned* keys, int* vals) {
+  const long j = (long)blockIdx.x * kB + threadIdx.x;
+  if (j >= M) return;
+  unsigned long long tile = 0, in = 0;
+  for (int t = 0; t < 3; t++) {
+    const NodeAxis<R> a = node_axis<R>(x[3 * j + t], g.n[t], g.m);
+    tile = tile * T.ntile[t] + (unsigned)(a.start / T.ts[t]);
+    in = in * T.ts[t] + (unsigned)(a.start % T.ts[t]);
+  }
+  keys[j] = (unsigned)(tile * (unsigned long long)(T.ts[0] * T.ts[1] * T.ts[2]) + in);
+  vals[j] = (int)j;
+}
+
+__global__ __launch_bounds__(kSortT) void sort_hist_k(const unsigned* __restrict__ keys, long M, int shift,
+                                                      unsigned* hist, int nblk) {
+  __shared__ unsigned h[256];
+  const int tid = threadIdx.x;
+  h[tid] = 0;
+  __syncthreads();
+  const long base = (long)blockIdx.x * kSortIPB;
+  for (int i = 0; i < kSortIPB / kSortT; i++) {
+    const long idx = base + (long)i * kSortT + tid;
+    if (idx < M) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(long)tid * nblk + blockIdx.x] = h[tid];
+}
+
+// Block-wide exclusive scan of 256 values held one per thread.
+__device__ inline unsigned block_exscan256(unsigned v, unsigned* tmp, unsigned* total) {
+  const int tid = threadIdx.x;
+  tmp[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    unsigned add = (tid >= off) ? tmp[tid - off] : 0u;
+    __syncthreads();
+    tmp[tid] += add;
+    __syncthreads();
+  }
+  const unsigned incl = tmp[tid];
+  if (total) *total = tmp[255];
+  __syncthreads();
+  return incl - v;
+}
+
+// chunk sums
+__global__ __launch_bounds__(kSortT) void scan_reduce_k(const unsigned* __restrict__ d, long n, unsigned* partial) {
+  __shared__ unsigned tmp[256];
+  const long base = (long)blockIdx.x * kScanChunk;
+  unsigned s = 0;
+  for (int i = 0; i < kScanChunk / kSortT; i++) {
+    const long idx = base + (long)threadIdx.x * (kScanChunk / kSortT) + i;
+    if (idx < n) s += d[idx];
+  }
+  unsigned total;
+  __shared__ unsigned tot;
+  (void)block_exscan256(s, tmp, &tot);
+  total = tot;
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// exclusive scan of the chunk sums, one block, sequential over 256-wide pieces
+__global__ __launch_bounds__(kSortT) void scan_partials_k(unsigned* partial, long np) {
+  __shared__ unsigned tmp[256];
+  __shared__ unsigned tot;
+  unsigned carry = 0;
+  for (long base = 0; base < np; base += 256) {
+    const long idx = base + threadIdx.x;
+    const unsigned v = idx < np ? partial[idx] : 0u;
+    const unsigned ex = block_exscan256(v, tmp, &tot);
+    if (idx < np) partial[idx] = ex + carry;
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kSortT) void scan_apply_k(unsigned* d, long n, const unsigned* __restrict__ partial) {
+  __shared__ unsigned tmp[256];
+  __shared__ unsigned tot;
+  constexpr int PER = kScanChunk / kSortT;
+  const long base = (long)blockIdx.x * kScanChunk + (long)threadIdx.x * PER;
+  unsigned loc[PER];
+  unsigned s = 0;
+  for (int i = 0; i < PER; i++) {
+    const long idx = base + i;
+    loc[i] = idx < n ? d[idx] : 0u;
+    s += loc[i];
+  }
+  unsigned ex = block_exscan256(s, tmp, &tot) + partial[blockIdx.x];
+  for (int i = 0; i < PER; i++) {
+    const long idx = base + i;
+    if (idx < n) d[idx] = ex;
+    ex += loc[i];
+  }
+}
+
+// Stable scatter of one digit pass: items of a block are ranked in index order.
+__global__ __launch_bounds__(kSortT) void sort_scatter_k(const unsigned* __restrict__ kin, const int* __restrict__ vin,
+                                                         unsigned* kout, int* vout, long M, int shift,
+                                                         const unsigned* __restrict__ offs, int nblk) {
+  __shared__ unsigned run[256];
+  __shared__ unsigned wcnt[4][256];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  run[tid] = 0;
+  const unsigned boff = 0;
+  (void)boff;
+  const long base = (long)blockIdx.x * kSortIPB;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < kSortIPB / kSortT; r++) {
+    const long idx = base + (long)r * kSortT + tid;
+    const bool valid = idx < M;
+    unsigned key = 0;
+    int val = 0;
+    unsigned d = 0;
+    if (valid) {
+      key = kin[idx];
+      val = vin[idx];
+      d = (key >> shift) & 255u;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
+    __syncthreads();
+    unsigned long long mask = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+      const bool on = (d >> bit) & 1u;
+      const unsigned long long bb = __ballot(valid && on);
+      mask &= on ? bb : ~bb;
+    }
+    const unsigned rank = (unsigned)__popcll(mask & lt_mask);
+    if (valid && rank == 0) wcnt[wave][d] = (unsigned)__popcll(mask);
+    __syncthreads();
+    if (valid) {
+      unsigned pre = run[d] + rank;
+      for (int w = 0; w < wave; w++) pre += wcnt[w][d];
+      const unsigned pos = offs[(long)d * nblk + blockIdx.x] + pre;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+    __syncthreads();
+  }
+}
+
+__global__ void fill_int_k(int* p, long n, int v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kB) void tile_bounds_k(const unsigned* __restrict__ skeys, long M, unsigned vol,
+                                                    int* tile_start) {
+  const long i = (long)blockIdx.x * kB + threadIdx.x;
+  if (i >= M) return;
+  const long t = skeys[i] / vol;
+  const long tprev = (i == 0) ? -1 : (long)(skeys[i - 1] / vol);
+  for (long tt = tprev + 1; tt <= t; tt++) tile_start[tt] = (int)i;
+}
+
+template <class R>
+__global__ void fill_zero_k(Cplx<R>* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = cmk<R>(R(0), R(0));
+}
+
+inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+int sort_passes(unsigned long long key_limit) {
+  int bits = 0;
+  while (bits < 32 && (1ull << bits) < key_limit) bits++;
+  return (bits + 7) / 8;
+}
+
+static int scan_exclusive(unsigned* d, long n, unsigned* partial, hipStream_t s) {
+  const long nch = (n + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(scan_reduce_k, dim3(grid1d(nch * 1, 1)), dim3(kSortT), 0, s, d, n, partial);
+  hipLaunchKernelGGL(scan_partials_k, dim3(1), dim3(kSortT), 0, s, partial, nch);
+  hipLaunchKernelGGL(scan_apply_k, dim3(grid1d(nch, 1)), dim3(kSortT), 0, s, d, n, partial);
+  return kOk;
+}
+
+template <class R>
+int bin_nodes(const R* x, long M, const GridGeom& g, const TileLayout& T, SortWork& W, int* perm_out,
+              unsigned* keys_out, int* tile_start, hipStream_t s) {
+  const long ntiles = T.ntiles();
+  hipLaunchKernelGGL(fill_int_k, dim3(grid1d(ntiles + 1, 256)), dim3(256), 0, s, tile_start, ntiles + 1, (int)M);
+  if (M == 0) return hipGetLastError() == hipSuccess ? kOk : kHipError;
+  const unsigned long long key_limit = (unsigned long long)ntiles * (unsigned long long)T.tile_vol();
+  if (key_limit > (1ull << 32)) return kUnsupported;
+  const int passes = sort_passes(key_limit);
+  const int nblk = (int)((M + kSortIPB - 1) / kSortIPB);
+  hipLaunchKernelGGL((bin_keys_k<R>), dim3(grid1d(M, kB)), dim3(kB), 0, s, x, M, g, T, W.keys[0], W.vals[0]);
+  int cur = 0;
+  for (int ps = 0; ps < passes; ps++) {
+    const int shift = 8 * ps;
+    hipLaunchKernelGGL(sort_hist_k, dim3(nblk), dim3(kSortT), 0, s, W.keys[cur], M, shift, W.hist, nblk);
+    scan_exclusive(W.hist, 256L * nblk, W.partial, s);
+    hipLaunchKernelGGL(sort_scatter_k, dim3(nblk), dim3(kSortT), 0, s, W.keys[cur], W.vals[cur], W.keys[1 - cur],
+                       W.vals[1 - cur], M, shift, W.hist, nblk);
+    cur = 1 - cur;
+  }
+  if (hipMemcpyAsync(perm_out, W.vals[cur], sizeof(int) * M, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return kHipError;
+  if (hipMemcpyAsync(keys_out, W.keys[cur], sizeof(unsigned) * M, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return kHipError;
+  hipLaunchKernelGGL(tile_bounds_k, dim3(grid1d(M, kB)), dim3(kB), 0, s, keys_out, M, (unsigned)T.tile_vol(),
+                     tile_start);
+  return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
+template <class R>
+int gather(const BArgs<R>& A, int mode, hipStream_t s) {
+  if (A.M == 0) return kOk;
+  if (A.g.m < 1 || A.g.m > kMaxM) return kUnsupported;
+  switch (mode) {
+    case B_F: return b_launch<R, 0, B_F>(A, s);
+    case B_GRAD: return b_launch<R, 0, B_GRAD>(A, s);
+    case B_FGRAD: return b_launch<R, 0, B_FGRAD>(A, s);
+    case B_VJP: return b_launch<R, 0, B_VJP>(A, s);
+    default: return kUnsupported;
+  }
+}
+
+template <class R>
+int scatter(const BArgs<R>& A, int mode, hipStream_t s) {
+  if (A.M == 0) return kOk;
+  if (A.g.m < 1 || A.g.m > kMaxM) return kUnsupported;
+  switch (mode) {
+    case B_F: return b_launch<R, 1, B_F>(A, s);
+    case B_GRAD: return b_launch<R, 1, B_GRAD>(A, s);
+    case B_FGRAD: return b_launch<R, 1, B_FGRAD>(A, s);
+    default: return kUnsupported;
+  }
+}
+
+template <class R>
+int fill_zero(Cplx<R>* p, size_t n, hipStream_t s) {
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL((fill_zero_k<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
+template int bin_nodes<float>(const float*, long, const GridGeom&, const TileLayout&, SortWork&, int*, unsigned*,
+                              int*, hipStream_t);
+template int bin_nodes<double>(const double*, long, const GridGeom&, const TileLayout&, SortWork&, int*, unsigned*,
+                               int*, hipStream_t);
+template int gather<float>(const BArgs<float>&, int, hipStream_t);
+template int gather<double>(const BArgs<double>&, int, hipStream_t);
+template int scatter<float>(const BArgs<float>&, int, hipStream_t);
+template int scatter<double>(const BArgs<double>&, int, hipStream_t);
+template int fill_zero<float>(Cplx<float>*, size_t, hipStream_t);
+template int fill_zero<double>(Cplx<double>*, size_t, hipStream_t);
+
+}  // namespace pnfft_amd

@@ -1,0 +1,77 @@
+// nfft_b.hpp -- host interface of matrix B (gather / scatter) and node binning.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace pnfft_amd {
+
+// B modes.  Gather: f, grad, f+grad, or the x-VJP epilogue.  Scatter: f, grad, f+grad.
+enum BMode : int { B_F = 1, B_GRAD = 2, B_FGRAD = 3, B_VJP = 4 };
+
+template <class R>
+struct BArgs {
+  GridGeom g;
+  long M;
+  const R* x;              // R[3M]
+  const int* perm;         // processing order (sorted node ids) or null
+  Cplx<R>* grid;           // n0*n1*n2, natural order
+  Cplx<R>* f;              // C[M]
+  Cplx<R>* grad;           // C[3M]
+  const Cplx<R>* w;        // VJP cotangent C[M]
+  R* gx;                   // VJP output R[3M]
+  int accumulate;          // gather: add onto outputs
+  int real_only;           // gather: PNFFT_REAL_F, write the real part only
+  // tiled path
+  const int* tile_start;   // [ntiles+1] offsets into perm
+  const int* items;        // work items: packed (tile id, chunk)
+  int n_items;
+  int ts[3];               // tile size per axis
+  int ntile[3];            // tiles per axis
+};
+
+// Tile layout used by the binning: a node belongs to the tile that holds its
+// stencil start (floor(n x) - m) mod n on every axis.
+struct TileLayout {
+  int ts[3];
+  int ntile[3];
+  long ntiles() const { return (long)ntile[0] * ntile[1] * ntile[2]; }
+  long tile_vol() const { return (long)ts[0] * ts[1] * ts[2]; }
+};
+
+// Sorting scratch (device): keys/values double-buffered, histograms, scan partials.
+struct SortWork {
+  unsigned* keys[2] = {nullptr, nullptr};
+  int* vals[2] = {nullptr, nullptr};
+  unsigned* hist = nullptr;      // 256 * nblocks
+  unsigned* partial = nullptr;   // scan partials
+  long cap = 0;                  // capacity in nodes
+  long hist_cap = 0;
+  long partial_cap = 0;
+};
+
+// Compute bin keys of every node and sort the node ids by key (stable LSD
+// radix sort, 8-bit digits).  Result: perm (sorted node ids, == vals[res]),
+// sorted keys, and tile_start[ntiles+1].  Returns kOk or an error code.
+template <class R>
+int bin_nodes(const R* x, long M, const GridGeom& g, const TileLayout& T, SortWork& W, int* perm_out,
+              unsigned* keys_out, int* tile_start, hipStream_t s);
+
+// Kernel launchers, one object file per (R, KIND, MODE); KIND 0 = gather, 1 = scatter.
+template <class R, int KIND, int MODE>
+int b_launch(const BArgs<R>& A, hipStream_t s);
+
+template <class R>
+int gather(const BArgs<R>& A, int mode, hipStream_t s);
+
+template <class R>
+int scatter(const BArgs<R>& A, int mode, hipStream_t s);
+
+template <class R>
+int fill_zero(Cplx<R>* p, size_t n, hipStream_t s);
+
+// Number of sort passes for keys below key_limit.
+int sort_passes(unsigned long long key_limit);
+
+}  // namespace pnfft_amd

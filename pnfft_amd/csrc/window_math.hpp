@@ -1,0 +1,216 @@
+// window_math.hpp -- window functions psi / psi' on the (2m+1)-point stencil
+// of one axis, for host and device, templated on the compute precision.
+//
+// For a node coordinate x on an axis with oversampled length n, let
+// fl = floor(n x) (taken in FP64) and frac = n x - fl in [0, 1).  Stencil
+// point s = 0..2m is grid index l_s = fl - m + s and
+//   psi[s]  = psi(x - l_s/n),   dpsi[s] = psi'(x - l_s/n)  (d/dx).
+// Window formulas follow kernel/ndft-parallel.c:1678-1797 (psi) and
+// :1850-1953 (psi'), with the 1-d kernels of :2241-2269 and the sinc of
+// kernel/sinc.c:26-52.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace pnfft_amd {
+
+enum WindowId : int { WIN_KB = 0, WIN_GAUSS = 1, WIN_BSPLINE = 2, WIN_SINCPOW = 3, WIN_BESSEL_I0 = 4 };
+
+constexpr int kMaxM = 16;                 // largest supported cutoff parameter m
+constexpr int kMaxC = 2 * kMaxM + 1;      // largest stencil width
+
+template <class R> struct Consts;
+template <> struct Consts<double> {
+  // PNFFT_EPSILON of the double library (kernel/ipnfft.h:99)
+  static constexpr double eps = 2.220446049250313080847e-16;
+};
+template <> struct Consts<float> {
+  // PNFFT_EPSILON of the single library is LDBL_EPSILON (kernel/ipnfft.h:81)
+  static constexpr float eps = 1.08420217248550443e-19f;
+};
+
+__host__ __device__ inline double pi_v(double) { return 3.14159265358979323846; }
+__host__ __device__ inline float pi_v(float) { return 3.14159265358979323846f; }
+
+__host__ __device__ inline double m_sqrt(double v) { return sqrt(v); }
+__host__ __device__ inline float m_sqrt(float v) { return sqrtf(v); }
+__host__ __device__ inline double m_sin(double v) { return sin(v); }
+__host__ __device__ inline float m_sin(float v) { return sinf(v); }
+__host__ __device__ inline double m_cos(double v) { return cos(v); }
+__host__ __device__ inline float m_cos(float v) { return cosf(v); }
+__host__ __device__ inline double m_sinh(double v) { return sinh(v); }
+__host__ __device__ inline float m_sinh(float v) { return sinhf(v); }
+__host__ __device__ inline double m_cosh(double v) { return cosh(v); }
+__host__ __device__ inline float m_cosh(float v) { return coshf(v); }
+__host__ __device__ inline double m_exp(double v) { return exp(v); }
+__host__ __device__ inline float m_exp(float v) { return expf(v); }
+__host__ __device__ inline double m_tan(double v) { return tan(v); }
+__host__ __device__ inline float m_tan(float v) { return tanf(v); }
+__host__ __device__ inline double m_fabs(double v) { return fabs(v); }
+__host__ __device__ inline float m_fabs(float v) { return fabsf(v); }
+
+// sin(x)/x with the small-argument branches of kernel/sinc.c:26-52.
+template <class R>
+__host__ __device__ inline R sinc_ref(R x) {
+  const R b = Consts<R>::eps;
+  const R bs = m_sqrt(b);
+  const R bs2 = m_sqrt(bs);
+  if (m_fabs(x) >= bs2) return m_sin(x) / x;
+  R r = R(1);
+  if (m_fabs(x) >= b) {
+    const R x2 = x * x;
+    r -= x2 / R(6);
+    if (m_fabs(x) >= bs) r += (x2 * x2) / R(120);
+  }
+  return r;
+}
+
+// Integer power by repeated squaring (the exponent 2m of the sinc-power window).
+template <class R>
+__host__ __device__ inline R ipow(R v, int e) {
+  R r = R(1);
+  while (e > 0) {
+    if (e & 1) r *= v;
+    v *= v;
+    e >>= 1;
+  }
+  return r;
+}
+
+// V[d] = M_k(u + d), d = 0..k-1, for the cardinal B-spline M_k on [0, k] and
+// u in (0, 1]; recursion M_r(t) = (t M_{r-1}(t) + (r - t) M_{r-1}(t-1))/(r-1).
+template <class R>
+__host__ __device__ inline void bspline_values(int k, R u, R* V) {
+  V[0] = R(1);
+  for (int r = 2; r <= k; r++) {
+    const R inv = R(1) / R(r - 1);
+    V[r - 1] = R(0);
+    for (int d = r - 1; d >= 0; d--) {
+      const R left = (d > 0) ? V[d - 1] : R(0);
+      V[d] = ((u + R(d)) * V[d] + (R(r) - u - R(d)) * left) * inv;
+    }
+  }
+}
+
+// (sinh z - z cosh z) / z^3 and (sin z - z cos z) / z^3 without cancellation.
+template <class R>
+__host__ __device__ inline R kb_h_sinh(R z) {
+  const R z2 = z * z;
+  if (z2 < R(0.25)) {
+    // -sum_{k>=1} 2k z^{2k-2} / (2k+1)!
+    R term = R(1), sum = R(0);
+    R fact = R(6);  // (2k+1)! for k = 1
+    for (int k = 1; k <= 8; k++) {
+      sum += R(2 * k) * term / fact;
+      term *= z2;
+      fact *= R((2 * k + 2) * (2 * k + 3));
+    }
+    return -sum;
+  }
+  return (m_sinh(z) - z * m_cosh(z)) / (z2 * z);
+}
+
+template <class R>
+__host__ __device__ inline R kb_h_sin(R z) {
+  const R z2 = z * z;
+  if (z2 < R(0.25)) {
+    // sum_{k>=1} (-1)^{k+1} 2k z^{2k-2} / (2k+1)!
+    R term = R(1), sum = R(0), sgn = R(1);
+    R fact = R(6);
+    for (int k = 1; k <= 8; k++) {
+      sum += sgn * R(2 * k) * term / fact;
+      term *= z2;
+      sgn = -sgn;
+      fact *= R((2 * k + 2) * (2 * k + 3));
+    }
+    return sum;
+  }
+  return (m_sin(z) - z * m_cos(z)) / (z2 * z);
+}
+
+// Kaiser-Bessel psi at grid offset u = l - n x (kaiser_bessel_1d, :2241-2254).
+template <class R>
+__host__ __device__ inline R kb_psi(R u, R b, int m) {
+  const R d = R(m) * R(m) - u * u;
+  const R r = m_sqrt(m_fabs(d));
+  const R pi = pi_v(R(0));
+  if (d < R(0)) return m_sin(b * r) / (pi * r);
+  return (d > R(0)) ? m_sinh(b * r) / (pi * r) : b / pi;
+}
+
+// Kaiser-Bessel psi' at xn = n x - l (kaiser_bessel_derivative_1d, :2256-2269).
+// FP64 keeps the reference's formula; FP32 uses the cancellation-free form
+// n xn b^3 h(b r)/pi.  The d == 0 value is the reference's -n m b^3/(3 pi) on
+// both stencil edges.
+template <class R>
+__host__ __device__ inline R kb_dpsi(R xn, R nR, R b, int m, R psi) {
+  const R d = R(m) * R(m) - xn * xn;
+  const R r = m_sqrt(m_fabs(d));
+  const R pi = pi_v(R(0));
+  if (d == R(0)) return -nR * R(m) * b * b * b / (R(3) * pi);
+  if (sizeof(R) == sizeof(double)) {
+    if (d < R(0)) return nR * xn / d * (psi - b * m_cos(b * r) / pi);
+    return nR * xn / d * (psi - b * m_cosh(b * r) / pi);
+  }
+  const R z = b * r;
+  if (d > R(0)) return nR * xn * b * b * b * kb_h_sinh(z) / pi;
+  return -nR * xn * b * b * b * kb_h_sin(z) / pi;
+}
+
+// Window values of one axis.  frac in [0,1), nR = n as R.  dpsi may be null.
+template <class R>
+__host__ __device__ inline void window_axis(int win, int m, R frac, R nR, R b, R* psi, R* dpsi) {
+  const int c = 2 * m + 1;
+  const R pi = pi_v(R(0));
+  switch (win) {
+    case WIN_GAUSS: {
+      const R norm = R(1) / m_sqrt(pi * b);
+      for (int s = 0; s < c; s++) {
+        const R u = R(s - m) - frac;
+        psi[s] = m_exp(-(u * u) / b) * norm;
+      }
+      if (dpsi)
+        for (int s = 0; s < c; s++) dpsi[s] = R(-2) * nR / b * (frac + R(m - s)) * psi[s];
+      break;
+    }
+    case WIN_BSPLINE: {
+      R V[2 * kMaxM], W[2 * kMaxM];
+      const R u = R(1) - frac;
+      bspline_values<R>(2 * m, u, V);
+      psi[0] = R(0);
+      for (int s = 1; s < c; s++) psi[s] = V[s - 1];
+      if (dpsi) {
+        bspline_values<R>(2 * m - 1, u, W);
+        for (int s = 0; s < c; s++) {
+          const R a = (s >= 1 && s - 1 < 2 * m - 1) ? W[s - 1] : R(0);
+          const R bb = (s >= 2 && s - 2 < 2 * m - 1) ? W[s - 2] : R(0);
+          dpsi[s] = -nR * (a - bb);
+        }
+      }
+      break;
+    }
+    case WIN_SINCPOW: {
+      for (int s = 0; s < c; s++) {
+        const R u = R(s - m) - frac;
+        psi[s] = ipow(sinc_ref<R>(pi * u / b), 2 * m) / b;
+      }
+      if (dpsi)
+        for (int s = 0; s < c; s++) {
+          const R y = pi * (frac + R(m - s)) / b;
+          dpsi[s] = (m_fabs(y) > Consts<R>::eps)
+                        ? R(2 * m) * pi * nR / b * (R(1) / m_tan(y) - R(1) / y) * psi[s]
+                        : R(0);
+        }
+      break;
+    }
+    default: {  // Kaiser-Bessel
+      for (int s = 0; s < c; s++) psi[s] = kb_psi<R>(R(s - m) - frac, b, m);
+      if (dpsi)
+        for (int s = 0; s < c; s++) dpsi[s] = kb_dpsi<R>(frac + R(m - s), nR, b, m, psi[s]);
+      break;
+    }
+  }
+}
+
+}  // namespace pnfft_amd

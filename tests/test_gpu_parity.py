@@ -1,0 +1,274 @@
+"""HIP path vs the FP64 oracle, through the C interface (GPU).
+
+Tolerances: FP64 plans match the oracle's restatement of the reference
+algorithm to rel-l2 <= 1e-12 (same algorithm, different FFT and summation
+order); FP32 plans match it to rel-l2 <= 1e-5; both match the NDFT to the
+window's accuracy.  Node binning is integer work and must equal numpy's stable
+sort bit for bit.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import pnfft_amd as PA
+from pnfft_amd.device import DevicePlan
+from pnfft_amd.workloads import SplitMix64, make
+
+pytestmark = pytest.mark.gpu
+
+WINDOWS = {"kb": 0, "gauss": 1 << 13, "bspline": 1 << 14, "sincpow": 1 << 15}
+NDFT_ACC = {"kb": 1e-10, "gauss": 2e-5, "bspline": 8e-6, "sincpow": 3e-6}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a).reshape(-1)
+    b = np.asarray(b).reshape(-1)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run_gpu(N, n, m, flags, x, fh, fv=None, gv=None, precision="double", grad=True, precompute=False):
+    M = len(x)
+    dp = DevicePlan(N, n, m, flags, M, precision=precision)
+    dp.set_x(x)
+    dp.set_f_hat(fh)
+    if precompute:
+        dp.precompute()
+    out = {}
+    dp.trafo(PA.PNFFT_COMPUTE_F | (PA.PNFFT_COMPUTE_GRAD_F if grad else 0))
+    out["f"] = dp.f.cpu().numpy().copy()
+    if grad:
+        out["grad"] = dp.grad.cpu().numpy().reshape(-1).copy()
+    if fv is not None:
+        dp.set_f(fv)
+        dp.adj(PA.PNFFT_COMPUTE_F)
+        out["adj"] = dp.f_hat.cpu().numpy().reshape(-1).copy()
+    if gv is not None:
+        dp.set_grad(gv)
+        dp.adj(PA.PNFFT_COMPUTE_GRAD_F)
+        out["adj_grad"] = dp.f_hat.cpu().numpy().reshape(-1).copy()
+    err = dp.api.last_error()
+    dp.close()
+    assert err is None, err
+    return out
+
+
+@pytest.mark.parametrize("wname", list(WINDOWS))
+def test_fp64_matches_oracle_cfg1(gpu, oracle, wname):
+    w = make(1)
+    N, n, m = w["N"], w["n"], w["m"]
+    flags = WINDOWS[wname]
+    say("case fp64 cfg1", wname)
+    rng = SplitMix64(7)
+    gv = rng.complex_normal(3 * w["M"])
+    g = run_gpu(N, n, m, flags, w["x"], w["f_hat"], w["f"], gv)
+    P = oracle.Plan(N, n, m, flags)
+    f_o, g_o = P.trafo(w["x"], w["f_hat"], compute_grad=True)
+    assert rel(g["f"], f_o) < 1e-12
+    assert rel(g["grad"], g_o) < 1e-12
+    assert rel(g["adj"], P.adj(w["x"], f=w["f"])) < 1e-12
+    assert rel(g["adj_grad"], P.adj(w["x"], grad=gv)) < 1e-12
+    f_nd, _ = oracle.ndft_trafo(N, w["x"], w["f_hat"])
+    assert rel(g["f"], f_nd) < NDFT_ACC[wname]
+
+
+@pytest.mark.parametrize("N,n,m,wname", [((8, 16, 32), (16, 32, 64), 3, "gauss"),
+                                         ((16, 8, 4), (32, 16, 8), 2, "kb"),
+                                         ((12, 10, 6), (32, 16, 16), 5, "bspline"),
+                                         ((4, 4, 4), (64, 64, 64), 8, "sincpow")])
+def test_fp64_anisotropic(gpu, oracle, N, n, m, wname):
+    say("case anisotropic", N, n, m, wname)
+    rng = SplitMix64(11)
+    M = 777
+    x = rng.uniform(3 * M).reshape(M, 3) - 0.5
+    fh = rng.complex_normal(int(np.prod(N)))
+    fv = rng.complex_normal(M)
+    g = run_gpu(N, n, m, WINDOWS[wname], x, fh, fv)
+    P = oracle.Plan(N, n, m, WINDOWS[wname])
+    f_o, g_o = P.trafo(x, fh, compute_grad=True)
+    assert rel(g["f"], f_o) < 1e-12
+    assert rel(g["grad"], g_o) < 1e-12
+    assert rel(g["adj"], P.adj(x, f=fv)) < 1e-12
+
+
+@pytest.mark.parametrize("wname", ["kb", "gauss"])
+def test_fp32_vs_fp64_oracle(gpu, oracle, wname):
+    w = make(1)
+    say("case fp32", wname)
+    x32 = w["x"].astype(np.float32).astype(np.float64)  # the FP32 plan sees float nodes
+    g = run_gpu(w["N"], w["n"], w["m"], WINDOWS[wname], x32, w["f_hat"], w["f"], precision="float")
+    P = oracle.Plan(w["N"], w["n"], w["m"], WINDOWS[wname])
+    f_o, g_o = P.trafo(x32, w["f_hat"], compute_grad=True)
+    assert rel(g["f"], f_o) < 1e-5
+    assert rel(g["grad"], g_o) < 1e-5
+    assert rel(g["adj"], P.adj(x32, f=w["f"])) < 1e-5
+
+
+def test_edge_nodes_and_sizes(gpu, oracle):
+    N, n, m = (8, 8, 8), (16, 16, 16), 3
+    P = oracle.Plan(N, n, m, 0)
+    rng = SplitMix64(12)
+    special = np.array([[-0.5, -0.5, -0.5], [0.0, 0.0, 0.0], [0.25, -0.125, 0.4375],  # on grid points
+                        [0.5 - 2 ** -40, -0.5 + 2 ** -40, 0.5 - 1e-9], [1 / 3, -1 / 7, 2 / 9]])
+    for M in [1, 5, 257, 1000]:
+        say("case edge M", M)
+        x = np.concatenate([special, rng.uniform(3 * M).reshape(M, 3) - 0.5])[:M]
+        fh = rng.complex_normal(512)
+        fv = rng.complex_normal(M)
+        g = run_gpu(N, n, m, 0, x, fh, fv)
+        f_o, g_o = P.trafo(x, fh, compute_grad=True)
+        assert rel(g["f"], f_o) < 1e-12
+        assert rel(g["grad"], g_o) < 1e-12
+        assert rel(g["adj"], P.adj(x, f=fv)) < 1e-12
+
+
+def test_empty_node_set(gpu):
+    say("case M=0")
+    dp = DevicePlan((8, 8, 8), (16, 16, 16), 3, 0, 0)
+    dp.set_f_hat(np.ones(512))
+    dp.trafo(PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_GRAD_F)
+    dp.adj(PA.PNFFT_COMPUTE_F)
+    assert float(dp.f_hat.abs().max()) == 0.0  # adj of no nodes is zero
+    assert dp.api.last_error() is None
+    dp.close()
+
+
+def test_accumulate_flags(gpu, oracle):
+    w = make(1, M=300)
+    say("case accumulate")
+    dp = DevicePlan(w["N"], w["n"], w["m"], 0, w["M"])
+    dp.set_x(w["x"])
+    dp.set_f_hat(w["f_hat"])
+    dp.trafo(PA.PNFFT_COMPUTE_F)
+    f1 = dp.f.cpu().numpy().copy()
+    dp.trafo(PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_ACCUMULATED)
+    assert rel(dp.f.cpu().numpy(), 2 * f1) < 1e-14
+    dp.set_f(w["f"])
+    dp.adj(PA.PNFFT_COMPUTE_F)
+    a1 = dp.f_hat.cpu().numpy().copy()
+    dp.adj(PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_ACCUMULATED)
+    assert rel(dp.f_hat.cpu().numpy(), 2 * a1) < 1e-14
+    dp.close()
+
+
+def test_host_pointer_mode(gpu, oracle):
+    """The reference's memory model: library-allocated host arrays written through the getters."""
+    w = make(1, M=500)
+    say("case host pointers")
+    api = PA.lib().d
+    plan = api.init_guru(3, api.arr3(w["N"]), api.arr3(w["n"]), (ctypes.c_double * 3)(0.5, 0.5, 0.5), w["m"],
+                         PA.PNFFT_MALLOC_F_HAT, 0, PA.MPI_COMM_WORLD)
+    nodes = api.init_nodes(w["M"], PA.PNFFT_MALLOC_X | PA.PNFFT_MALLOC_F | PA.PNFFT_MALLOC_GRAD_F)
+    M, Nt = w["M"], int(np.prod(w["N"]))
+
+    def view(ptr, count):
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)), shape=(count,))
+
+    view(api.get_x(nodes), 3 * M)[:] = w["x"].reshape(-1)
+    view(api.get_f_hat(plan), 2 * Nt)[:] = w["f_hat"].view(np.float64)
+    api.trafo(plan, nodes, PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_GRAD_F)
+    f = view(api.get_f(nodes), 2 * M).view(np.complex128).copy()
+    gr = view(api.get_grad_f(nodes), 6 * M).view(np.complex128).copy()
+    P = oracle.Plan(w["N"], w["n"], w["m"], 0)
+    f_o, g_o = P.trafo(w["x"], w["f_hat"], compute_grad=True)
+    assert rel(f, f_o) < 1e-12 and rel(gr, g_o) < 1e-12
+    view(api.get_f(nodes), 2 * M)[:] = w["f"].view(np.float64)
+    api.adj(plan, nodes, PA.PNFFT_COMPUTE_F)
+    a = view(api.get_f_hat(plan), 2 * Nt).view(np.complex128).copy()
+    assert rel(a, P.adj(w["x"], f=w["f"])) < 1e-12
+    assert api.last_error() is None
+    api.free_nodes(nodes, PA.PNFFT_FREE_X | PA.PNFFT_FREE_F | PA.PNFFT_FREE_GRAD_F)
+    api.finalize(plan, PA.PNFFT_FREE_F_HAT)
+
+
+def test_direct_ndft_matches_oracle(gpu, oracle):
+    w = make(1, M=200)
+    say("case direct")
+    rng = SplitMix64(13)
+    gv = rng.complex_normal(3 * w["M"])
+    dp = DevicePlan(w["N"], w["n"], w["m"], 0, w["M"])
+    dp.set_x(w["x"])
+    dp.set_f_hat(w["f_hat"])
+    dp.trafo(PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_GRAD_F | PA.PNFFT_COMPUTE_DIRECT)
+    f_nd, g_nd = oracle.ndft_trafo(w["N"], w["x"], w["f_hat"], compute_grad=True)
+    assert rel(dp.f.cpu().numpy(), f_nd) < 1e-12
+    assert rel(dp.grad.cpu().numpy(), g_nd) < 1e-12
+    dp.set_f(w["f"])
+    dp.set_grad(gv)
+    dp.adj(PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_GRAD_F | PA.PNFFT_COMPUTE_DIRECT)
+    assert rel(dp.f_hat.cpu().numpy(), oracle.ndft_adj(w["N"], w["x"], f=w["f"], grad=gv)) < 1e-12
+    dp.close()
+
+
+def _tile_keys(x, n, m):
+    ts = []
+    for nt in n:
+        t = 8
+        while t > 1 and nt % t:
+            t //= 2
+        ts.append(t)
+    nt_ = [(a + b - 1) // b for a, b in zip(n, ts)]
+    st = (np.floor(np.asarray(n, float) * x) - m).astype(np.int64) % np.asarray(n)
+    tile = ((st[:, 0] // ts[0]) * nt_[1] + st[:, 1] // ts[1]) * nt_[2] + st[:, 2] // ts[2]
+    inn = ((st[:, 0] % ts[0]) * ts[1] + st[:, 1] % ts[1]) * ts[2] + st[:, 2] % ts[2]
+    return (tile * (ts[0] * ts[1] * ts[2]) + inn).astype(np.int64)
+
+
+@pytest.mark.parametrize("cfg,M,precision", [(1, 4096, "double"), (3, 200000, "float"), (5, 100001, "float")])
+def test_binning_bit_exact(gpu, cfg, M, precision):
+    w = make(cfg, M=M)
+    say("case binning", cfg, M)
+    x = w["x"].copy()
+    x[::97] = x[5]  # ties keep input order
+    if precision == "float":
+        x = x.astype(np.float32).astype(np.float64)
+    dp = DevicePlan(w["N"], w["n"], w["m"], w["flags"], M, precision=precision)
+    dp.set_x(x)
+    order, keys = dp.node_bins()
+    order = order.cpu().numpy()
+    keys = keys.cpu().numpy().view(np.uint32).astype(np.int64)
+    k = _tile_keys(x, w["n"], w["m"])
+    ref = np.argsort(k, kind="stable")
+    assert np.array_equal(order, ref)
+    assert np.array_equal(keys, k[ref])
+    dp.close()
+
+
+def test_grad_x_vjp(gpu, oracle):
+    w = make(1, M=600)
+    say("case vjp")
+    rng = SplitMix64(21)
+    wv = rng.complex_normal(w["M"])
+    dp = DevicePlan(w["N"], w["n"], w["m"], 0, w["M"])
+    dp.set_x(w["x"])
+    dp.set_f_hat(w["f_hat"])
+    gx = dp.grad_x_vjp(torch.as_tensor(wv).to("cuda")).cpu().numpy()
+    _, g_o = oracle.Plan(w["N"], w["n"], w["m"], 0).trafo(w["x"], w["f_hat"], compute_grad=True)
+    ref = (np.conj(wv)[:, None] * g_o.reshape(-1, 3)).real
+    assert rel(gx, ref) < 1e-12
+    dp.close()
+
+
+def test_precompute_then_move_nodes(gpu, oracle):
+    """Bins from precompute_psi are reused; set_x invalidates them."""
+    w = make(1, M=1000)
+    say("case precompute")
+    P = oracle.Plan(w["N"], w["n"], w["m"], 0)
+    dp = DevicePlan(w["N"], w["n"], w["m"], 0, w["M"])
+    dp.set_x(w["x"])
+    dp.set_f_hat(w["f_hat"])
+    dp.precompute()
+    dp.trafo(PA.PNFFT_COMPUTE_F)
+    assert rel(dp.f.cpu().numpy(), P.trafo(w["x"], w["f_hat"])[0]) < 1e-12
+    x2 = np.roll(w["x"], 1, axis=1) * 0.9
+    dp.set_x(x2)
+    dp.trafo(PA.PNFFT_COMPUTE_F)
+    assert rel(dp.f.cpu().numpy(), P.trafo(x2, w["f_hat"])[0]) < 1e-12
+    dp.close()

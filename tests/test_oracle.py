@@ -1,0 +1,252 @@
+"""Pins of the FP64 CPU oracle (CPU only).
+
+Each check compares the oracle with something independent of its own
+formulas: a library routine (scipy, mpmath, numpy.fft, numpy's stable sort),
+a closed form, a numerical integral, finite differences, or an invariant
+(adjointness, convergence to the NDFT at the window's accuracy).
+"""
+import mpmath
+import numpy as np
+import pytest
+import scipy.fft
+import scipy.integrate
+import scipy.interpolate
+import scipy.special
+
+from pnfft_amd.workloads import SplitMix64
+
+WINDOWS = {"kb": 0, "gauss": 1 << 13, "bspline": 1 << 14, "sincpow": 1 << 15}
+
+
+# ---------------------------------------------------------------- special functions
+def test_bessel_i0_vs_mpmath_and_scipy(oracle):
+    xs = np.concatenate([np.linspace(0, 45, 181), [0.1, 1e-3, 17.77, 28.27, 39.99, 40.01, 60.0, 120.0, 300.0]])
+    for x in xs:
+        ref = float(mpmath.besseli(0, mpmath.mpf(float(x))))
+        got = oracle.bessel_i0(x)
+        assert abs(got - ref) <= 4e-15 * ref, (x, got, ref)
+        assert abs(got - scipy.special.i0(x)) <= 1e-14 * ref
+
+
+def test_sinc_branches(oracle):
+    for x in [1.0, -2.5, 0.3, 1e-3, 2e-4, 1.3e-4, 1e-6, 1e-9, 1e-17, 0.0]:
+        ref = float(mpmath.sinc(mpmath.mpf(x))) if x != 0 else 1.0
+        assert abs(oracle.sinc(x) - ref) <= 2e-16, x
+    # numpy.sinc(t) = sin(pi t)/(pi t)
+    t = np.linspace(-3.7, 3.7, 41)
+    assert np.allclose([oracle.sinc(np.pi * v) for v in t], np.sinc(t), atol=1e-15, rtol=0)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 6, 8, 9, 12, 16])
+def test_bspline_vs_scipy(oracle, k):
+    basis = scipy.interpolate.BSpline.basis_element(np.arange(k + 1), extrapolate=False)
+    xs = np.linspace(-0.5, k + 0.5, 97)
+    for x in xs:
+        ref = float(np.nan_to_num(basis(x)))
+        assert abs(oracle.bspline(k, x) - ref) <= 1e-14, (k, x)
+    # partition of unity of the integer shifts
+    u = 0.3712
+    assert abs(sum(oracle.bspline(k, u + j) for j in range(k)) - 1.0) < 1e-14
+
+
+# ---------------------------------------------------------------- windows
+@pytest.mark.parametrize("wname", list(WINDOWS))
+@pytest.mark.parametrize("m", [2, 4, 6])
+def test_psi_axis_matches_scalar_window(oracle, wname, m):
+    """psi_axis(s) must be psi(x - l_s/n) at l_s = floor(n x) - m + s (stencil convention)."""
+    flags = WINDOWS[wname]
+    N, n = [16] * 3, [32] * 3
+    w = oracle.window_id(flags)
+    b = oracle.window_b(N, n, m, flags)[0]
+    rng = np.random.default_rng(3)
+    for x in list(rng.uniform(-0.5, 0.5, 25)) + [0.0, -0.5, 3.0 / 32, 0.25 + 1e-12]:
+        nx = 32 * x
+        fl = np.floor(nx)
+        psi, dpsi = oracle.psi_axis(w, 32, b, m, nx)
+        for s in range(2 * m + 1):
+            l = fl - m + s
+            if wname == "kb" and s == 0 and nx - fl > 0:
+                continue  # outside the support: the sin continuation (checked below)
+            assert abs(psi[s] - oracle.psi(w, 32, b, m, x - l / 32)) <= 1e-13 * max(1, abs(psi[s])), (s, x)
+            assert abs(dpsi[s] - oracle.dpsi(w, 32, b, m, x - l / 32)) <= 1e-11 * max(1, abs(dpsi[s])), (s, x)
+
+
+@pytest.mark.parametrize("wname", list(WINDOWS))
+def test_dpsi_is_derivative(oracle, wname):
+    flags = WINDOWS[wname]
+    m = 5
+    w = oracle.window_id(flags)
+    b = oracle.window_b([16] * 3, [32] * 3, m, flags)[0]
+    h = 1e-6
+    for x in np.linspace(-(m - 0.3) / 32, (m - 0.3) / 32, 23):
+        fd = (oracle.psi(w, 32, b, m, x + h) - oracle.psi(w, 32, b, m, x - h)) / (2 * h)
+        d = oracle.dpsi(w, 32, b, m, x)
+        assert abs(fd - d) <= 1e-6 * max(1.0, abs(d)), (wname, x, fd, d)
+
+
+def test_kb_special_values(oracle):
+    """KB at the stencil centre / edge: b/pi at d = 0, sin continuation outside."""
+    m, n = 4, 64
+    b = np.pi * (2 - 1 / 2.0)
+    assert abs(oracle.psi(0, n, b, m, m / n) - b / np.pi) < 1e-15
+    x = (m + 0.4) / n
+    r = np.sqrt((n * x) ** 2 - m * m)
+    assert abs(oracle.psi(0, n, b, m, x) - np.sin(b * r) / (np.pi * r)) < 1e-15
+
+
+@pytest.mark.parametrize("wname", list(WINDOWS))
+def test_phi_hat_is_fourier_transform_of_psi(oracle, wname):
+    """1/inv_phi_hat(k) == n * integral psi(x) e^{-2 pi i k x} dx over the window's support."""
+    flags = WINDOWS[wname]
+    m, N, n = 4, 16, 32
+    w = oracle.window_id(flags)
+    b = oracle.window_b([N] * 3, [n] * 3, m, flags)[0]
+    if wname in ("kb", "bspline"):
+        lim, pts = m / n, None
+    else:
+        lim, pts = 40.0 / n, None
+    for k in [0, 1, 3, 5, 8]:
+        val, _ = scipy.integrate.quad(lambda x: oracle.psi(w, n, b, m, x) * np.cos(2 * np.pi * k * x), -lim, lim,
+                                      limit=400, epsabs=1e-14, epsrel=1e-12, points=pts)
+        inv = oracle.inv_phi_hat(w, k, n, b, m)
+        # quad loses ~1e-9 at KB's square-root edge; a wrong factor or sign is O(1)
+        assert abs(n * val * inv - 1.0) < 1e-8, (wname, k, n * val, 1 / inv)
+
+
+# ---------------------------------------------------------------- NDFT
+def _equispaced_case(N=8, seed=5):
+    rng = np.random.default_rng(seed)
+    fh = rng.normal(size=(N, N, N)) + 1j * rng.normal(size=(N, N, N))
+    l = np.stack(np.meshgrid(*[np.arange(-N // 2, N // 2)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    return fh, l
+
+
+def test_ndft_equispaced_is_fft(oracle):
+    N = 8
+    fh, l = _equispaced_case(N)
+    x = l / N
+    f, _ = oracle.ndft_trafo([N] * 3, x, fh)
+    A = np.zeros((N, N, N), complex)
+    idx = np.arange(-N // 2, N // 2) % N
+    A[np.ix_(idx, idx, idx)] = fh
+    F = scipy.fft.fftn(A)
+    ref = F[l[:, 0] % N, l[:, 1] % N, l[:, 2] % N]
+    assert np.max(np.abs(f - ref)) < 1e-11 * np.max(np.abs(ref))
+    # adjoint on the same nodes: N^3 * ifft
+    g = np.random.default_rng(1).normal(size=len(x)) + 0j
+    a = oracle.ndft_adj([N] * 3, x, f=g)
+    G = np.zeros((N, N, N), complex)
+    G[l[:, 0] % N, l[:, 1] % N, l[:, 2] % N] = g
+    ref_a = (scipy.fft.ifftn(G) * N ** 3)[np.ix_(idx, idx, idx)].reshape(-1)
+    assert np.max(np.abs(a - ref_a)) < 1e-11 * np.max(np.abs(ref_a))
+
+
+def test_ndft_grad_finite_difference(oracle):
+    N = [6, 8, 4]
+    rng = np.random.default_rng(2)
+    fh = rng.normal(size=np.prod(N)) + 1j * rng.normal(size=np.prod(N))
+    x = rng.uniform(-0.5, 0.5, (7, 3))
+    _, g = oracle.ndft_trafo(N, x, fh, compute_grad=True)
+    g = g.reshape(7, 3)
+    h = 1e-6
+    for t in range(3):
+        xp, xm = x.copy(), x.copy()
+        xp[:, t] += h
+        xm[:, t] -= h
+        fd = (oracle.ndft_trafo(N, xp, fh)[0] - oracle.ndft_trafo(N, xm, fh)[0]) / (2 * h)
+        assert np.max(np.abs(fd - g[:, t])) < 1e-6 * np.max(np.abs(g[:, t]))
+
+
+def test_ndft_adjointness(oracle):
+    N = [6, 4, 8]
+    M = 11
+    rng = np.random.default_rng(4)
+    fh = rng.normal(size=np.prod(N)) + 1j * rng.normal(size=np.prod(N))
+    x = rng.uniform(-0.5, 0.5, (M, 3))
+    fv = rng.normal(size=M) + 1j * rng.normal(size=M)
+    gv = rng.normal(size=3 * M) + 1j * rng.normal(size=3 * M)
+    f, g = oracle.ndft_trafo(N, x, fh, compute_grad=True)
+    a = oracle.ndft_adj(N, x, f=fv)
+    ag = oracle.ndft_adj(N, x, grad=gv)
+    assert abs(np.vdot(fv, f) - np.vdot(a, fh)) < 1e-12 * np.linalg.norm(f) * np.linalg.norm(fv)
+    assert abs(np.vdot(gv, g) - np.vdot(ag, fh)) < 1e-12 * np.linalg.norm(g) * np.linalg.norm(gv)
+
+
+# ---------------------------------------------------------------- fast NFFT vs NDFT
+# thresholds: each window's accuracy at sigma = 2, m = 6 (relative l2), with margin
+ACCURACY = {"kb": 1e-10, "gauss": 2e-5, "bspline": 8e-6, "sincpow": 3e-6}
+
+
+@pytest.mark.parametrize("wname", list(WINDOWS))
+def test_fast_nfft_matches_ndft(oracle, wname):
+    N, M = [16, 16, 16], 1024
+    P = oracle.Plan(N, m=6, flags=WINDOWS[wname])
+    rng = SplitMix64(99)
+    x = rng.uniform(3 * M).reshape(M, 3) - 0.5
+    fh = rng.complex_normal(int(np.prod(N)))
+    fv = rng.complex_normal(M)
+    gv = rng.complex_normal(3 * M)
+    f_nd, g_nd = oracle.ndft_trafo(N, x, fh, compute_grad=True)
+    f_nf, g_nf = P.trafo(x, fh, compute_grad=True)
+    tol = ACCURACY[wname]
+    assert oracle.rel_l2(f_nf, f_nd) < tol
+    assert oracle.rel_l2(g_nf, g_nd) < 10 * tol
+    assert oracle.rel_l2(P.adj(x, f=fv), oracle.ndft_adj(N, x, f=fv)) < tol
+    assert oracle.rel_l2(P.adj(x, grad=gv), oracle.ndft_adj(N, x, grad=gv)) < 10 * tol
+
+
+def test_fast_nfft_error_decreases_with_m(oracle):
+    """the m -> m+2 comparison of tests/check_trafo.c:47-48."""
+    N, M = [8, 8, 8], 300
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-0.5, 0.5, (M, 3))
+    fh = rng.normal(size=512) + 1j * rng.normal(size=512)
+    f_nd, _ = oracle.ndft_trafo(N, x, fh)
+    errs = [oracle.rel_l2(oracle.Plan(N, m=m).trafo(x, fh)[0], f_nd) for m in (2, 4, 6, 8)]
+    assert all(b < a / 20 for a, b in zip(errs, errs[1:])), errs
+
+
+def test_fast_nfft_adjointness(oracle):
+    N, M = [8, 16, 8], 500
+    P = oracle.Plan(N, m=4, flags=WINDOWS["gauss"])
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-0.5, 0.5, (M, 3))
+    fh = rng.normal(size=np.prod(N)) + 1j * rng.normal(size=np.prod(N))
+    fv = rng.normal(size=M) + 1j * rng.normal(size=M)
+    f, _ = P.trafo(x, fh)
+    a = P.adj(x, f=fv)
+    assert abs(np.vdot(fv, f) - np.vdot(a, fh)) < 1e-13 * np.linalg.norm(f) * np.linalg.norm(fv)
+
+
+def test_equispaced_round_trip(oracle):
+    """tests/check_vs_pfft.c:115-205: x = k/N, f_hat = 1000/(2g+1) + 1000i/(2g+2), adj(trafo)/N^3 = f_hat."""
+    N = 8
+    g = np.arange(N ** 3)
+    fh = 1000.0 / (2 * g + 1) + 1000j / (2 * g + 2)
+    l = np.stack(np.meshgrid(*[np.arange(-N // 2, N // 2)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = l / N
+    fd, _ = oracle.ndft_trafo([N] * 3, x, fh)
+    back = oracle.ndft_adj([N] * 3, x, f=fd) / N ** 3
+    assert np.max(np.abs(back - fh)) / np.sum(np.abs(fh)) < 1e-15
+    # the NFFT reproduces it to its window error, which falls with m
+    errs = []
+    for m in (4, 6, 8):
+        P = oracle.Plan([N] * 3, m=m)
+        f, _ = P.trafo(x, fh)
+        back = P.adj(x, f=f) / N ** 3
+        errs.append(np.max(np.abs(back - fh)) / np.sum(np.abs(fh)))
+    assert errs[1] < 1e-9 and errs[2] < 1e-13 and errs[0] > errs[1] > errs[2], errs
+
+
+# ---------------------------------------------------------------- sort
+def test_reference_sort_matches_stable_argsort(oracle):
+    n, m = [32, 16, 64], 3
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-0.5, 0.5, (5000, 3))
+    x[::7] = x[3]  # ties
+    keys, perm = oracle.sort_nodes(n, m, x)
+    u = np.floor(np.array(n) * x - m).astype(np.int64) % np.array(n)
+    k = (u[:, 0] * n[1] + u[:, 1]) * n[2] + u[:, 2]
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(perm, order)
+    assert np.array_equal(keys, k[order])
