@@ -236,6 +236,12 @@ struct NodesT {
   unsigned* d_keys = nullptr;
   int* d_tile_start = nullptr;
   long tile_cap = 0;
+  unsigned* d_chunks = nullptr;      // 2 * ntiles: per-tile chunk counts and their offsets
+  unsigned* d_tpartial = nullptr;    // scan partials of the tile counts
+  unsigned char* d_dense = nullptr;  // per-tile LDS-path flag
+  int* d_items = nullptr;            // (tile, chunk) pairs
+  long items_cap = 0;
+  int n_items = 0;
   SortWork sw;
   bool bins_valid = false;
   GridGeom bin_geom{};
@@ -253,11 +259,12 @@ static void tables_compute(PlanT<R>* P) {
     (void)hipMemcpy(P->d_tab, P->h_tab.data(), sizeof(double) * P->h_tab.size(), hipMemcpyHostToDevice);
 }
 
-// Tile layout for binning: tiles of 8 cells per axis (fewer if the grid is small).
-static TileLayout choose_tiles(const long* n) {
+// Tile layout for binning: tiles of 8 cells per axis for m <= 6 and 4 beyond, so
+// that the padded subgrid (TS + 2m)^3 of FP64 accumulators fits in 160 KB of LDS.
+static TileLayout choose_tiles(const long* n, int m) {
   TileLayout T{};
   for (int t = 0; t < 3; t++) {
-    int ts = 8;
+    int ts = m <= 6 ? 8 : 4;
     while (ts > 1 && n[t] % ts) ts >>= 1;
     T.ts[t] = ts;
     T.ntile[t] = (int)((n[t] + ts - 1) / ts);
@@ -346,7 +353,7 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
     P->geom.n[t] = (int)n[t];
     P->geom.b[t] = P->b[t];
   }
-  P->tiles = choose_tiles(P->n);
+  P->tiles = choose_tiles(P->n, m);
 
   bool ok = hipStreamCreate(&P->stream) == hipSuccess;
   int Ni[3] = {(int)N[0], (int)N[1], (int)N[2]}, ni[3] = {(int)n[0], (int)n[1], (int)n[2]};
@@ -383,6 +390,12 @@ static void nodes_release_device(NodesT<R>* D) {
   dfree(D->d_perm);
   dfree(D->d_keys);
   dfree(D->d_tile_start);
+  dfree(D->d_chunks);
+  dfree(D->d_tpartial);
+  dfree(D->d_dense);
+  dfree(D->d_items);
+  D->items_cap = 0;
+  D->n_items = 0;
   dfree(D->sw.keys[0]);
   dfree(D->sw.keys[1]);
   dfree(D->sw.vals[0]);
@@ -461,11 +474,46 @@ static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
   }
   if (D->tile_cap < T.ntiles() + 1) {
     dfree(D->d_tile_start);
+    dfree(D->d_chunks);
+    dfree(D->d_tpartial);
+    dfree(D->d_dense);
     D->d_tile_start = dalloc<int>(T.ntiles() + 1);
-    if (!D->d_tile_start) return false;
+    D->d_chunks = dalloc<unsigned>(2 * T.ntiles());
+    D->d_tpartial = dalloc<unsigned>((T.ntiles() + 4095) / 4096 + 1);
+    D->d_dense = dalloc<unsigned char>(T.ntiles());
+    if (!D->d_tile_start || !D->d_chunks || !D->d_tpartial || !D->d_dense) return false;
     D->tile_cap = T.ntiles() + 1;
   }
+  const long icap = T.ntiles() + (M + kTileNodes - 1) / kTileNodes + 1;
+  if (D->items_cap < icap) {
+    dfree(D->d_items);
+    D->d_items = dalloc<int>(2 * icap);
+    if (!D->d_items) return false;
+    D->items_cap = icap;
+  }
   return true;
+}
+
+// Minimum node count for a tile to take the LDS path (tunable by PNFFT_AMD_DENSE_MIN).
+static int dense_min() {
+  static int v = [] {
+    const char* e = getenv("PNFFT_AMD_DENSE_MIN");
+    return e ? atoi(e) : 8;
+  }();
+  return v;
+}
+
+template <class R>
+static void set_tiles(BArgs<R>& A, const NodesT<R>* D) {
+  A.tile_start = D->d_tile_start;
+  A.keys = D->d_keys;
+  A.tile_dense = D->d_dense;
+  A.items = D->d_items;
+  A.n_items = D->n_items;
+  for (int t = 0; t < 3; t++) {
+    A.ts[t] = D->bin_tiles.ts[t];
+    A.ntile[t] = D->bin_tiles.ntile[t];
+  }
 }
 
 static bool same_geom(const GridGeom& a, const GridGeom& b) {
@@ -483,6 +531,16 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx) {
   if (st != kOk) {
     set_error("node binning failed (%d)", st);
     return st;
+  }
+  D->n_items = 0;
+  if (P->m >= 2 && P->m <= 8) {
+    const int ni = build_items(D->d_tile_start, P->tiles.ntiles(), dense_min(), D->d_chunks, D->d_tpartial,
+                               D->d_dense, D->d_items, D->items_cap, P->stream);
+    if (ni < 0) {
+      set_error("building the tile work list failed");
+      return kHipError;
+    }
+    D->n_items = ni;
   }
   D->bin_geom = P->geom;
   D->bin_tiles = P->tiles;
@@ -572,7 +630,7 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
       A.grad = dg;
       A.accumulate = accumulate;
       A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
-      A.tile_start = D->d_tile_start;
+      set_tiles(A, D);
       int mode = (want_f && want_g) ? B_FGRAD : (want_f ? B_F : B_GRAD);
       P->evt.rec(4, s);
       if (gather<R>(A, mode, s) != kOk) set_error("gather launch failed");
@@ -646,7 +704,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
         A.grid = P->d_grid;
         A.f = df;
         A.grad = dg;
-        A.tile_start = D->d_tile_start;
+        set_tiles(A, D);
         const int mode = (use_f && use_g) ? B_FGRAD : (use_f ? B_F : B_GRAD);
         P->evt.rec(4, s);
         if (scatter<R>(A, mode, s) != kOk) set_error("scatter launch failed");
@@ -706,7 +764,7 @@ static void vjp_impl(PlanT<R>* P, NodesT<R>* D, const Cplx<R>* w, R* gx) {
   A.grid = P->d_grid;
   A.w = dw;
   A.gx = dgx;
-  A.tile_start = D->d_tile_start;
+  set_tiles(A, D);
   if (gather<R>(A, B_VJP, s) != kOk) set_error("gather launch failed");
   if (host_gx) (void)hipMemcpyAsync(gx, dgx, sizeof(R) * 3 * M, hipMemcpyDeviceToHost, s);
   if (hipStreamSynchronize(s) != hipSuccess) set_error("vjp: HIP error %s", hipGetErrorString(hipGetLastError()));

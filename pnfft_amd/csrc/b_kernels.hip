@@ -7,6 +7,18 @@
 //
 // Reference: the node loops of kernel/ndft-parallel.c:2703-3009 and the
 // stencil kernels of kernel/assign.c:478-1066 (cutoff 2m+1, :910).
+//
+// Two paths:
+//  * tiled: one block per work item (tile, chunk of <= kTileNodes sorted
+//    nodes).  The tile's padded subgrid ((TS + 2m) cells per axis, starting at
+//    the tile origin because tiles are keyed by the stencil start) is staged in
+//    LDS; the gather reads every stencil point from LDS, the scatter
+//    accumulates into LDS in FP64 (ds_add_f64) and adds the nonzero cells of
+//    the subgrid to the grid once (global float atomics), so overlapping halos
+//    of neighbouring tiles meet in L2 instead of per stencil point.
+//  * direct: one thread per node, stencil read from / added to the grid in
+//    HBM; used for nodes of sparse tiles and for any node that does not lie in
+//    the tile it was binned to (bins older than the node coordinates).
 #include "b_common.hpp"
 #include "device_common.hpp"
 #include "nfft_b.hpp"
@@ -16,129 +28,135 @@ namespace pnfft_amd {
 
 namespace {
 
-constexpr int kB = 256;  // threads per block
+constexpr int kB = 256;  // threads per block, direct kernels
 
-// ---------------------------------------------------------------------------
-// direct gather: one thread per node, stencil read from the grid in HBM/L2
-// ---------------------------------------------------------------------------
-template <class R, int MT, int MODE>
-__global__ __launch_bounds__(kB) void gather_direct_k(BArgs<R> A) {
-  const long p = (long)blockIdx.x * kB + threadIdx.x;
-  if (p >= A.M) return;
-  const long j = A.perm ? (long)A.perm[p] : p;
-  const int m = MT > 0 ? MT : A.g.m;
-  constexpr int CM = MT > 0 ? 2 * MT + 1 : kMaxC;
-  const int c = 2 * m + 1;
-  constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
-  constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
+template <class R, int CM>
+struct NodeWin {
+  int start[3];  // natural index of stencil point 0 per axis
+  R px[CM], py[CM], pz[CM];
+  R dx[CM], dy[CM], dz[CM];
+};
+
+template <class R, int CM, bool WG>
+__device__ inline void node_windows(const BArgs<R>& A, long j, int m, NodeWin<R, CM>& w) {
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
-
-  R px[CM], py[CM], pz[CM], dx[CM], dy[CM], dz[CM];
   const NodeAxis<R> a0 = node_axis<R>(A.x[3 * j + 0], n0, m);
   const NodeAxis<R> a1 = node_axis<R>(A.x[3 * j + 1], n1, m);
   const NodeAxis<R> a2 = node_axis<R>(A.x[3 * j + 2], n2, m);
-  window_axis<R>(A.g.win, m, a0.frac, (R)n0, (R)A.g.b[0], px, WG ? dx : nullptr);
-  window_axis<R>(A.g.win, m, a1.frac, (R)n1, (R)A.g.b[1], py, WG ? dy : nullptr);
-  window_axis<R>(A.g.win, m, a2.frac, (R)n2, (R)A.g.b[2], pz, WG ? dz : nullptr);
+  w.start[0] = a0.start;
+  w.start[1] = a1.start;
+  w.start[2] = a2.start;
+  window_axis<R>(A.g.win, m, a0.frac, (R)n0, (R)A.g.b[0], w.px, WG ? w.dx : nullptr);
+  window_axis<R>(A.g.win, m, a1.frac, (R)n1, (R)A.g.b[1], w.py, WG ? w.dy : nullptr);
+  window_axis<R>(A.g.win, m, a2.frac, (R)n2, (R)A.g.b[2], w.pz, WG ? w.dz : nullptr);
+}
 
-  int i2s[CM];
-#pragma unroll
-  for (int s = 0; s < CM; s++) {
-    if (s < c) {
-      int v = a2.start + s;
-      i2s[s] = v >= n2 ? v - n2 : v;
-    }
-  }
-
+// Separable gather sum f = sum_a px sum_b py sum_c pz G (and the three
+// gradient sums with one factor replaced by its derivative).
+// ROW(a, b) returns a pointer p with p[IDX(c)] the grid value at stencil (a,b,c).
+template <class R, int CM, int MODE, class RowFn, class IdxFn>
+__device__ inline void gather_sum(const NodeWin<R, CM>& w, int c, RowFn row_of, IdxFn idx, Cplx<R>* out) {
+  constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
   Cplx<R> f = cmk<R>(0, 0), g0 = cmk<R>(0, 0), g1 = cmk<R>(0, 0), g2 = cmk<R>(0, 0);
   for (int a = 0; a < c; a++) {
-    int i0 = a0.start + a;
-    if (i0 >= n0) i0 -= n0;
     Cplx<R> ha = cmk<R>(0, 0), hay = cmk<R>(0, 0), haz = cmk<R>(0, 0);
     for (int b = 0; b < c; b++) {
-      int i1 = a1.start + b;
-      if (i1 >= n1) i1 -= n1;
-      const Cplx<R>* row = A.grid + ((long)i0 * n1 + i1) * n2;
+      const Cplx<R>* row = row_of(a, b);
       Cplx<R> hb = cmk<R>(0, 0), hbz = cmk<R>(0, 0);
 #pragma unroll
       for (int s = 0; s < CM; s++) {
         if (s < c) {
-          const Cplx<R> gv = row[i2s[s]];
-          hb.x += pz[s] * gv.x;
-          hb.y += pz[s] * gv.y;
+          const Cplx<R> gv = row[idx(s)];
+          hb.x += w.pz[s] * gv.x;
+          hb.y += w.pz[s] * gv.y;
           if (WG) {
-            hbz.x += dz[s] * gv.x;
-            hbz.y += dz[s] * gv.y;
+            hbz.x += w.dz[s] * gv.x;
+            hbz.y += w.dz[s] * gv.y;
           }
         }
       }
-      ha.x += py[b] * hb.x;
-      ha.y += py[b] * hb.y;
+      ha.x += w.py[b] * hb.x;
+      ha.y += w.py[b] * hb.y;
       if (WG) {
-        hay.x += dy[b] * hb.x;
-        hay.y += dy[b] * hb.y;
-        haz.x += py[b] * hbz.x;
-        haz.y += py[b] * hbz.y;
+        hay.x += w.dy[b] * hb.x;
+        hay.y += w.dy[b] * hb.y;
+        haz.x += w.py[b] * hbz.x;
+        haz.y += w.py[b] * hbz.y;
       }
     }
-    f.x += px[a] * ha.x;
-    f.y += px[a] * ha.y;
+    f.x += w.px[a] * ha.x;
+    f.y += w.px[a] * ha.y;
     if (WG) {
-      g0.x += dx[a] * ha.x;
-      g0.y += dx[a] * ha.y;
-      g1.x += px[a] * hay.x;
-      g1.y += px[a] * hay.y;
-      g2.x += px[a] * haz.x;
-      g2.y += px[a] * haz.y;
+      g0.x += w.dx[a] * ha.x;
+      g0.y += w.dx[a] * ha.y;
+      g1.x += w.px[a] * hay.x;
+      g1.y += w.px[a] * hay.y;
+      g2.x += w.px[a] * haz.x;
+      g2.y += w.px[a] * haz.y;
     }
   }
-  if (WF) {
+  out[0] = f;
+  out[1] = g0;
+  out[2] = g1;
+  out[3] = g2;
+}
+
+template <class R, int MODE>
+__device__ inline void gather_store(const BArgs<R>& A, long j, Cplx<R>* v) {
+  if (MODE == B_F || MODE == B_FGRAD) {
     Cplx<R> o = A.accumulate ? A.f[j] : cmk<R>(0, 0);
+    Cplx<R> f = v[0];
     if (A.real_only) f.y = 0;  // assign_f_r2r reads the real grid part only (kernel/assign.c:715-739)
     A.f[j] = cadd(o, f);
   }
   if (MODE == B_GRAD || MODE == B_FGRAD) {
     Cplx<R>* gp = A.grad + 3 * j;
     if (A.accumulate) {
-      g0 = cadd(g0, gp[0]);
-      g1 = cadd(g1, gp[1]);
-      g2 = cadd(g2, gp[2]);
+      v[1] = cadd(v[1], gp[0]);
+      v[2] = cadd(v[2], gp[1]);
+      v[3] = cadd(v[3], gp[2]);
     }
-    gp[0] = g0;
-    gp[1] = g1;
-    gp[2] = g2;
+    gp[0] = v[1];
+    gp[1] = v[2];
+    gp[2] = v[3];
   }
   if (MODE == B_VJP) {
-    const Cplx<R> w = A.w[j];
-    A.gx[3 * j + 0] = w.x * g0.x + w.y * g0.y;
-    A.gx[3 * j + 1] = w.x * g1.x + w.y * g1.y;
-    A.gx[3 * j + 2] = w.x * g2.x + w.y * g2.y;
+    const Cplx<R> wv = A.w[j];
+    A.gx[3 * j + 0] = wv.x * v[1].x + wv.y * v[1].y;
+    A.gx[3 * j + 1] = wv.x * v[2].x + wv.y * v[2].y;
+    A.gx[3 * j + 2] = wv.x * v[3].x + wv.y * v[3].y;
   }
 }
 
-// ---------------------------------------------------------------------------
-// direct scatter: one thread per node, hardware float atomics into the grid
-// ---------------------------------------------------------------------------
-template <class R, int MT, int MODE>
-__global__ __launch_bounds__(kB) void scatter_direct_k(BArgs<R> A) {
-  const long p = (long)blockIdx.x * kB + threadIdx.x;
-  if (p >= A.M) return;
-  const long j = A.perm ? (long)A.perm[p] : p;
-  const int m = MT > 0 ? MT : A.g.m;
-  constexpr int CM = MT > 0 ? 2 * MT + 1 : kMaxC;
-  const int c = 2 * m + 1;
+// gather one node straight from the grid (periodic wrap on every axis)
+template <class R, int CM, int MODE>
+__device__ inline void gather_node_global(const BArgs<R>& A, long j, const NodeWin<R, CM>& w, int c) {
+  const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
+  int i2s[CM];
+#pragma unroll
+  for (int s = 0; s < CM; s++) {
+    const int v = w.start[2] + s;
+    i2s[s] = v >= n2 ? v - n2 : v;
+  }
+  const Cplx<R>* grid = A.grid;
+  auto row_of = [&](int a, int b) {
+    int i0 = w.start[0] + a;
+    if (i0 >= n0) i0 -= n0;
+    int i1 = w.start[1] + b;
+    if (i1 >= n1) i1 -= n1;
+    return grid + ((long)i0 * n1 + i1) * n2;
+  };
+  auto idx = [&](int s) { return i2s[s]; };
+  Cplx<R> v[4];
+  gather_sum<R, CM, MODE>(w, c, row_of, idx, v);
+  gather_store<R, MODE>(A, j, v);
+}
+
+template <class R, int CM, int MODE>
+__device__ inline void scatter_node_global(const BArgs<R>& A, long j, const NodeWin<R, CM>& w, int c) {
   constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
-
-  R px[CM], py[CM], pz[CM], dx[CM], dy[CM], dz[CM];
-  const NodeAxis<R> a0 = node_axis<R>(A.x[3 * j + 0], n0, m);
-  const NodeAxis<R> a1 = node_axis<R>(A.x[3 * j + 1], n1, m);
-  const NodeAxis<R> a2 = node_axis<R>(A.x[3 * j + 2], n2, m);
-  window_axis<R>(A.g.win, m, a0.frac, (R)n0, (R)A.g.b[0], px, WG ? dx : nullptr);
-  window_axis<R>(A.g.win, m, a1.frac, (R)n1, (R)A.g.b[1], py, WG ? dy : nullptr);
-  window_axis<R>(A.g.win, m, a2.frac, (R)n2, (R)A.g.b[2], pz, WG ? dz : nullptr);
-
   Cplx<R> fv = cmk<R>(0, 0), gv0 = cmk<R>(0, 0), gv1 = cmk<R>(0, 0), gv2 = cmk<R>(0, 0);
   if (WF) fv = A.f[j];
   if (WG) {
@@ -147,36 +165,223 @@ __global__ __launch_bounds__(kB) void scatter_direct_k(BArgs<R> A) {
     gv2 = A.grad[3 * j + 2];
   }
   for (int a = 0; a < c; a++) {
-    int i0 = a0.start + a;
+    int i0 = w.start[0] + a;
     if (i0 >= n0) i0 -= n0;
     for (int b = 0; b < c; b++) {
-      int i1 = a1.start + b;
+      int i1 = w.start[1] + b;
       if (i1 >= n1) i1 -= n1;
       Cplx<R>* row = A.grid + ((long)i0 * n1 + i1) * n2;
-      const R pxy = px[a] * py[b];
+      const R pxy = w.px[a] * w.py[b];
       R pdxy = 0, pxdy = 0;
       if (WG) {
-        pdxy = dx[a] * py[b];
-        pxdy = px[a] * dy[b];
+        pdxy = w.dx[a] * w.py[b];
+        pxdy = w.px[a] * w.dy[b];
       }
       for (int s = 0; s < c; s++) {
-        int i2 = a2.start + s;
+        int i2 = w.start[2] + s;
         if (i2 >= n2) i2 -= n2;
         R vr = 0, vi = 0;
         if (WF) {
-          const R w = pxy * pz[s];
-          vr += w * fv.x;
-          vi += w * fv.y;
+          const R q = pxy * w.pz[s];
+          vr += q * fv.x;
+          vi += q * fv.y;
         }
         if (WG) {
-          const R w0 = pdxy * pz[s], w1 = pxdy * pz[s], w2 = pxy * dz[s];
-          vr += w0 * gv0.x + w1 * gv1.x + w2 * gv2.x;
-          vi += w0 * gv0.y + w1 * gv1.y + w2 * gv2.y;
+          const R q0 = pdxy * w.pz[s], q1 = pxdy * w.pz[s], q2 = pxy * w.dz[s];
+          vr += q0 * gv0.x + q1 * gv1.x + q2 * gv2.x;
+          vi += q0 * gv0.y + q1 * gv1.y + q2 * gv2.y;
         }
         atomicAdd(&row[i2].x, vr);
         atomicAdd(&row[i2].y, vi);
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// direct kernels: positions [0, M) of the sorted order; with tiles present,
+// only the nodes of sparse tiles (tile_dense[tile] == 0) are handled here.
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ inline bool skip_dense(const BArgs<R>& A, long p) {
+  if (!A.tile_dense) return false;
+  const unsigned tile = A.keys[p] / (unsigned)(A.ts[0] * A.ts[1] * A.ts[2]);
+  return A.tile_dense[tile] != 0;
+}
+
+template <class R, int MT, int MODE>
+__global__ __launch_bounds__(kB) void gather_direct_k(BArgs<R> A) {
+  const long p = (long)blockIdx.x * kB + threadIdx.x;
+  if (p >= A.M) return;
+  if (skip_dense(A, p)) return;
+  const long j = A.perm ? (long)A.perm[p] : p;
+  const int m = MT > 0 ? MT : A.g.m;
+  constexpr int CM = MT > 0 ? 2 * MT + 1 : kMaxC;
+  constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
+  NodeWin<R, CM> w;
+  node_windows<R, CM, WG>(A, j, m, w);
+  gather_node_global<R, CM, MODE>(A, j, w, 2 * m + 1);
+}
+
+template <class R, int MT, int MODE>
+__global__ __launch_bounds__(kB) void scatter_direct_k(BArgs<R> A) {
+  const long p = (long)blockIdx.x * kB + threadIdx.x;
+  if (p >= A.M) return;
+  if (skip_dense(A, p)) return;
+  const long j = A.perm ? (long)A.perm[p] : p;
+  const int m = MT > 0 ? MT : A.g.m;
+  constexpr int CM = MT > 0 ? 2 * MT + 1 : kMaxC;
+  constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
+  NodeWin<R, CM> w;
+  node_windows<R, CM, WG>(A, j, m, w);
+  scatter_node_global<R, CM, MODE>(A, j, w, 2 * m + 1);
+}
+
+// ---------------------------------------------------------------------------
+// tiled kernels
+// ---------------------------------------------------------------------------
+struct TileItem {
+  int tile;
+  int o[3];      // tile origin (natural grid index)
+  int P[3];      // padded extent TS + 2m
+  int beg, end;  // node range in the sorted order
+};
+
+template <class R>
+__device__ inline TileItem tile_item(const BArgs<R>& A, int m) {
+  TileItem it;
+  const int t = A.items[2 * blockIdx.x];
+  const int chunk = A.items[2 * blockIdx.x + 1];
+  it.tile = t;
+  const int t2 = t % A.ntile[2];
+  const int t1 = (t / A.ntile[2]) % A.ntile[1];
+  const int t0 = t / (A.ntile[2] * A.ntile[1]);
+  it.o[0] = t0 * A.ts[0];
+  it.o[1] = t1 * A.ts[1];
+  it.o[2] = t2 * A.ts[2];
+  for (int d = 0; d < 3; d++) it.P[d] = A.ts[d] + 2 * m;
+  const int s = A.tile_start[t], e = A.tile_start[t + 1];
+  it.beg = s + chunk * kTileNodes;
+  it.end = min(e, it.beg + kTileNodes);
+  return it;
+}
+
+__device__ inline int wrapi(int v, int n) { return v >= n ? v - n : v; }
+
+// local offset of the node's stencil start inside the tile, or -1 if outside
+__device__ inline int local_off(int start, int o, int ts, int n) {
+  int l = start - o;
+  if (l < 0) l += n;
+  return (l < ts) ? l : -1;
+}
+
+template <class R, int MT, int MODE>
+__global__ __launch_bounds__(kTileThreads) void gather_tiled_k(BArgs<R> A) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<R>* tile = reinterpret_cast<Cplx<R>*>(smem_raw);
+  constexpr int m = MT;
+  constexpr int CM = 2 * MT + 1;
+  constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
+  const TileItem it = tile_item(A, m);
+  const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
+  const int P0 = it.P[0], P1 = it.P[1], P2 = it.P[2];
+  const int vol = P0 * P1 * P2;
+  for (int idx = threadIdx.x; idx < vol; idx += kTileThreads) {
+    const int cz = idx % P2, cy = (idx / P2) % P1, cx = idx / (P1 * P2);
+    const int g0 = wrapi(it.o[0] + cx, n0), g1 = wrapi(it.o[1] + cy, n1), g2 = wrapi(it.o[2] + cz, n2);
+    tile[idx] = A.grid[((long)g0 * n1 + g1) * n2 + g2];
+  }
+  __syncthreads();
+  for (int p = it.beg + threadIdx.x; p < it.end; p += kTileThreads) {
+    const long j = A.perm[p];
+    NodeWin<R, CM> w;
+    node_windows<R, CM, WG>(A, j, m, w);
+    const int l0 = local_off(w.start[0], it.o[0], A.ts[0], n0);
+    const int l1 = local_off(w.start[1], it.o[1], A.ts[1], n1);
+    const int l2 = local_off(w.start[2], it.o[2], A.ts[2], n2);
+    if (l0 < 0 || l1 < 0 || l2 < 0) {
+      gather_node_global<R, CM, MODE>(A, j, w, CM);
+      continue;
+    }
+    const Cplx<R>* base = tile + (l0 * P1 + l1) * P2 + l2;
+    auto row_of = [&](int a, int b) { return base + (a * P1 + b) * P2; };
+    auto idx = [](int s) { return s; };
+    Cplx<R> v[4];
+    gather_sum<R, CM, MODE>(w, CM, row_of, idx, v);
+    gather_store<R, MODE>(A, j, v);
+  }
+}
+
+template <class R, int MT, int MODE>
+__global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* acc = reinterpret_cast<double*>(smem_raw);  // interleaved (re, im) FP64 accumulators
+  constexpr int m = MT;
+  constexpr int CM = 2 * MT + 1;
+  constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
+  constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
+  const TileItem it = tile_item(A, m);
+  const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
+  const int P1 = it.P[1], P2 = it.P[2];
+  const int vol = it.P[0] * P1 * P2;
+  for (int idx = threadIdx.x; idx < 2 * vol; idx += kTileThreads) acc[idx] = 0.0;
+  __syncthreads();
+  for (int p = it.beg + threadIdx.x; p < it.end; p += kTileThreads) {
+    const long j = A.perm[p];
+    NodeWin<R, CM> w;
+    node_windows<R, CM, WG>(A, j, m, w);
+    const int l0 = local_off(w.start[0], it.o[0], A.ts[0], n0);
+    const int l1 = local_off(w.start[1], it.o[1], A.ts[1], n1);
+    const int l2 = local_off(w.start[2], it.o[2], A.ts[2], n2);
+    if (l0 < 0 || l1 < 0 || l2 < 0) {
+      scatter_node_global<R, CM, MODE>(A, j, w, CM);
+      continue;
+    }
+    Cplx<R> fv = cmk<R>(0, 0), gv0 = cmk<R>(0, 0), gv1 = cmk<R>(0, 0), gv2 = cmk<R>(0, 0);
+    if (WF) fv = A.f[j];
+    if (WG) {
+      gv0 = A.grad[3 * j + 0];
+      gv1 = A.grad[3 * j + 1];
+      gv2 = A.grad[3 * j + 2];
+    }
+    double* base = acc + 2 * ((l0 * P1 + l1) * P2 + l2);
+    for (int a = 0; a < CM; a++) {
+      for (int b = 0; b < CM; b++) {
+        double* row = base + 2 * ((a * P1 + b) * P2);
+        const R pxy = w.px[a] * w.py[b];
+        R pdxy = 0, pxdy = 0;
+        if (WG) {
+          pdxy = w.dx[a] * w.py[b];
+          pxdy = w.px[a] * w.dy[b];
+        }
+#pragma unroll
+        for (int s = 0; s < CM; s++) {
+          R vr = 0, vi = 0;
+          if (WF) {
+            const R q = pxy * w.pz[s];
+            vr += q * fv.x;
+            vi += q * fv.y;
+          }
+          if (WG) {
+            const R q0 = pdxy * w.pz[s], q1 = pxdy * w.pz[s], q2 = pxy * w.dz[s];
+            vr += q0 * gv0.x + q1 * gv1.x + q2 * gv2.x;
+            vi += q0 * gv0.y + q1 * gv1.y + q2 * gv2.y;
+          }
+          atomicAdd(&row[2 * s], (double)vr);
+          atomicAdd(&row[2 * s + 1], (double)vi);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < vol; idx += kTileThreads) {
+    const double vr = acc[2 * idx], vi = acc[2 * idx + 1];
+    if (vr == 0.0 && vi == 0.0) continue;
+    const int cz = idx % P2, cy = (idx / P2) % P1, cx = idx / (P1 * P2);
+    const int g0 = wrapi(it.o[0] + cx, n0), g1 = wrapi(it.o[1] + cy, n1), g2 = wrapi(it.o[2] + cz, n2);
+    Cplx<R>* cell = A.grid + ((long)g0 * n1 + g1) * n2 + g2;
+    atomicAdd(&cell->x, (R)vr);
+    atomicAdd(&cell->y, (R)vi);
   }
 }
 
@@ -186,12 +391,35 @@ inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 template <class R, int KIND, int MODE>
 int b_launch(const BArgs<R>& A, hipStream_t s) {
+  constexpr int SMODE = MODE > 3 ? 3 : MODE;
   const dim3 grid(grid1d(A.M, kB)), block(kB);
+  const bool tiled = A.items != nullptr && A.n_items > 0 && A.g.m >= 2 && A.g.m <= 8;
+  size_t lds = 0;
+  if (tiled) {
+    const long vol = (long)(A.ts[0] + 2 * A.g.m) * (A.ts[1] + 2 * A.g.m) * (A.ts[2] + 2 * A.g.m);
+    lds = (size_t)vol * (KIND == 0 ? sizeof(Cplx<R>) : 16);
+  }
+  const dim3 tgrid((unsigned)(tiled ? A.n_items : 1)), tblock(kTileThreads);
+  BArgs<R> D = A;
+  if (!tiled) D.tile_dense = nullptr;  // no tiled pass: the direct kernel takes every node
   switch (A.g.m) {
-#define PNFFT_B_CASE(MV)                                                                       \
-  case MV:                                                                                     \
-    if (KIND == 0) hipLaunchKernelGGL((gather_direct_k<R, MV, MODE>), grid, block, 0, s, A);   \
-    else hipLaunchKernelGGL((scatter_direct_k<R, MV, (MODE > 3 ? 3 : MODE)>), grid, block, 0, s, A); \
+#define PNFFT_B_CASE(MV)                                                                                  \
+  case MV:                                                                                                \
+    if (KIND == 0) {                                                                                      \
+      if (tiled) {                                                                                        \
+        (void)hipFuncSetAttribute((const void*)gather_tiled_k<R, MV, MODE>,                               \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                  \
+        hipLaunchKernelGGL((gather_tiled_k<R, MV, MODE>), tgrid, tblock, lds, s, A);                      \
+      }                                                                                                   \
+      hipLaunchKernelGGL((gather_direct_k<R, MV, MODE>), grid, block, 0, s, D);                           \
+    } else {                                                                                              \
+      if (tiled) {                                                                                        \
+        (void)hipFuncSetAttribute((const void*)scatter_tiled_k<R, MV, SMODE>,                             \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                  \
+        hipLaunchKernelGGL((scatter_tiled_k<R, MV, SMODE>), tgrid, tblock, lds, s, A);                    \
+      }                                                                                                   \
+      hipLaunchKernelGGL((scatter_direct_k<R, MV, SMODE>), grid, block, 0, s, D);                         \
+    }                                                                                                     \
     break;
     PNFFT_B_CASE(2)
     PNFFT_B_CASE(3)
@@ -201,10 +429,12 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
     PNFFT_B_CASE(7)
     PNFFT_B_CASE(8)
 #undef PNFFT_B_CASE
-    default:
-      if (KIND == 0) hipLaunchKernelGGL((gather_direct_k<R, 0, MODE>), grid, block, 0, s, A);
-      else hipLaunchKernelGGL((scatter_direct_k<R, 0, (MODE > 3 ? 3 : MODE)>), grid, block, 0, s, A);
+    default: {
+      D.tile_dense = nullptr;
+      if (KIND == 0) hipLaunchKernelGGL((gather_direct_k<R, 0, MODE>), grid, block, 0, s, D);
+      else hipLaunchKernelGGL((scatter_direct_k<R, 0, SMODE>), grid, block, 0, s, D);
       break;
+    }
   }
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }

@@ -194,6 +194,27 @@ __global__ __launch_bounds__(kB) void tile_bounds_k(const unsigned* __restrict__
   for (long tt = tprev + 1; tt <= t; tt++) tile_start[tt] = (int)i;
 }
 
+__global__ __launch_bounds__(kB) void tile_chunks_k(const int* __restrict__ tile_start, long ntiles, int dense_min,
+                                                    unsigned* chunks, unsigned char* dense) {
+  const long t = (long)blockIdx.x * kB + threadIdx.x;
+  if (t >= ntiles) return;
+  const int cnt = tile_start[t + 1] - tile_start[t];
+  const bool d = cnt >= dense_min && cnt > 0;
+  dense[t] = d ? 1 : 0;
+  chunks[t] = d ? (unsigned)((cnt + kTileNodes - 1) / kTileNodes) : 0u;
+}
+
+__global__ __launch_bounds__(kB) void tile_items_k(const unsigned* __restrict__ chunks,
+                                                   const unsigned* __restrict__ offs, long ntiles, int* items) {
+  const long t = (long)blockIdx.x * kB + threadIdx.x;
+  if (t >= ntiles) return;
+  const unsigned c = chunks[t], o = offs[t];
+  for (unsigned k = 0; k < c; k++) {
+    items[2 * (o + k)] = (int)t;
+    items[2 * (o + k) + 1] = (int)k;
+  }
+}
+
 template <class R>
 __global__ void fill_zero_k(Cplx<R>* p, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -245,6 +266,30 @@ int bin_nodes(const R* x, long M, const GridGeom& g, const TileLayout& T, SortWo
   hipLaunchKernelGGL(tile_bounds_k, dim3(grid1d(M, kB)), dim3(kB), 0, s, keys_out, M, (unsigned)T.tile_vol(),
                      tile_start);
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
+int build_items(const int* tile_start, long ntiles, int dense_min, unsigned* chunks, unsigned* partial,
+                unsigned char* tile_dense, int* items, long items_cap, hipStream_t s) {
+  hipLaunchKernelGGL(tile_chunks_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, dense_min, chunks,
+                     tile_dense);
+  unsigned last_cnt = 0, last_off = 0;
+  if (hipMemcpyAsync(&last_cnt, chunks + ntiles - 1, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return -1;
+  // chunks -> exclusive offsets in place (a copy of the last count was taken above)
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  unsigned* offs = chunks;
+  // keep counts: scan a copy held in the second half of the chunk buffer
+  if (hipMemcpyAsync(offs + ntiles, chunks, sizeof(unsigned) * ntiles, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return -1;
+  scan_exclusive(offs + ntiles, ntiles, partial, s);
+  if (hipMemcpyAsync(&last_off, offs + 2 * ntiles - 1, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return -1;
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  const long total = (long)last_off + (long)last_cnt;
+  if (total > items_cap) return -1;
+  hipLaunchKernelGGL(tile_items_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, chunks, offs + ntiles, ntiles, items);
+  if (hipGetLastError() != hipSuccess) return -1;
+  return (int)total;
 }
 
 template <class R>

@@ -25,11 +25,22 @@ struct BArgs {
   int real_only;           // gather: PNFFT_REAL_F, write the real part only
   // tiled path
   const int* tile_start;   // [ntiles+1] offsets into perm
-  const int* items;        // work items: packed (tile id, chunk)
+  const unsigned* keys;    // sorted bin keys [M]
+  const unsigned char* tile_dense;  // [ntiles] 1 if the tile's nodes go through the LDS path
+  const int* items;        // work items: pairs (tile id, chunk index)
   int n_items;
   int ts[3];               // tile size per axis
   int ntile[3];            // tiles per axis
 };
+
+constexpr int kTileThreads = 512;  // threads per block of the tiled kernels
+constexpr int kTileNodes = 512;    // nodes per work item
+
+// Work items of the tiled kernels: every tile with at least dense_min nodes is
+// cut into chunks of kTileNodes sorted nodes.  Writes tile_dense[ntiles],
+// items[2 * n_items] and returns n_items (host value; synchronizes s), or -1.
+int build_items(const int* tile_start, long ntiles, int dense_min, unsigned* chunks, unsigned* partial,
+                unsigned char* tile_dense, int* items, long items_cap, hipStream_t s);
 
 // Tile layout used by the binning: a node belongs to the tile that holds its
 // stencil start (floor(n x) - m) mod n on every axis.

