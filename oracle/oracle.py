@@ -207,13 +207,58 @@ def ndft_trafo_c2r(N, x, fhat_half, compute_grad=False):
     return f, grad
 
 
+def ndft_adj_c2r(N, x, f=None, grad=None):
+    """Adjoint NDFT of a c2r plan (kernel/ndft-parallel.c:617-722 over the half block k2 in [-N2/2, 0])
+    for real f / grad: the c2c adjoint sums restricted to the stored half spectrum."""
+    full = ndft_adj(N, x, f=None if f is None else np.asarray(f, np.complex128),
+                    grad=None if grad is None else np.asarray(grad, np.complex128))
+    return full.reshape(N)[:, :, : N[2] // 2 + 1].reshape(-1)
+
+
+def semirandom_hermitian(N, half=False):
+    """Deterministic Hermitian input of tests/check_trafo_vs_ndft_c2r.c:202-231:
+    semirandom(k) + conj(semirandom(-k)), zero wherever a component equals -N_t/2."""
+    def D(i):
+        return 1000.0 / (1 if i == 0 else i)
+
+    def sr(k0, k1, k2):
+        if k0 == -(N[0] // 2) or k1 == -(N[1] // 2) or k2 == -(N[2] // 2):  # C's -N/2 truncates
+            return 0j
+        if k0 == (N[0] + 1) // 2:
+            return sr(-k0, k1, k2)
+        if k1 == (N[1] + 1) // 2:
+            return sr(k0, -k1, k2)
+        if k2 == (N[2] + 1) // 2:
+            return sr(k0, k1, -k2)
+        l0 = k0 + N[0] if k0 else 1
+        l1 = k1 - 2 * N[1] if k1 else 1
+        l2 = k2 + N[2] if k2 else 1
+        re = D(l0 + l1 * l2) + 3 * D(l0 * l1 + l2) + D(l2 * l0 + l1)
+        im = 3 * D(l0 + l1 * l1) + D(l1 + l2 * l2) - D(l2 + l0 * l0)
+        return re + 1j * im
+
+    n2 = N[2] // 2 + 1 if half else N[2]
+    out = np.zeros((N[0], N[1], n2), np.complex128)
+    for i0 in range(N[0]):
+        for i1 in range(N[1]):
+            for i2 in range(n2):
+                k0, k1, k2 = i0 - N[0] // 2, i1 - N[1] // 2, i2 - N[2] // 2
+                out[i0, i1, i2] = sr(k0, k1, k2) + np.conj(sr(-k0, -k1, -k2))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Fast NFFT (D -> F -> ghost cells -> B), c2c
 # ---------------------------------------------------------------------------
 class Plan:
-    """Parameters of one oracle plan (api/api-guru.c:110-164, kernel/ndft-parallel.c:869-1069)."""
+    """Parameters of one oracle plan (api/api-guru.c:110-164, kernel/ndft-parallel.c:869-1069).
 
-    def __init__(self, N, n=None, m=6, flags=0, x_max=(0.5, 0.5, 0.5), workers=-1):
+    c2r=True restates the real-valued plans (PNX(init_guru_c2r)): f_hat is the half spectrum
+    [k0][k1][k2] with k2 in [-N2/2, 0] (N2/2+1 entries), the oversampled grid is real, and x,
+    f, grad are real (kernel/ndft-parallel.c:2809-2847, :2972-2990).
+    """
+
+    def __init__(self, N, n=None, m=6, flags=0, x_max=(0.5, 0.5, 0.5), workers=-1, c2r=False):
         self.N = [int(v) for v in N]
         self.n = [2 * v for v in self.N] if n is None else [int(v) for v in n]
         self.m = int(m)
@@ -224,6 +269,76 @@ class Plan:
         self.b = window_b(self.N, self.n, self.m, self.flags)
         self.tab = inv_phi_hat_tables(self.N, self.n, self.m, self.flags)
         self.workers = workers
+        self.c2r = bool(c2r)
+        self.Nh = [self.N[0], self.N[1], self.N[2] // 2 + 1]  # c2r half spectrum extent
+
+    # -- c2r: D on the half spectrum, k2 = i2 - N2/2 <= 0 (kernel/matrix_D.c:319-356 over local_N)
+    def _D_half(self, fhat_half, accumulate_into=None):
+        N0, N1, N2 = self.N
+        t0 = self.tab[:N0]
+        t1 = self.tab[N0:N0 + N1]
+        t2 = self.tab[N0 + N1:N0 + N1 + self.Nh[2]]
+        fac = (t0[:, None, None] * t1[None, :, None]) * t2[None, None, :]
+        return np.asarray(fhat_half).reshape(self.Nh) * fac
+
+    def _F_c2r(self, g1_half):
+        """Pruned shifted c2r: the real grid g_l = sum over the Hermitian extension of the half.
+
+        With the input shifted so that the stored k2 in [-N2/2, 0] lies in the FFTW half
+        [0, n2/2], FFTW's c2r adds every stored coefficient with its conjugate partner except
+        the k2 = 0 plane, which lands on the Nyquist slot and contributes its real part once.
+        """
+        N, n, no = self.N, self.n, self.no
+        A = np.zeros(n, np.complex128)
+        i0 = np.arange(-N[0] // 2, N[0] // 2) % n[0]
+        i1 = np.arange(-N[1] // 2, N[1] // 2) % n[1]
+        k2 = np.arange(-N[2] // 2, 1)
+        w = np.where(k2 < 0, 2.0, 1.0)
+        A[np.ix_(i0, i1, k2 % n[2])] = g1_half * w[None, None, :]
+        G = scipy.fft.fftn(A, workers=self.workers).real
+        oidx = [np.arange(-nt // 2, nt // 2) % nft for nt, nft in zip(no, n)]
+        return G[np.ix_(*oidx)]
+
+    def _FH_r2c(self, g2_real):
+        N, n, no = self.N, self.n, self.no
+        A = np.zeros(n, np.complex128)
+        oidx = [np.arange(-nt // 2, nt // 2) % nft for nt, nft in zip(no, n)]
+        A[np.ix_(*oidx)] = g2_real
+        G = scipy.fft.ifftn(A, workers=self.workers) * float(np.prod(n))
+        i0 = np.arange(-N[0] // 2, N[0] // 2) % n[0]
+        i1 = np.arange(-N[1] // 2, N[1] // 2) % n[1]
+        k2 = np.arange(-N[2] // 2, 1) % n[2]
+        return G[np.ix_(i0, i1, k2)]
+
+    def trafo_c2r(self, x, fhat_half, compute_f=True, compute_grad=False):
+        x = _f64(x).reshape(-1)
+        M = x.size // 3
+        g2 = self._F_c2r(self._D_half(fhat_half))
+        gpad = np.ascontiguousarray(np.pad(g2, [(self.m, self.m)] * 3, mode="wrap"))
+        f = np.zeros(M) if compute_f else None
+        grad = np.zeros(3 * M) if compute_grad else None
+        fl = (PNFFT_COMPUTE_F if compute_f else 0) | (PNFFT_COMPUTE_GRAD_F if compute_grad else 0)
+        lib().orc_B_trafo(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
+                          _dp(x), _dp(gpad), 1, _dp(f), _dp(grad), fl)
+        return f, grad
+
+    def adj_c2r(self, x, f=None, grad=None):
+        x = _f64(x).reshape(-1)
+        M = x.size // 3
+        ng = [v + 2 * self.m for v in self.no]
+        gpad = np.zeros(ng)
+        fl = 0
+        fa = ga = None
+        if f is not None:
+            fa = _f64(f).reshape(-1)
+            fl |= PNFFT_COMPUTE_F
+        if grad is not None:
+            ga = _f64(grad).reshape(-1)
+            fl |= PNFFT_COMPUTE_GRAD_F
+        lib().orc_B_adj(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
+                        _dp(x), _dp(fa), _dp(ga), 1, _dp(gpad), fl)
+        g2 = self.ghost_reduce(gpad)
+        return self._D_half(self._FH_r2c(g2)).reshape(-1)
 
     # -- D (kernel/matrix_D.c:227-280)
     def D(self, fhat):

@@ -20,7 +20,7 @@ B_VARIANTS = [
                       f"-DPNFFT_B_MODE={mi}"])
     for r in ("f32", "f64")
     for k, ki, modes in (("gather", 0, ((1, "f"), (2, "grad"), (3, "fgrad"), (4, "vjp"))),
-                         ("scatter", 1, ((1, "f"), (2, "grad"), (3, "fgrad"))))
+                         ("scatter", 1, ((1, "f"), (2, "grad"))))
     for mi, m in modes
 ]
 ARCH = os.environ.get("PNFFT_AMD_ARCH", "gfx950")

@@ -196,6 +196,8 @@ struct PlanT {
   unsigned pnfft_flags = 0, pfft_flags = 0;
   int win = WIN_KB;
   long N_total = 0, n_total = 0;
+  bool c2r = false;     // real plan (PNX(init_guru_c2r)): f_hat is the half spectrum k2 in [-N2/2, 0]
+  long fhat_elems = 0;  // complex entries of f_hat: N_total, or N0*N1*(N2/2+1) for c2r
 
   void* f_hat = nullptr;  // user-visible pointer (host or device)
   bool own_f_hat = false;
@@ -238,10 +240,13 @@ struct NodesT {
   long tile_cap = 0;
   unsigned* d_chunks = nullptr;      // 2 * ntiles: per-tile chunk counts and their offsets
   unsigned* d_tpartial = nullptr;    // scan partials of the tile counts
-  unsigned char* d_dense = nullptr;  // per-tile LDS-path flag
-  int* d_items = nullptr;            // (tile, chunk) pairs
-  long items_cap = 0;
+  unsigned char* d_dense = nullptr;  // gather: per-tile LDS-path flag
+  int* d_items = nullptr;            // gather: (tile, chunk) pairs
   int n_items = 0;
+  unsigned char* d_dense_s = nullptr;  // scatter: every nonempty tile takes the LDS path
+  int* d_items_s = nullptr;
+  int n_items_s = 0;
+  long items_cap = 0;
   SortWork sw;
   bool bins_valid = false;
   GridGeom bin_geom{};
@@ -295,8 +300,8 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
     set_error("d != 3 not yet implemented");
     return nullptr;
   }
-  if (c2r) {
-    set_error("c2r plans are not supported by this build");
+  if (c2r && (N[2] % 2)) {
+    set_error("c2r plans need an even N[2] (half spectrum N[2]/2+1)");
     return nullptr;
   }
   const unsigned unsupported = fl::INTERLACED | fl::DIFF_IK | fl::TRANSPOSED_F_HAT | fl::WINDOW_BESSEL_I0 |
@@ -347,6 +352,8 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
     P->N_total *= N[t];
     P->n_total *= n[t];
   }
+  P->c2r = c2r;
+  P->fhat_elems = c2r ? (long)N[0] * N[1] * (N[2] / 2 + 1) : P->N_total;
   P->geom.m = m;
   P->geom.win = P->win;
   for (int t = 0; t < 3; t++) {
@@ -375,7 +382,7 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
   P->evt.init();
   tables_compute(P);
   if (pnfft_flags & fl::MALLOC_F_HAT) {
-    P->f_hat = host_alloc(sizeof(Cplx<R>) * P->N_total);
+    P->f_hat = host_alloc(sizeof(Cplx<R>) * P->fhat_elems);
     P->own_f_hat = true;
   }
   return P;
@@ -394,8 +401,10 @@ static void nodes_release_device(NodesT<R>* D) {
   dfree(D->d_tpartial);
   dfree(D->d_dense);
   dfree(D->d_items);
+  dfree(D->d_dense_s);
+  dfree(D->d_items_s);
   D->items_cap = 0;
-  D->n_items = 0;
+  D->n_items = D->n_items_s = 0;
   dfree(D->sw.keys[0]);
   dfree(D->sw.keys[1]);
   dfree(D->sw.vals[0]);
@@ -425,15 +434,20 @@ static NodesT<R>* nodes_create(long M, unsigned malloc_flags, bool device) {
 
 // Resolve an array for the kernels: device pointers are used in place, host
 // arrays get a device mirror (allocated on demand, uploaded when `upload`).
+// A mirror always holds `count` elements of T (the complex size for node
+// values, whichever plan type uses the node set); `copy_bytes` (default: all)
+// is what is transferred, e.g. the M reals of a c2r plan's f.
 template <class T>
-static T* stage_in(void* user, T*& mirror, size_t count, bool upload, hipStream_t s, bool* is_host) {
+static T* stage_in(void* user, T*& mirror, size_t count, bool upload, hipStream_t s, bool* is_host,
+                   size_t copy_bytes = 0) {
   *is_host = false;
   if (!user) return nullptr;
   if (is_device_ptr(user)) return static_cast<T*>(user);
   *is_host = true;
   if (!mirror) mirror = dalloc<T>(count);
   if (!mirror) return nullptr;
-  if (upload) (void)hipMemcpyAsync(mirror, user, sizeof(T) * count, hipMemcpyHostToDevice, s);
+  if (upload)
+    (void)hipMemcpyAsync(mirror, user, copy_bytes ? copy_bytes : sizeof(T) * count, hipMemcpyHostToDevice, s);
   return mirror;
 }
 
@@ -477,18 +491,22 @@ static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
     dfree(D->d_chunks);
     dfree(D->d_tpartial);
     dfree(D->d_dense);
+    dfree(D->d_dense_s);
     D->d_tile_start = dalloc<int>(T.ntiles() + 1);
     D->d_chunks = dalloc<unsigned>(2 * T.ntiles());
     D->d_tpartial = dalloc<unsigned>((T.ntiles() + 4095) / 4096 + 1);
     D->d_dense = dalloc<unsigned char>(T.ntiles());
-    if (!D->d_tile_start || !D->d_chunks || !D->d_tpartial || !D->d_dense) return false;
+    D->d_dense_s = dalloc<unsigned char>(T.ntiles());
+    if (!D->d_tile_start || !D->d_chunks || !D->d_tpartial || !D->d_dense || !D->d_dense_s) return false;
     D->tile_cap = T.ntiles() + 1;
   }
   const long icap = T.ntiles() + (M + kTileNodes - 1) / kTileNodes + 1;
   if (D->items_cap < icap) {
     dfree(D->d_items);
+    dfree(D->d_items_s);
     D->d_items = dalloc<int>(2 * icap);
-    if (!D->d_items) return false;
+    D->d_items_s = dalloc<int>(2 * icap);
+    if (!D->d_items || !D->d_items_s) return false;
     D->items_cap = icap;
   }
   return true;
@@ -504,12 +522,12 @@ static int dense_min() {
 }
 
 template <class R>
-static void set_tiles(BArgs<R>& A, const NodesT<R>* D) {
+static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false) {
   A.tile_start = D->d_tile_start;
   A.keys = D->d_keys;
-  A.tile_dense = D->d_dense;
-  A.items = D->d_items;
-  A.n_items = D->n_items;
+  A.tile_dense = for_scatter ? D->d_dense_s : D->d_dense;
+  A.items = for_scatter ? D->d_items_s : D->d_items;
+  A.n_items = for_scatter ? D->n_items_s : D->n_items;
   for (int t = 0; t < 3; t++) {
     A.ts[t] = D->bin_tiles.ts[t];
     A.ntile[t] = D->bin_tiles.ntile[t];
@@ -532,15 +550,18 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx) {
     set_error("node binning failed (%d)", st);
     return st;
   }
-  D->n_items = 0;
+  D->n_items = D->n_items_s = 0;
   if (P->m >= 2 && P->m <= 8) {
     const int ni = build_items(D->d_tile_start, P->tiles.ntiles(), dense_min(), D->d_chunks, D->d_tpartial,
                                D->d_dense, D->d_items, D->items_cap, P->stream);
-    if (ni < 0) {
+    const int ns = build_items(D->d_tile_start, P->tiles.ntiles(), 1, D->d_chunks, D->d_tpartial, D->d_dense_s,
+                               D->d_items_s, D->items_cap, P->stream);
+    if (ni < 0 || ns < 0) {
       set_error("building the tile work list failed");
       return kHipError;
     }
     D->n_items = ni;
+    D->n_items_s = ns;
   }
   D->bin_geom = P->geom;
   D->bin_tiles = P->tiles;
@@ -575,14 +596,18 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
   const bool accumulate = (cf & fl::COMPUTE_ACCUMULATED) != 0;
   bool host_x = false, host_f = false, host_g = false, host_fh = false;
 
+  // c2r plans keep f and grad_f as reals (kernel/ndft-parallel.c:2826-2847)
+  const size_t vsz = P->c2r ? sizeof(R) : sizeof(Cplx<R>);
   R* dx = D ? stage_in<R>(D->x, D->d_x, 3 * M, true, s, &host_x) : nullptr;
-  Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, true, s, &host_fh);
+  Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, true, s, &host_fh,
+                                   sizeof(Cplx<R>) * P->fhat_elems);
   const bool direct = cf & fl::COMPUTE_DIRECT;
   // outputs: uploaded only when accumulating (the NDFT overwrites them regardless, :400-414)
   Cplx<R>* df = nullptr;
   Cplx<R>* dg = nullptr;
-  if (D && want_f) df = stage_in<Cplx<R>>(D->f, D->d_f, M, accumulate && !direct, s, &host_f);
-  if (D && want_g) dg = stage_in<Cplx<R>>(D->grad_f, D->d_grad, 3 * M, accumulate && !direct, s, &host_g);
+  if (D && want_f) df = stage_in<Cplx<R>>(D->f, D->d_f, M, accumulate && !direct, s, &host_f, vsz * M);
+  if (D && want_g)
+    dg = stage_in<Cplx<R>>(D->grad_f, D->d_grad, 3 * M, accumulate && !direct, s, &host_g, vsz * 3 * M);
   if ((want_f && M > 0 && !df) || (want_g && M > 0 && !dg) || (M > 0 && !dx && !(cf & fl::OMIT_CONV)) || !dfh) {
     set_error("trafo: missing node or f_hat arrays");
     return;
@@ -599,13 +624,14 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     A.grad = dg;
     A.want_f = want_f;
     A.want_grad = want_g;
+    A.c2r = P->c2r;
     if (ndft_trafo<R>(A, s) != kOk) set_error("NDFT trafo launch failed");
   } else {
     // D and F (fused): grid <- F D f_hat
     if (!(cf & fl::OMIT_FFT)) {
       P->evt.rec(2, s);
       const double* tab = (cf & fl::OMIT_DECONV) ? nullptr : P->d_tab;
-      if (fft3d_forward<R>(P->fft, dfh, tab, P->d_buf1, P->d_buf2, P->d_grid, s) != kOk)
+      if (fft3d_forward<R>(P->fft, dfh, tab, P->d_buf1, P->d_buf2, P->d_grid, s, P->c2r) != kOk)
         set_error("FFT launch failed");
       P->evt.rec(3, s);
       had_f = true;
@@ -630,6 +656,7 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
       A.grad = dg;
       A.accumulate = accumulate;
       A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
+      A.c2r = P->c2r;
       set_tiles(A, D);
       int mode = (want_f && want_g) ? B_FGRAD : (want_f ? B_F : B_GRAD);
       P->evt.rec(4, s);
@@ -638,8 +665,8 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
       had_b = true;
     }
   }
-  if (host_f) (void)hipMemcpyAsync(D->f, df, sizeof(Cplx<R>) * M, hipMemcpyDeviceToHost, s);
-  if (host_g) (void)hipMemcpyAsync(D->grad_f, dg, sizeof(Cplx<R>) * 3 * M, hipMemcpyDeviceToHost, s);
+  if (host_f) (void)hipMemcpyAsync(D->f, df, vsz * M, hipMemcpyDeviceToHost, s);
+  if (host_g) (void)hipMemcpyAsync(D->grad_f, dg, vsz * 3 * M, hipMemcpyDeviceToHost, s);
   if (hipStreamSynchronize(s) != hipSuccess) {
     set_error("trafo: HIP error %s", hipGetErrorString(hipGetLastError()));
     return;
@@ -660,16 +687,19 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
   const bool use_f = cf & fl::COMPUTE_F, use_g = cf & fl::COMPUTE_GRAD_F;
   const bool accumulate = (cf & fl::COMPUTE_ACCUMULATED) != 0;
   bool host_x = false, host_f = false, host_g = false, host_fh = false;
+  const size_t vsz = P->c2r ? sizeof(R) : sizeof(Cplx<R>);
+  const size_t fh_bytes = sizeof(Cplx<R>) * P->fhat_elems;
   R* dx = D ? stage_in<R>(D->x, D->d_x, 3 * M, true, s, &host_x) : nullptr;
-  Cplx<R>* df = (D && use_f) ? stage_in<Cplx<R>>(D->f, D->d_f, M, true, s, &host_f) : nullptr;
-  Cplx<R>* dg = (D && use_g) ? stage_in<Cplx<R>>(D->grad_f, D->d_grad, 3 * M, true, s, &host_g) : nullptr;
-  Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, accumulate, s, &host_fh);
+  Cplx<R>* df = (D && use_f) ? stage_in<Cplx<R>>(D->f, D->d_f, M, true, s, &host_f, vsz * M) : nullptr;
+  Cplx<R>* dg =
+      (D && use_g) ? stage_in<Cplx<R>>(D->grad_f, D->d_grad, 3 * M, true, s, &host_g, vsz * 3 * M) : nullptr;
+  Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, accumulate, s, &host_fh, fh_bytes);
   if (!dfh || (M > 0 && !(cf & fl::OMIT_CONV) && (!dx || (use_f && !df) || (use_g && !dg)))) {
     set_error("adj: missing node or f_hat arrays");
     return;
   }
   // zero f_hat unless accumulating (:355-356)
-  if (!accumulate) fill_zero<R>(dfh, P->N_total, s);
+  if (!accumulate) fill_zero<R>(dfh, P->fhat_elems, s);
 
   bool had_sort = false, had_f = false, had_b = false;
   if (cf & fl::COMPUTE_DIRECT) {
@@ -682,6 +712,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     A.grad = dg;
     A.want_f = use_f;
     A.want_grad = use_g;
+    A.c2r = P->c2r;
     if (ndft_adj<R>(A, s) != kOk) set_error("NDFT adj launch failed");
   } else {
     if (!(cf & fl::OMIT_CONV)) {
@@ -704,7 +735,8 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
         A.grid = P->d_grid;
         A.f = df;
         A.grad = dg;
-        set_tiles(A, D);
+        A.c2r = P->c2r;
+        set_tiles(A, D, true);
         const int mode = (use_f && use_g) ? B_FGRAD : (use_f ? B_F : B_GRAD);
         P->evt.rec(4, s);
         if (scatter<R>(A, mode, s) != kOk) set_error("scatter launch failed");
@@ -715,13 +747,13 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     // F^H and D^T (fused), f_hat += D^T F^H grid
     if (!(cf & fl::OMIT_FFT) && !(cf & fl::OMIT_DECONV)) {
       P->evt.rec(2, s);
-      if (fft3d_backward<R>(P->fft, P->d_grid, P->d_tab, P->d_buf1, P->d_buf2, dfh, 1, s) != kOk)
+      if (fft3d_backward<R>(P->fft, P->d_grid, P->d_tab, P->d_buf1, P->d_buf2, dfh, 1, s, P->c2r) != kOk)
         set_error("FFT launch failed");
       P->evt.rec(3, s);
       had_f = true;
     }
   }
-  if (host_fh) (void)hipMemcpyAsync(P->f_hat, dfh, sizeof(Cplx<R>) * P->N_total, hipMemcpyDeviceToHost, s);
+  if (host_fh) (void)hipMemcpyAsync(P->f_hat, dfh, fh_bytes, hipMemcpyDeviceToHost, s);
   if (hipStreamSynchronize(s) != hipSuccess) {
     set_error("adj: HIP error %s", hipGetErrorString(hipGetLastError()));
     return;
@@ -740,16 +772,16 @@ static void vjp_impl(PlanT<R>* P, NodesT<R>* D, const Cplx<R>* w, R* gx) {
   if (M == 0) return;
   bool host_x = false, host_fh = false, host_w = false, host_gx = false;
   R* dx = stage_in<R>(D->x, D->d_x, 3 * M, true, s, &host_x);
-  Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, true, s, &host_fh);
-  Cplx<R>* dw_m = nullptr;
-  Cplx<R>* dw = stage_in<Cplx<R>>((void*)w, D->d_f, M, true, s, &host_w);
-  (void)dw_m;
+  Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, true, s, &host_fh,
+                                   sizeof(Cplx<R>) * P->fhat_elems);
+  const size_t vsz = P->c2r ? sizeof(R) : sizeof(Cplx<R>);
+  Cplx<R>* dw = stage_in<Cplx<R>>((void*)w, D->d_f, M, true, s, &host_w, vsz * M);
   R* dgx = stage_in<R>(gx, D->d_gx, 3 * M, false, s, &host_gx);
   if (!dx || !dfh || !dw || !dgx) {
     set_error("trafo_grad_x_vjp: missing arrays");
     return;
   }
-  if (fft3d_forward<R>(P->fft, dfh, P->d_tab, P->d_buf1, P->d_buf2, P->d_grid, s) != kOk) {
+  if (fft3d_forward<R>(P->fft, dfh, P->d_tab, P->d_buf1, P->d_buf2, P->d_grid, s, P->c2r) != kOk) {
     set_error("FFT launch failed");
     return;
   }
@@ -764,6 +796,7 @@ static void vjp_impl(PlanT<R>* P, NodesT<R>* D, const Cplx<R>* w, R* gx) {
   A.grid = P->d_grid;
   A.w = dw;
   A.gx = dgx;
+  A.c2r = P->c2r;
   set_tiles(A, D);
   if (gather<R>(A, B_VJP, s) != kOk) set_error("gather launch failed");
   if (host_gx) (void)hipMemcpyAsync(gx, dgx, sizeof(R) * 3 * M, hipMemcpyDeviceToHost, s);
@@ -1018,8 +1051,8 @@ using namespace pnfft_amd;
   void PREFIX##zero_f_hat(void* p) {                                                                        \
     auto* P = static_cast<PlanT<R>*>(p); if (!P || !P->f_hat) return;                                       \
     if (is_device_ptr(P->f_hat)) {                                                                          \
-      fill_zero<R>((Cplx<R>*)P->f_hat, P->N_total, P->stream); (void)hipStreamSynchronize(P->stream);        \
-    } else memset(P->f_hat, 0, sizeof(Cplx<R>) * P->N_total);                                               \
+      fill_zero<R>((Cplx<R>*)P->f_hat, P->fhat_elems, P->stream); (void)hipStreamSynchronize(P->stream);     \
+    } else memset(P->f_hat, 0, sizeof(Cplx<R>) * P->fhat_elems);                                            \
   }                                                                                                        \
   R PREFIX##inv_phi_hat(const void* p, int dim, ptrdiff_t k) {                                               \
     auto* P = static_cast<const PlanT<R>*>(p);                                                              \

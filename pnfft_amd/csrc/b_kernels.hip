@@ -101,8 +101,30 @@ __device__ inline void gather_sum(const NodeWin<R, CM>& w, int c, RowFn row_of, 
   out[3] = g2;
 }
 
+// node values: complex arrays, or real arrays of a c2r plan (imaginary part 0)
+template <class R>
+__device__ inline Cplx<R> load_f(const BArgs<R>& A, long j) {
+  return A.c2r ? cmk<R>(reinterpret_cast<const R*>(A.f)[j], R(0)) : A.f[j];
+}
+template <class R>
+__device__ inline Cplx<R> load_g(const BArgs<R>& A, long j, int t) {
+  return A.c2r ? cmk<R>(reinterpret_cast<const R*>(A.grad)[3 * j + t], R(0)) : A.grad[3 * j + t];
+}
+
 template <class R, int MODE>
 __device__ inline void gather_store(const BArgs<R>& A, long j, Cplx<R>* v) {
+  if (A.c2r) {  // real plan: f = Re(...), grad = Re(...) (kernel/assign.c:715-739, :821-851)
+    R* fr = reinterpret_cast<R*>(A.f);
+    R* gr = reinterpret_cast<R*>(A.grad);
+    if (MODE == B_F || MODE == B_FGRAD) fr[j] = (A.accumulate ? fr[j] : R(0)) + v[0].x;
+    if (MODE == B_GRAD || MODE == B_FGRAD)
+      for (int t = 0; t < 3; t++) gr[3 * j + t] = (A.accumulate ? gr[3 * j + t] : R(0)) + v[1 + t].x;
+    if (MODE == B_VJP) {
+      const R wv = reinterpret_cast<const R*>(A.w)[j];
+      for (int t = 0; t < 3; t++) A.gx[3 * j + t] = wv * v[1 + t].x;
+    }
+    return;
+  }
   if (MODE == B_F || MODE == B_FGRAD) {
     Cplx<R> o = A.accumulate ? A.f[j] : cmk<R>(0, 0);
     Cplx<R> f = v[0];
@@ -158,11 +180,11 @@ __device__ inline void scatter_node_global(const BArgs<R>& A, long j, const Node
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
   Cplx<R> fv = cmk<R>(0, 0), gv0 = cmk<R>(0, 0), gv1 = cmk<R>(0, 0), gv2 = cmk<R>(0, 0);
-  if (WF) fv = A.f[j];
+  if (WF) fv = load_f(A, j);
   if (WG) {
-    gv0 = A.grad[3 * j + 0];
-    gv1 = A.grad[3 * j + 1];
-    gv2 = A.grad[3 * j + 2];
+    gv0 = load_g(A, j, 0);
+    gv1 = load_g(A, j, 1);
+    gv2 = load_g(A, j, 2);
   }
   for (int a = 0; a < c; a++) {
     int i0 = w.start[0] + a;
@@ -275,6 +297,19 @@ __device__ inline int local_off(int start, int o, int ts, int n) {
   return (l < ts) ? l : -1;
 }
 
+// LDS subgrid layout: cell (a, b, c) at a*PP + b*RP + c with an odd row pitch
+// RP = P2 + 1 and plane pitch PP = P1*RP + 1, so that rows and planes of the
+// subgrid start in different banks.
+struct SubgridPitch {
+  int RP, PP;
+};
+__device__ inline SubgridPitch subgrid_pitch(const TileItem& it) {
+  SubgridPitch s;
+  s.RP = it.P[2] + 1;
+  s.PP = it.P[1] * s.RP + 1;
+  return s;
+}
+
 template <class R, int MT, int MODE>
 __global__ __launch_bounds__(kTileThreads) void gather_tiled_k(BArgs<R> A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -283,13 +318,14 @@ __global__ __launch_bounds__(kTileThreads) void gather_tiled_k(BArgs<R> A) {
   constexpr int CM = 2 * MT + 1;
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
   const TileItem it = tile_item(A, m);
+  const SubgridPitch sp = subgrid_pitch(it);
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
-  const int P0 = it.P[0], P1 = it.P[1], P2 = it.P[2];
-  const int vol = P0 * P1 * P2;
+  const int P1 = it.P[1], P2 = it.P[2];
+  const int vol = it.P[0] * P1 * P2;
   for (int idx = threadIdx.x; idx < vol; idx += kTileThreads) {
     const int cz = idx % P2, cy = (idx / P2) % P1, cx = idx / (P1 * P2);
     const int g0 = wrapi(it.o[0] + cx, n0), g1 = wrapi(it.o[1] + cy, n1), g2 = wrapi(it.o[2] + cz, n2);
-    tile[idx] = A.grid[((long)g0 * n1 + g1) * n2 + g2];
+    tile[cx * sp.PP + cy * sp.RP + cz] = A.grid[((long)g0 * n1 + g1) * n2 + g2];
   }
   __syncthreads();
   for (int p = it.beg + threadIdx.x; p < it.end; p += kTileThreads) {
@@ -303,8 +339,8 @@ __global__ __launch_bounds__(kTileThreads) void gather_tiled_k(BArgs<R> A) {
       gather_node_global<R, CM, MODE>(A, j, w, CM);
       continue;
     }
-    const Cplx<R>* base = tile + (l0 * P1 + l1) * P2 + l2;
-    auto row_of = [&](int a, int b) { return base + (a * P1 + b) * P2; };
+    const Cplx<R>* base = tile + l0 * sp.PP + l1 * sp.RP + l2;
+    auto row_of = [&](int a, int b) { return base + a * sp.PP + b * sp.RP; };
     auto idx = [](int s) { return s; };
     Cplx<R> v[4];
     gather_sum<R, CM, MODE>(w, CM, row_of, idx, v);
@@ -321,12 +357,19 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
   constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
   const TileItem it = tile_item(A, m);
+  const SubgridPitch sp = subgrid_pitch(it);
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
   const int P1 = it.P[1], P2 = it.P[2];
   const int vol = it.P[0] * P1 * P2;
-  for (int idx = threadIdx.x; idx < 2 * vol; idx += kTileThreads) acc[idx] = 0.0;
+  for (int idx = threadIdx.x; idx < 2 * it.P[0] * sp.PP; idx += kTileThreads) acc[idx] = 0.0;
   __syncthreads();
-  for (int p = it.beg + threadIdx.x; p < it.end; p += kTileThreads) {
+  // Sorted nodes that share a cell are consecutive; dealing them out across the
+  // waves (lane-major) keeps the lanes of one wave on different cells, so one
+  // ds_add instruction rarely hits the same LDS address twice.
+  constexpr int kWaves = kTileThreads / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int q = lane * kWaves + wave; it.beg + q < it.end; q += kTileThreads) {
+    const int p = it.beg + q;
     const long j = A.perm[p];
     NodeWin<R, CM> w;
     node_windows<R, CM, WG>(A, j, m, w);
@@ -338,16 +381,16 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
       continue;
     }
     Cplx<R> fv = cmk<R>(0, 0), gv0 = cmk<R>(0, 0), gv1 = cmk<R>(0, 0), gv2 = cmk<R>(0, 0);
-    if (WF) fv = A.f[j];
+    if (WF) fv = load_f(A, j);
     if (WG) {
-      gv0 = A.grad[3 * j + 0];
-      gv1 = A.grad[3 * j + 1];
-      gv2 = A.grad[3 * j + 2];
+      gv0 = load_g(A, j, 0);
+      gv1 = load_g(A, j, 1);
+      gv2 = load_g(A, j, 2);
     }
-    double* base = acc + 2 * ((l0 * P1 + l1) * P2 + l2);
+    double* base = acc + 2 * (l0 * sp.PP + l1 * sp.RP + l2);
     for (int a = 0; a < CM; a++) {
       for (int b = 0; b < CM; b++) {
-        double* row = base + 2 * ((a * P1 + b) * P2);
+        double* row = base + 2 * (a * sp.PP + b * sp.RP);
         const R pxy = w.px[a] * w.py[b];
         R pdxy = 0, pxdy = 0;
         if (WG) {
@@ -375,9 +418,10 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
   }
   __syncthreads();
   for (int idx = threadIdx.x; idx < vol; idx += kTileThreads) {
-    const double vr = acc[2 * idx], vi = acc[2 * idx + 1];
-    if (vr == 0.0 && vi == 0.0) continue;
     const int cz = idx % P2, cy = (idx / P2) % P1, cx = idx / (P1 * P2);
+    const int li = cx * sp.PP + cy * sp.RP + cz;
+    const double vr = acc[2 * li], vi = acc[2 * li + 1];
+    if (vr == 0.0 && vi == 0.0) continue;
     const int g0 = wrapi(it.o[0] + cx, n0), g1 = wrapi(it.o[1] + cy, n1), g2 = wrapi(it.o[2] + cz, n2);
     Cplx<R>* cell = A.grid + ((long)g0 * n1 + g1) * n2 + g2;
     atomicAdd(&cell->x, (R)vr);
@@ -396,8 +440,10 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
   const bool tiled = A.items != nullptr && A.n_items > 0 && A.g.m >= 2 && A.g.m <= 8;
   size_t lds = 0;
   if (tiled) {
-    const long vol = (long)(A.ts[0] + 2 * A.g.m) * (A.ts[1] + 2 * A.g.m) * (A.ts[2] + 2 * A.g.m);
-    lds = (size_t)vol * (KIND == 0 ? sizeof(Cplx<R>) : 16);
+    // padded subgrid: P0 planes of pitch P1*(P2+1)+1 cells (SubgridPitch)
+    const long P0 = A.ts[0] + 2 * A.g.m, P1 = A.ts[1] + 2 * A.g.m, P2 = A.ts[2] + 2 * A.g.m;
+    const long cells = P0 * (P1 * (P2 + 1) + 1);
+    lds = (size_t)cells * (KIND == 0 ? sizeof(Cplx<R>) : 16);
   }
   const dim3 tgrid((unsigned)(tiled ? A.n_items : 1)), tblock(kTileThreads);
   BArgs<R> D = A;
@@ -405,7 +451,7 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
   switch (A.g.m) {
 #define PNFFT_B_CASE(MV)                                                                                  \
   case MV:                                                                                                \
-    if (KIND == 0) {                                                                                      \
+    if constexpr (KIND == 0) {                                                                            \
       if (tiled) {                                                                                        \
         (void)hipFuncSetAttribute((const void*)gather_tiled_k<R, MV, MODE>,                               \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                  \
@@ -431,7 +477,7 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
 #undef PNFFT_B_CASE
     default: {
       D.tile_dense = nullptr;
-      if (KIND == 0) hipLaunchKernelGGL((gather_direct_k<R, 0, MODE>), grid, block, 0, s, D);
+      if constexpr (KIND == 0) hipLaunchKernelGGL((gather_direct_k<R, 0, MODE>), grid, block, 0, s, D);
       else hipLaunchKernelGGL((scatter_direct_k<R, 0, SMODE>), grid, block, 0, s, D);
       break;
     }

@@ -52,6 +52,11 @@ struct PassArgs {
   const double* tab2;
   int N1;
   int accumulate;            // backward store: out += value (D^T onto f_hat)
+  // frequency <-> index map of the pruned side: index e holds frequency k = e - h;
+  // forward loads take k in [-h, pos); the c2r half spectrum is h = N2/2, pos = 1
+  // (k2 in [-N2/2, 0]) and its k2 < 0 entries enter the pencil twice (weight 2)
+  int h, pos;
+  int half;
 };
 
 template <class R>
@@ -145,10 +150,9 @@ __device__ inline void run_stages(Cplx<R>* sh, int LD, const PassArgs<R>& P) {
 }
 
 // position p of a length-L pencil holds frequency k = p (p small) or p - L;
-// input index e = k + Lin/2 when k lies in [-Lin/2, Lin - Lin/2).
-__device__ inline int pruned_src(int p, int L, int Lin) {
-  const int h = Lin / 2;
-  if (p < Lin - h) return p + h;
+// input index e = k + h when k lies in [-h, pos).
+__device__ inline int pruned_src(int p, int L, int h, int pos) {
+  if (p < pos) return p + h;
   if (p >= L - h) return p - L + h;
   return -1;
 }
@@ -166,31 +170,47 @@ __global__ __launch_bounds__(kThreads) void fft_pass(PassArgs<R> P) {
   const bool fwd = !P.backward;
   const bool deconv = (P.tab0 != nullptr);
 
+  // Every thread owns exactly EPT = L*C/blockDim positions of the block's
+  // pencils; all of its global loads are issued before the LDS writes (and all
+  // LDS reads before the global stores) to keep many requests in flight.
+  constexpr int EPT = FftEpt<R>::value;
+  Cplx<R> v[EPT];
+
   // ---- load
-  const int nload = L * C;
-  for (int idx = tid; idx < nload; idx += kThreads) {
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int idx = tid + i * kThreads;
     int p, c;
     if (ROWS) { c = idx / L; p = idx % L; }
     else { p = idx / C; c = idx % C; }
     const int col = c0 + c;
-    Cplx<R> v = cmk<R>(R(0), R(0));
-    if (col < P.n_inner) {
-      const int e = fwd ? pruned_src(p, L, P.Lin) : p;
-      if (e >= 0) {
-        long addr;
-        if (ROWS) addr = (long)col * P.in_outer + (long)e * P.in_elem;
-        else addr = outer * P.in_outer + col + (long)e * P.in_elem;
-        v = P.in[addr];
-        if (deconv && fwd) {
-          // row = i0*N1 + i1 (ROWS pass over f_hat), e = i2
-          const int i0 = col / P.N1, i1 = col % P.N1;
-          const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
-          v = cmk<R>(R((double)v.x * f), R((double)v.y * f));
-        }
-        if (!fwd) v = cconj(v);
-      }
+    v[i] = cmk<R>(R(0), R(0));
+    const int e = fwd ? pruned_src(p, L, P.h, P.pos) : p;
+    if (col < P.n_inner && e >= 0) {
+      long addr;
+      if (ROWS) addr = (long)col * P.in_outer + (long)e * P.in_elem;
+      else addr = outer * P.in_outer + col + (long)e * P.in_elem;
+      v[i] = P.in[addr];
     }
-    sh[p * LD + c] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int idx = tid + i * kThreads;
+    int p, c;
+    if (ROWS) { c = idx / L; p = idx % L; }
+    else { p = idx / C; c = idx % C; }
+    Cplx<R> w = v[i];
+    const int col = c0 + c;
+    const int e = fwd ? pruned_src(p, L, P.h, P.pos) : p;
+    if (deconv && fwd && col < P.n_inner && e >= 0) {
+      // row = i0*N1 + i1 (ROWS pass over f_hat), e = i2
+      const int i0 = col / P.N1, i1 = col % P.N1;
+      const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
+      w = cmk<R>(R((double)w.x * f), R((double)w.y * f));
+    }
+    if (P.half && fwd && e >= 0 && e < P.h) w = cmk<R>(R(2) * w.x, R(2) * w.y);
+    if (!fwd) w = cconj(w);
+    sh[p * LD + c] = w;
   }
   __syncthreads();
 
@@ -198,32 +218,39 @@ __global__ __launch_bounds__(kThreads) void fft_pass(PassArgs<R> P) {
 
   // ---- store
   const int nstore = P.Lout * C;
-  for (int idx = tid; idx < nstore; idx += kThreads) {
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int idx = tid + i * kThreads;
+    if (idx < nstore) {
+      int e, c;
+      if (ROWS) { c = idx / P.Lout; e = idx % P.Lout; }
+      else { e = idx / C; c = idx % C; }
+      int p = e;
+      if (!fwd) p = (e - P.h + L) % L;
+      v[i] = sh[p * LD + c];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int idx = tid + i * kThreads;
+    if (idx >= nstore) continue;
     int e, c;
     if (ROWS) { c = idx / P.Lout; e = idx % P.Lout; }
     else { e = idx / C; c = idx % C; }
     const int col = c0 + c;
     if (col >= P.n_inner) continue;
-    int p = e;
-    if (!fwd) {
-      const int k = e - P.Lout / 2;
-      p = (k + L) % L;
-    }
-    Cplx<R> v = sh[p * LD + c];
-    if (!fwd) v = cconj(v);
+    Cplx<R> w = v[i];
+    if (!fwd) w = cconj(w);
     long addr;
     if (ROWS) addr = (long)col * P.out_outer + (long)e * P.out_elem;
     else addr = outer * P.out_outer + col + (long)e * P.out_elem;
     if (deconv && !fwd) {
       const int i0 = col / P.N1, i1 = col % P.N1;
       const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
-      v = cmk<R>(R((double)v.x * f), R((double)v.y * f));
+      w = cmk<R>(R((double)w.x * f), R((double)w.y * f));
     }
-    if (P.accumulate) {
-      Cplx<R> o = P.out[addr];
-      v = cadd(o, v);
-    }
-    P.out[addr] = v;
+    if (P.accumulate) w = cadd(P.out[addr], w);
+    P.out[addr] = w;
   }
 }
 
@@ -286,18 +313,21 @@ static size_t lds_bytes(const FftAxis<R>& ax) {
 
 template <class R>
 int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
-                  Cplx<R>* grid, hipStream_t s) {
+                  Cplx<R>* grid, hipStream_t s, bool half) {
   const int N0 = F.N[0], N1 = F.N[1], N2 = F.N[2];
   const int n0 = F.n[0], n1 = F.n[1], n2 = F.n[2];
+  const int N2in = half ? N2 / 2 + 1 : N2;  // stored length of f_hat rows
   // axis 2: rows (i0,i1) of f_hat -> buf1[i0][i1][p2]
   {
     PassArgs<R> P = make_args(F.ax[2]);
     P.in = fhat; P.out = buf1;
-    P.in_outer = N2; P.in_elem = 1;
+    P.in_outer = N2in; P.in_elem = 1;
     P.out_outer = n2; P.out_elem = 1;
     P.n_inner = N0 * N1;
-    P.Lin = N2; P.Lout = n2;
-    P.backward = 0;
+    P.Lin = N2in; P.Lout = n2;
+    P.h = N2 / 2;
+    P.pos = half ? 1 : N2 - N2 / 2;
+    P.half = half ? 1 : 0;
     if (tab) { P.tab0 = tab; P.tab1 = tab + N0; P.tab2 = tab + N0 + N1; }
     P.N1 = N1;
     dim3 grid_dim((P.n_inner + P.C - 1) / P.C, 1);
@@ -311,6 +341,7 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.out_outer = (long)n1 * n2; P.out_elem = n2;
     P.n_inner = n2;
     P.Lin = N1; P.Lout = n1;
+    P.h = N1 / 2; P.pos = N1 - N1 / 2;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);
     hipLaunchKernelGGL((fft_pass<R, false>), grid_dim, dim3(kThreads), lds_bytes(F.ax[1]), s, P);
   }
@@ -322,6 +353,7 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.out_outer = n2; P.out_elem = (long)n1 * n2;
     P.n_inner = n2;
     P.Lin = N0; P.Lout = n0;
+    P.h = N0 / 2; P.pos = N0 - N0 / 2;
     dim3 grid_dim((n2 + P.C - 1) / P.C, n1);
     hipLaunchKernelGGL((fft_pass<R, false>), grid_dim, dim3(kThreads), lds_bytes(F.ax[0]), s, P);
   }
@@ -330,9 +362,10 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
 
 template <class R>
 int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
-                   Cplx<R>* fhat, int accumulate, hipStream_t s) {
+                   Cplx<R>* fhat, int accumulate, hipStream_t s, bool half) {
   const int N0 = F.N[0], N1 = F.N[1], N2 = F.N[2];
   const int n0 = F.n[0], n1 = F.n[1], n2 = F.n[2];
+  const int N2out = half ? N2 / 2 + 1 : N2;
   // axis 0: grid pencils (p1,p2) -> buf2[i0][p1][p2]
   {
     PassArgs<R> P = make_args(F.ax[0]);
@@ -340,7 +373,7 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.in_outer = n2; P.in_elem = (long)n1 * n2;
     P.out_outer = n2; P.out_elem = (long)n1 * n2;
     P.n_inner = n2;
-    P.Lin = n0; P.Lout = N0;
+    P.Lin = n0; P.Lout = N0; P.h = N0 / 2;
     P.backward = 1;
     dim3 grid_dim((n2 + P.C - 1) / P.C, n1);
     hipLaunchKernelGGL((fft_pass<R, false>), grid_dim, dim3(kThreads), lds_bytes(F.ax[0]), s, P);
@@ -352,19 +385,19 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.in_outer = (long)n1 * n2; P.in_elem = n2;
     P.out_outer = (long)N1 * n2; P.out_elem = n2;
     P.n_inner = n2;
-    P.Lin = n1; P.Lout = N1;
+    P.Lin = n1; P.Lout = N1; P.h = N1 / 2;
     P.backward = 1;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);
     hipLaunchKernelGGL((fft_pass<R, false>), grid_dim, dim3(kThreads), lds_bytes(F.ax[1]), s, P);
   }
-  // axis 2: rows buf1[i0][i1][:] -> f_hat[i0][i1][:] (D^T fused)
+  // axis 2: rows buf1[i0][i1][:] -> f_hat[i0][i1][:] (D^T fused; c2r keeps k2 in [-N2/2, 0])
   {
     PassArgs<R> P = make_args(F.ax[2]);
     P.in = buf1; P.out = fhat;
     P.in_outer = n2; P.in_elem = 1;
-    P.out_outer = N2; P.out_elem = 1;
+    P.out_outer = N2out; P.out_elem = 1;
     P.n_inner = N0 * N1;
-    P.Lin = n2; P.Lout = N2;
+    P.Lin = n2; P.Lout = N2out; P.h = N2 / 2;
     P.backward = 1;
     P.accumulate = accumulate;
     if (tab) { P.tab0 = tab; P.tab1 = tab + N0; P.tab2 = tab + N0 + N1; }
@@ -408,12 +441,12 @@ template struct FftAxis<double>;
 template struct Fft3d<float>;
 template struct Fft3d<double>;
 template int fft3d_forward<float>(const Fft3d<float>&, const Cplx<float>*, const double*, Cplx<float>*,
-                                  Cplx<float>*, Cplx<float>*, hipStream_t);
+                                  Cplx<float>*, Cplx<float>*, hipStream_t, bool);
 template int fft3d_forward<double>(const Fft3d<double>&, const Cplx<double>*, const double*, Cplx<double>*,
-                                   Cplx<double>*, Cplx<double>*, hipStream_t);
+                                   Cplx<double>*, Cplx<double>*, hipStream_t, bool);
 template int fft3d_backward<float>(const Fft3d<float>&, const Cplx<float>*, const double*, Cplx<float>*,
-                                   Cplx<float>*, Cplx<float>*, int, hipStream_t);
+                                   Cplx<float>*, Cplx<float>*, int, hipStream_t, bool);
 template int fft3d_backward<double>(const Fft3d<double>&, const Cplx<double>*, const double*, Cplx<double>*,
-                                    Cplx<double>*, Cplx<double>*, int, hipStream_t);
+                                    Cplx<double>*, Cplx<double>*, int, hipStream_t, bool);
 
 }  // namespace pnfft_amd

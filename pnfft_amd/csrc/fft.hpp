@@ -36,14 +36,16 @@ struct Fft3d {
   size_t buf2_elems() const { return (size_t)N[0] * n[1] * n[2]; }
 };
 
-// grid <- F D fhat  (tab = concatenated FP64 D factors, or null to skip D)
+// grid <- F D fhat  (tab = concatenated FP64 D factors, or null to skip D).
+// half: fhat holds the c2r half spectrum k2 in [-N2/2, 0]; the grid's real part is then the
+// c2r transform (the k2 < 0 coefficients count twice, the k2 = 0 plane once).
 template <class R>
 int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
-                  Cplx<R>* grid, hipStream_t s);
+                  Cplx<R>* grid, hipStream_t s, bool half = false);
 
-// fhat (+)= D^T F^H grid
+// fhat (+)= D^T F^H grid; half: only k2 in [-N2/2, 0] is produced (r2c of a real grid).
 template <class R>
 int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
-                   Cplx<R>* fhat, int accumulate, hipStream_t s);
+                   Cplx<R>* fhat, int accumulate, hipStream_t s, bool half = false);
 
 }  // namespace pnfft_amd

@@ -88,20 +88,94 @@ __global__ __launch_bounds__(256) void ndft_adj_k(NdftArgs<R> A) {
   A.fhat[kidx] = cadd(A.fhat[kidx], acc);
 }
 
+// is_hermitian of kernel/ndft-parallel.c:356-374: coefficients of the half
+// spectrum that are self-conjugate or whose conjugate partner is also stored.
+__device__ inline bool is_hermitian(int k0, int k1, int k2, int N0, int N1, int N2) {
+  if ((k0 == 0 || k0 == -N0 / 2) && (k1 == 0 || k1 == -N1 / 2) && (k2 == 0 || k2 == -N2 / 2)) return true;
+  if (k1 > 0 && (k2 == 0 || k2 == -N2 / 2)) return true;
+  if (k0 > 0 && (k1 == 0 || k1 == -N1 / 2) && (k2 == 0 || k2 == -N2 / 2)) return true;
+  return false;
+}
+
+// c2r trafo NDFT (:464-476, :524-533, :569-573, :594-603): f = Re f_0 + sum' 2 Re(f_k e^{-2 pi i k x}),
+// grad_t = -2 pi sum' 2 k_t Im(f_k e^{-2 pi i k x}).
+template <class R>
+__global__ __launch_bounds__(256) void ndft_trafo_c2r_k(NdftArgs<R> A) {
+  const long j = (long)blockIdx.x * 256 + threadIdx.x;
+  if (j >= A.M) return;
+  const int N0 = A.N[0], N1 = A.N[1], N2 = A.N[2], H2 = N2 / 2 + 1;
+  const R twopi = R(6.28318530717958647692);
+  const R x0 = A.x[3 * j], x1 = A.x[3 * j + 1], x2 = A.x[3 * j + 2];
+  R f = 0, g0 = 0, g1 = 0, g2 = 0;
+  for (int i0 = 0; i0 < N0; i0++)
+    for (int i1 = 0; i1 < N1; i1++)
+      for (int i2 = 0; i2 < H2; i2++) {
+        const int k0 = i0 - N0 / 2, k1 = i1 - N1 / 2, k2 = i2 - N2 / 2;
+        const Cplx<R> c = A.fhat[((long)i0 * N1 + i1) * H2 + i2];
+        if (k0 == 0 && k1 == 0 && k2 == 0) {
+          f += c.x;
+        } else if (!is_hermitian(k0, k1, k2, N0, N1, N2)) {
+          R sn, cs;
+          sc(-twopi * (R(k0) * x0 + R(k1) * x1 + R(k2) * x2), &sn, &cs);
+          const Cplx<R> z = cmul(c, cmk<R>(cs, sn));
+          f += R(2) * z.x;
+          g0 += R(2 * k0) * z.y;
+          g1 += R(2 * k1) * z.y;
+          g2 += R(2 * k2) * z.y;
+        }
+      }
+  R* fr = reinterpret_cast<R*>(A.f);
+  R* gr = reinterpret_cast<R*>(A.grad);
+  if (A.want_f) fr[j] = f;
+  if (A.want_grad) {
+    gr[3 * j + 0] = -twopi * g0;
+    gr[3 * j + 1] = -twopi * g1;
+    gr[3 * j + 2] = -twopi * g2;
+  }
+}
+
+// c2r adjoint NDFT: the c2c sums over the stored half with real f and grad (:667-715)
+template <class R>
+__global__ __launch_bounds__(256) void ndft_adj_c2r_k(NdftArgs<R> A) {
+  const long kidx = (long)blockIdx.x * 256 + threadIdx.x;
+  const int N0 = A.N[0], N1 = A.N[1], N2 = A.N[2], H2 = N2 / 2 + 1;
+  if (kidx >= (long)N0 * N1 * H2) return;
+  const int i2 = (int)(kidx % H2), i1 = (int)((kidx / H2) % N1), i0 = (int)(kidx / ((long)N1 * H2));
+  const R k0 = R(i0 - N0 / 2), k1 = R(i1 - N1 / 2), k2 = R(i2 - N2 / 2);
+  const R twopi = R(6.28318530717958647692);
+  const R* fr = reinterpret_cast<const R*>(A.f);
+  const R* gr = reinterpret_cast<const R*>(A.grad);
+  Cplx<R> acc = cmk<R>(0, 0);
+  for (long j = 0; j < A.M; j++) {
+    R sn, cs;
+    sc(twopi * (k0 * A.x[3 * j] + k1 * A.x[3 * j + 1] + k2 * A.x[3 * j + 2]), &sn, &cs);
+    Cplx<R> c = cmk<R>(A.want_f ? fr[j] : R(0), R(0));
+    if (A.want_grad) c.y += twopi * (k0 * gr[3 * j] + k1 * gr[3 * j + 1] + k2 * gr[3 * j + 2]);
+    acc = cadd(acc, cmul(c, cmk<R>(cs, sn)));
+  }
+  A.fhat[kidx] = cadd(A.fhat[kidx], acc);
+}
+
 }  // namespace
 
 template <class R>
 int ndft_trafo(const NdftArgs<R>& A, hipStream_t s) {
   if (A.M == 0) return kOk;
-  hipLaunchKernelGGL((ndft_trafo_k<R>), dim3((unsigned)((A.M + 255) / 256)), dim3(256), 0, s, A);
+  if (A.c2r)
+    hipLaunchKernelGGL((ndft_trafo_c2r_k<R>), dim3((unsigned)((A.M + 255) / 256)), dim3(256), 0, s, A);
+  else
+    hipLaunchKernelGGL((ndft_trafo_k<R>), dim3((unsigned)((A.M + 255) / 256)), dim3(256), 0, s, A);
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
 
 template <class R>
 int ndft_adj(const NdftArgs<R>& A, hipStream_t s) {
-  const long K = (long)A.N[0] * A.N[1] * A.N[2];
+  const long K = (long)A.N[0] * A.N[1] * (A.c2r ? A.N[2] / 2 + 1 : A.N[2]);
   if (K == 0) return kOk;
-  hipLaunchKernelGGL((ndft_adj_k<R>), dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, A);
+  if (A.c2r)
+    hipLaunchKernelGGL((ndft_adj_c2r_k<R>), dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, A);
+  else
+    hipLaunchKernelGGL((ndft_adj_k<R>), dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, A);
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
 

@@ -17,6 +17,7 @@ struct NdftArgs {
   Cplx<R>* grad;
   int want_f;
   int want_grad;
+  int c2r;  // real plan: f_hat is the half spectrum k2 in [-N2/2, 0]; f, grad are real arrays
 };
 
 template <class R>

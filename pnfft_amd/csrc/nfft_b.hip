@@ -312,7 +312,10 @@ int scatter(const BArgs<R>& A, int mode, hipStream_t s) {
   switch (mode) {
     case B_F: return b_launch<R, 1, B_F>(A, s);
     case B_GRAD: return b_launch<R, 1, B_GRAD>(A, s);
-    case B_FGRAD: return b_launch<R, 1, B_FGRAD>(A, s);
+    case B_FGRAD: {  // both contributions add into the same grid
+      const int st = b_launch<R, 1, B_F>(A, s);
+      return st != kOk ? st : b_launch<R, 1, B_GRAD>(A, s);
+    }
     default: return kUnsupported;
   }
 }

@@ -23,6 +23,7 @@ struct BArgs {
   R* gx;                   // VJP output R[3M]
   int accumulate;          // gather: add onto outputs
   int real_only;           // gather: PNFFT_REAL_F, write the real part only
+  int c2r;                 // real plan: f is R[M], grad R[3M], w R[M]; the real grid part is used
   // tiled path
   const int* tile_start;   // [ntiles+1] offsets into perm
   const unsigned* keys;    // sorted bin keys [M]

@@ -22,7 +22,7 @@ def _ptr(t):
 
 class DevicePlan:
     def __init__(self, N, n=None, m=6, flags=0, M=0, precision="double", x_max=(0.5, 0.5, 0.5),
-                 device="cuda:0", alloc_grad=True):
+                 device="cuda:0", alloc_grad=True, c2r=False):
         self.precision = precision
         self.api = lib().api(precision)
         self.N = tuple(int(v) for v in N)
@@ -30,19 +30,23 @@ class DevicePlan:
         self.m = int(m)
         self.flags = int(flags)
         self.M = int(M)
+        self.c2r = bool(c2r)
         self.device = torch.device(device)
         R = self.api.real
-        self.plan = self.api.init_guru(3, self.api.arr3(self.N), self.api.arr3(self.n), (R * 3)(*x_max), self.m,
-                                       self.flags, 0, MPI_COMM_WORLD)
+        init = self.api.init_guru_c2r if self.c2r else self.api.init_guru
+        self.plan = init(3, self.api.arr3(self.N), self.api.arr3(self.n), (R * 3)(*x_max), self.m, self.flags, 0,
+                         MPI_COMM_WORLD)
         if not self.plan:
             err = self.api.last_error()
             raise RuntimeError("plan rejected: " + (err.decode() if err else "unknown"))
         self.nodes = self.api.init_nodes(self.M, 0)
         cdt, rdt = _CDT[precision], _RDT[precision]
+        vdt = rdt if self.c2r else cdt  # c2r plans: real f and grad_f
+        self.fh_shape = (self.N[0], self.N[1], self.N[2] // 2 + 1) if self.c2r else self.N
         self.x = torch.zeros((self.M, 3), dtype=rdt, device=self.device)
-        self.f = torch.zeros(self.M, dtype=cdt, device=self.device)
-        self.grad = torch.zeros((self.M, 3), dtype=cdt, device=self.device) if alloc_grad else None
-        self.f_hat = torch.zeros(self.N, dtype=cdt, device=self.device)
+        self.f = torch.zeros(self.M, dtype=vdt, device=self.device)
+        self.grad = torch.zeros((self.M, 3), dtype=vdt, device=self.device) if alloc_grad else None
+        self.f_hat = torch.zeros(self.fh_shape, dtype=cdt, device=self.device)
         self.api.set_x(_ptr(self.x), self.nodes)
         self.api.set_f(_ptr(self.f), self.nodes)
         if self.grad is not None:
@@ -55,7 +59,7 @@ class DevicePlan:
         self.api.set_x(_ptr(self.x), self.nodes)  # marks the bins stale
 
     def set_f_hat(self, fh):
-        self.f_hat.copy_(torch.as_tensor(fh).reshape(self.N).to(self.f_hat.dtype))
+        self.f_hat.copy_(torch.as_tensor(fh).reshape(self.fh_shape).to(self.f_hat.dtype))
 
     def set_f(self, f):
         self.f.copy_(torch.as_tensor(f).reshape(self.M).to(self.f.dtype))

@@ -238,6 +238,48 @@ def test_equispaced_round_trip(oracle):
     assert errs[1] < 1e-9 and errs[2] < 1e-13 and errs[0] > errs[1] > errs[2], errs
 
 
+# ---------------------------------------------------------------- c2r plans
+@pytest.mark.parametrize("wname,m", [("kb", 6), ("gauss", 8)])
+def test_c2r_matches_c2c_and_ndft(oracle, wname, m):
+    """tests/check_trafo_vs_ndft_c2r.c:147-175,202-252: c2c on the full Hermitian input equals c2r
+    on its half; both match the c2r NDFT at the window's accuracy."""
+    N = [8, 8, 8]
+    P = oracle.Plan(N, m=m, flags=WINDOWS[wname], c2r=True)
+    Pc = oracle.Plan(N, m=m, flags=WINDOWS[wname])
+    full = oracle.semirandom_hermitian(N)
+    half = oracle.semirandom_hermitian(N, half=True)
+    assert np.array_equal(full[:, :, : N[2] // 2 + 1], half)
+    x = np.random.default_rng(0).uniform(-0.5, 0.5, (300, 3))
+    f_fast, g_fast = P.trafo_c2r(x, half, compute_grad=True)
+    fc, gc = Pc.trafo(x, full.reshape(-1), compute_grad=True)
+    assert oracle.rel_l2(f_fast, fc.real) < 1e-14 and np.abs(fc.imag).max() < 1e-13 * np.abs(fc).max()
+    assert oracle.rel_l2(g_fast, gc.real) < 1e-14
+    f_nd, g_nd = oracle.ndft_trafo_c2r(N, x, half, compute_grad=True)
+    tol = 1e-10 if wname == "kb" else 1e-7
+    assert oracle.rel_l2(f_fast, f_nd) < tol
+    # the reference's c2r NDFT gradient (kernel/ndft-parallel.c:530-532, :596-599) is the
+    # negative of d/dx of its own c2r f; the fast path's window-derivative gather is d/dx.
+    assert oracle.rel_l2(g_fast, -g_nd) < 100 * tol
+    fv = np.random.default_rng(1).normal(size=300)
+    assert oracle.rel_l2(P.adj_c2r(x, f=fv), oracle.ndft_adj_c2r(N, x, f=fv)) < 100 * tol
+
+
+def test_c2r_grad_is_derivative(oracle):
+    N = [8, 8, 8]
+    P = oracle.Plan(N, m=6, c2r=True)
+    half = oracle.semirandom_hermitian(N, half=True)
+    x = np.random.default_rng(2).uniform(-0.45, 0.45, (20, 3))
+    _, g = P.trafo_c2r(x, half, compute_grad=True)
+    g = g.reshape(-1, 3)
+    h = 1e-6
+    for t in range(3):
+        xp, xm = x.copy(), x.copy()
+        xp[:, t] += h
+        xm[:, t] -= h
+        fd = (P.trafo_c2r(xp, half)[0] - P.trafo_c2r(xm, half)[0]) / (2 * h)
+        assert np.max(np.abs(fd - g[:, t])) < 1e-5 * np.max(np.abs(g[:, t]))
+
+
 # ---------------------------------------------------------------- sort
 def test_reference_sort_matches_stable_argsort(oracle):
     n, m = [32, 16, 64], 3
