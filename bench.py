@@ -31,22 +31,58 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(w, precision):
-    """Per-launch algorithmic HBM bytes of each stage (grid read/write once)."""
+def algorithmic_bytes(w, precision, c2r=False):
+    """Algorithmic HBM bytes of one launch of each kernel class (every array read or written once).
+
+    FFT passes: elements in + elements out (the pruned sides hold N, the padded sides n points);
+    the backward rows pass adds D^T onto f_hat (read + write).  B stages: the oversampled grid once
+    plus per node x (3 reals), its sort position (4 B) and f (1 complex) or grad_f (3 complex).
+    """
     cs = 8 if precision == "float" else 16
     rs = cs // 2
-    Nt = int(np.prod(w["N"]))
     nt = int(np.prod(w["n"]))
     M = w["M"]
     N0, N1, N2 = w["N"]
     n0, n1, n2 = w["n"]
-    fft_pruned = cs * (Nt + N0 * N1 * n2 + N0 * N1 * n2 + N0 * n1 * n2 + N0 * n1 * n2 + nt)
+    nfh = N0 * N1 * (N2 // 2 + 1 if c2r else N2)
+    b1 = N0 * N1 * n2
+    b2 = N0 * n1 * n2
+    node = 3 * rs + 4
     return {
-        "gather_f": cs * nt + 3 * rs * M + 4 * M + cs * M,
-        "scatter_f": cs * nt + 3 * rs * M + 4 * M + cs * M,
-        "gather_grad": cs * nt + 3 * rs * M + 4 * M + 3 * cs * M,
-        "fft_pruned": fft_pruned,
+        "fft_rows_fwd": cs * (nfh + b1),
+        "fft_rows_bwd": cs * (b1 + 2 * nfh),
+        "fft_cols1": cs * (b1 + b2),
+        "fft_cols0": cs * (b2 + nt),
+        "gather_f": cs * nt + M * (node + cs),
+        "gather_grad": cs * nt + M * (node + 3 * cs),
+        "scatter_f": cs * nt + M * (node + cs),
+        "grid_zero": cs * nt,
     }
+
+
+def roofline_from_kernels(kt, ab, steps):
+    """Dominant kernel class of one step and its achieved HBM rate.
+
+    kt: {class: (ms, launches)} over the timed steps (HIP events on the plan's stream).  One step
+    launches: FFT rows 2 forward + 1 backward, cols1 3, cols0 3; gather (tiled + direct pair) 2
+    (f, grad); scatter (tiled + direct pair) 1; grid clearing 1.
+    """
+    def ms(k):
+        return kt[k][0] / steps
+
+    classes = {
+        "fft_rows": (ms("fft_rows"), 2 * ab["fft_rows_fwd"] + ab["fft_rows_bwd"], 3),
+        "fft_cols1": (ms("fft_cols1"), 3 * ab["fft_cols1"], 3),
+        "fft_cols0": (ms("fft_cols0"), 3 * ab["fft_cols0"], 3),
+        "gather_tiled+direct": (ms("gather_tiled") + ms("gather_direct"), ab["gather_f"] + ab["gather_grad"], 2),
+        "scatter_tiled+direct": (ms("scatter_tiled") + ms("scatter_direct"), ab["scatter_f"], 1),
+        "grid_zero": (ms("grid_zero"), ab["grid_zero"], 1),
+    }
+    dom = max(classes, key=lambda k: classes[k][0])
+    t_ms, nbytes, nl = classes[dom]
+    achieved = (nbytes / nl) / (t_ms / nl * 1e-3) / 1e9 if t_ms > 0 else 0.0
+    per_kernel = {k: round(v[0], 3) for k, v in classes.items()}
+    return dom, achieved, nbytes / nl, t_ms / nl, per_kernel
 
 
 def run_plan(w, precision, steps, warmup, x_np):
@@ -91,7 +127,7 @@ def run_plan(w, precision, steps, warmup, x_np):
     for i in range(warmup):
         step()
     torch.cuda.synchronize()
-    dp.api.reset_timer(dp.plan)
+    dp.reset_timers()
     t0 = time.perf_counter()
     for i in range(steps):
         step()
@@ -99,10 +135,11 @@ def run_plan(w, precision, steps, warmup, x_np):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     tt, ta = dp.timers()
+    kt = dp.kernel_times()
     err = dp.api.last_error()
     if err:
         raise RuntimeError(err.decode())
-    res = dict(ms_per_step=1e3 * el / steps, setup_ms=setup_ms, timer_trafo=tt, timer_adj=ta,
+    res = dict(ms_per_step=1e3 * el / steps, setup_ms=setup_ms, timer_trafo=tt, timer_adj=ta, kernel_times=kt,
                out=(out_f.cpu().numpy(), out_fh.cpu().numpy().reshape(-1), out_g.cpu().numpy().reshape(-1)))
     dp.close()
     del dp
@@ -142,17 +179,25 @@ def cpu_baseline(w, sample_M, threads):
                   None, dp(gr.view(np.float64)), 2)
     t["Bgrad"] = time.perf_counter() - t0
     del gpad
+    fsc = np.ascontiguousarray(fs.astype(np.complex128).reshape(-1))
+    gpad = np.zeros([v + 2 * P.m for v in P.no], np.complex128)
     t0 = time.perf_counter()
-    P.adj(xs, f=fs)  # B^T (serial scatter) + ghost fold + F^H + D^T
-    t["adj"] = time.perf_counter() - t0
+    L.orc_B_adj_blocks(lp(n3), lp(no3), P.m, P.window, dp(P.b), sample_M, dp(xsc), dp(fsc.view(np.float64)), None,
+                       0, dp(gpad.view(np.float64)), 1)
+    t["BT"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fh = np.zeros(int(np.prod(P.N)), np.complex128)
+    P.DT_accumulate(P.FH(P.ghost_reduce(gpad)), fh)  # ghost fold + F^H + D^T
+    t["FDT"] = time.perf_counter() - t0
+    del gpad, fh
     scale = w["M"] / sample_M
-    # D+F is independent of M; the B stages scale with M (the adj time includes its FFT, counted once)
-    total = 2 * t["DF"] + (t["B"] + t["Bgrad"]) * scale + t["adj"] * scale
+    # D+F and F^H+D^T are independent of M (the step has two trafos and one adj); the B stages scale with M
+    total = 2 * t["DF"] + t["FDT"] + (t["B"] + t["Bgrad"] + t["BT"]) * scale
     return dict(value=w["M"] / total, unit="pts/s", cores=threads, kind="port",
                 sample=(f"FP64 oracle (reference-algorithm restatement) on N={w['N'][0]}^3 n={w['n'][0]}^3 "
                         f"m={w['m']}: D+F once ({t['DF']:.2f}s), gather f and grad and adj on {sample_M} of "
-                        f"{w['M']} nodes ({t['B']:.2f}+{t['Bgrad']:.2f}+{t['adj']:.2f}s), B and adj times "
-                        f"scaled by {scale:.0f} to the full node count"),
+                        f"{w['M']} nodes ({t['B']:.2f}+{t['Bgrad']:.2f}+{t['BT']:.2f}s) scaled by {scale:.0f} to "
+                        f"the full node count, F^H+D^T once ({t['FDT']:.2f}s); {threads} threads"),
                 seconds=t)
 
 
@@ -195,25 +240,14 @@ def main():
                for nm, a, b in zip(names, res["out"], fp64["out"])}
         log(f"  rel-l2 vs FP64: {acc}")
 
-    tt, ta = res["timer_trafo"], res["timer_adj"]
     steps = args.steps
-    # stage times per step (seconds): trafo runs twice per step (f, grad), adj once
-    per = {"trafo_F": tt[6] / steps, "trafo_B": tt[5] / steps, "adj_B": ta[5] / steps, "adj_F": ta[6] / steps}
     ab = algorithmic_bytes(w, precision)
-    # dominant kernel = the stage with the largest time
-    cands = {
-        "scatter_f": (ta[5] / steps, ab["scatter_f"], 1),
-        "gather_f+grad": (tt[5] / steps, ab["gather_f"] + ab["gather_grad"], 2),
-        "fft_pruned": ((tt[6] + ta[6]) / steps, ab["fft_pruned"], 3),
-    }
-    dom = max(cands, key=lambda k: cands[k][0])
-    dt, dbytes, nlaunch = cands[dom]
-    per_launch_s = dt / nlaunch
-    per_launch_bytes = dbytes / nlaunch
-    achieved = per_launch_bytes / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
+    dom, achieved, lbytes, lms, per_kernel = roofline_from_kernels(res["kernel_times"], ab, steps)
+    log(f"  dominant kernel {dom}: {lms:.3f} ms/launch, {lbytes / 1e9:.3f} GB/launch, {achieved:.0f} GB/s")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": dom,
-                "stage_ms_per_step": {k: round(1e3 * v, 3) for k, v in per.items()}}
+                "launch_ms": round(lms, 4), "algorithmic_bytes_per_launch": int(lbytes),
+                "kernel_ms_per_step": per_kernel}
     cpu = None
     if not args.no_cpu:
         log("  cpu baseline ...")

@@ -173,12 +173,29 @@ extern "C" {
    * node_bins: bins the nodes for the plan's grid (if not binned yet) and copies              \
    * the node processing order (node ids sorted stably by bin key, int[M]) and                \
    * the sorted keys (unsigned[M]) to host or device arrays; returns 0 on success.            \
+   * kernel_times: per-kernel device time (ms, from HIP events on the plan's                   \
+   * stream) and launch counts accumulated since the plan was made or                         \
+   * reset_timer was called, for the kernel classes PNFFT_KT_* below; copies                  \
+   * min(n, PNFFT_KT_COUNT) entries and returns PNFFT_KT_COUNT.                               \
    * last_error: message of the last rejected call, or NULL. */                               \
   void *PNX(malloc_device)(size_t n);                                                         \
   PNX(nodes) PNX(init_nodes_device)(INT local_M, unsigned malloc_flags);                      \
   void PNX(trafo_grad_x_vjp)(PNX(plan) ths, PNX(nodes) nodes, const C *w, R *grad_x);         \
   int PNX(node_bins)(PNX(plan) ths, PNX(nodes) nodes, int *order, unsigned *keys);            \
+  int PNX(kernel_times)(PNX(plan) ths, double *ms, long *launches, int n);                     \
   const char *PNX(last_error)(void);
+
+/* kernel classes of PNX(kernel_times) (extension) */
+#define PNFFT_KT_FFT_ROWS 0        /* axis-2 FFT pass (D or D^T fused) */
+#define PNFFT_KT_FFT_COLS1 1       /* axis-1 FFT pass */
+#define PNFFT_KT_FFT_COLS0 2       /* axis-0 FFT pass */
+#define PNFFT_KT_GATHER_TILED 3    /* trafo B, LDS-tiled kernel */
+#define PNFFT_KT_GATHER_DIRECT 4   /* trafo B, direct kernel (sparse tiles) */
+#define PNFFT_KT_SCATTER_TILED 5   /* adj B^T, LDS-tiled kernel */
+#define PNFFT_KT_SCATTER_DIRECT 6  /* adj B^T, direct kernel */
+#define PNFFT_KT_BINNING 7         /* node binning (keys, radix sort, work items) */
+#define PNFFT_KT_GRID_ZERO 8       /* clearing the oversampled grid before B^T */
+#define PNFFT_KT_COUNT 9
 
 #define PNFFT_MANGLE_DOUBLE(name) pnfft_##name
 #define PNFFT_MANGLE_FLOAT(name) pnfftf_##name

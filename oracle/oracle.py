@@ -74,6 +74,7 @@ def lib():
                                   ctypes.c_int, dp, dp, ctypes.c_uint]
         L.orc_B_adj.argtypes = [lp, lp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_long, dp, dp, dp,
                                 ctypes.c_int, dp, ctypes.c_uint]
+        L.orc_B_adj_blocks.argtypes = L.orc_B_adj.argtypes
         L.orc_sort_nodes.argtypes = [lp, ctypes.c_int, ctypes.c_long, dp, lp]
         _lib = L
     return _lib
@@ -250,6 +251,45 @@ def semirandom_hermitian(N, half=False):
 # ---------------------------------------------------------------------------
 # Fast NFFT (D -> F -> ghost cells -> B), c2c
 # ---------------------------------------------------------------------------
+def _embed(B, n):
+    """Centered array B (index k + N_t/2, k in [-N_t/2, N_t/2)) placed into a zero
+    array of shape n in DFT order (k mod n_t); even N_t <= n_t."""
+    if any(v % 2 for v in B.shape):
+        A = np.zeros(n, B.dtype)
+        idx = [np.arange(-(v // 2), v - v // 2) % nt for v, nt in zip(B.shape, n)]
+        A[np.ix_(*idx)] = B
+        return A
+    B = np.fft.ifftshift(B)
+    for t in range(3):
+        Nt, nt = B.shape[t], n[t]
+        if nt == Nt:
+            continue
+        h = Nt // 2
+        zshape = list(B.shape)
+        zshape[t] = nt - Nt
+        lo, hi = np.split(B, [h], axis=t)
+        B = np.concatenate([lo, np.zeros(zshape, B.dtype), hi], axis=t)
+    return B
+
+
+def _extract(G, N):
+    """Inverse of _embed: the entries k in [-N_t/2, N_t/2) of the DFT-ordered G, centered."""
+    if any(v % 2 for v in N):
+        idx = [np.arange(-(v // 2), v - v // 2) % nt for v, nt in zip(N, G.shape)]
+        return G[np.ix_(*idx)]
+    for t in range(3):
+        Nt, nt = N[t], G.shape[t]
+        if nt == Nt:
+            continue
+        h = Nt // 2
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[t] = slice(0, h)
+        hi[t] = slice(nt - h, nt)
+        G = np.concatenate([G[tuple(lo)], G[tuple(hi)]], axis=t)
+    return np.fft.fftshift(G)
+
+
 class Plan:
     """Parameters of one oracle plan (api/api-guru.c:110-164, kernel/ndft-parallel.c:869-1069).
 
@@ -354,22 +394,15 @@ class Plan:
     # -- F: pruned, shifted, unnormalized forward FFT N -> no (doc/manual.tex:178-243)
     def F(self, g1):
         N, n, no = self.N, self.n, self.no
-        A = np.zeros(n, np.complex128)
-        idx = [np.arange(-Nt // 2, Nt // 2) % nt for Nt, nt in zip(N, n)]
-        A[np.ix_(*idx)] = np.asarray(g1).reshape(N)
-        G = scipy.fft.fftn(A, workers=self.workers)
-        oidx = [np.arange(-nt // 2, nt // 2) % nft for nt, nft in zip(no, n)]
-        return G[np.ix_(*oidx)]
+        G = scipy.fft.fftn(_embed(np.asarray(g1).reshape(N), n), workers=self.workers)
+        return _extract(G, no)
 
     # -- F^H: backward FFT no -> N
     def FH(self, g2):
         N, n, no = self.N, self.n, self.no
-        A = np.zeros(n, np.complex128)
-        oidx = [np.arange(-nt // 2, nt // 2) % nft for nt, nft in zip(no, n)]
-        A[np.ix_(*oidx)] = g2
-        G = scipy.fft.ifftn(A, workers=self.workers) * float(np.prod(n))
-        idx = [np.arange(-Nt // 2, Nt // 2) % nt for Nt, nt in zip(N, n)]
-        return G[np.ix_(*idx)].reshape(-1)
+        G = scipy.fft.ifftn(_embed(np.asarray(g2).reshape(no), n), workers=self.workers)
+        G *= float(np.prod(n))
+        return _extract(G, N).reshape(-1)
 
     # -- ghost cells (PX(exchange)/PX(reduce), kernel/ndft-parallel.c:2558,2679):
     # m planes below, m above, periodic over the FFT output of size no.
@@ -378,16 +411,21 @@ class Plan:
         return np.pad(g2, [(m, m)] * 3, mode="wrap")
 
     def ghost_reduce(self, gpad):
+        """Fold every padded index i of each axis back to (i - m) mod no, axis by axis.
+        Works in place on gpad (overwritten) and returns the folded no-shaped array."""
         m = self.m
-        arr = gpad
-        # fold every padded index i of each axis back to (i - m) mod no, axis by axis
+
+        def sl(t, a, b):
+            r = [slice(None)] * 3
+            r[t] = slice(a, b)
+            return tuple(r)
+
         for t in range(3):
             nt = self.no[t]
-            moved = np.moveaxis(arr, t, 0)
-            res = np.zeros((nt,) + moved.shape[1:], arr.dtype)
-            np.add.at(res, (np.arange(moved.shape[0]) - m) % nt, moved)
-            arr = np.moveaxis(res, 0, t)
-        return arr
+            core = gpad[sl(t, m, m + nt)]
+            core[sl(t, nt - m, nt)] += gpad[sl(t, 0, m)]  # i in [0, m) -> nt - m + i
+            core[sl(t, 0, m)] += gpad[sl(t, m + nt, nt + 2 * m)]  # i in [m + nt, nt + 2m) -> i - m - nt
+        return np.ascontiguousarray(gpad[m:m + self.no[0], m:m + self.no[1], m:m + self.no[2]])
 
     def trafo(self, x, fhat, compute_f=True, compute_grad=False):
         """pnfft_trafo (api/api-basic.c:199-244) on host arrays; returns (f, grad)."""

@@ -570,48 +570,101 @@ void orc_B_trafo(const long *n, const long *no, int m, int window, const double 
   }
 }
 
+/* Scatter of one node into the ghost-padded grid (the inner loops of
+ * kernel/assign.c:478-1066 for the adjoint). */
+static void scatter_one(const long *n, const long *no, int m, int window, const double *b, long j,
+                        const double *x, const double *f, const double *grad, int is_real, double *grid,
+                        int use_f, int use_g)
+{
+  const int c = 2 * m + 1;
+  const long g1 = no[1] + 2 * m, g2 = no[2] + 2 * m;
+  const int cplx = is_real ? 1 : 2;
+  double psi[3 * 64], dpsi[3 * 64];
+  long u[3];
+  node_windows(window, n, b, m, x + 3 * j, psi, use_g ? dpsi : NULL, u, no);
+  double fr = 0, fi = 0, gv[6] = {0, 0, 0, 0, 0, 0};
+  if (use_f) { fr = is_real ? f[j] : f[2 * j]; fi = is_real ? 0.0 : f[2 * j + 1]; }
+  if (use_g) for (int t = 0; t < 3; t++) {
+    gv[2 * t] = is_real ? grad[3 * j + t] : grad[6 * j + 2 * t];
+    gv[2 * t + 1] = is_real ? 0.0 : grad[6 * j + 2 * t + 1];
+  }
+  for (int l0 = 0; l0 < c; l0++)
+    for (int l1 = 0; l1 < c; l1++) {
+      double pxy = psi[l0] * psi[c + l1];
+      double pdxy = use_g ? dpsi[l0] * psi[c + l1] : 0.0;
+      double pxdy = use_g ? psi[l0] * dpsi[c + l1] : 0.0;
+      long row = ((u[0] + l0) * g1 + (u[1] + l1)) * g2 + u[2];
+      for (int l2 = 0; l2 < c; l2++) {
+        double *gp = grid + cplx * (row + l2);
+        double pz = psi[2 * c + l2];
+        double ar = 0, ai = 0;
+        if (use_f) { double w = pxy * pz; ar += w * fr; ai += w * fi; }
+        if (use_g) {
+          double w0 = pdxy * pz, w1 = pxdy * pz, w2 = pxy * dpsi[2 * c + l2];
+          ar += w0 * gv[0] + w1 * gv[2] + w2 * gv[4];
+          ai += w0 * gv[1] + w1 * gv[3] + w2 * gv[5];
+        }
+        gp[0] += ar;
+        if (!is_real) gp[1] += ai;
+      }
+    }
+}
+
 /* Adjoint B^T: scatter into the zeroed ghost-padded grid, node order as given
  * (sequential, so the accumulation order is the reference's unsorted order). */
 void orc_B_adj(const long *n, const long *no, int m, int window, const double *b, long M,
                const double *x, const double *f, const double *grad, int is_real, double *grid,
                unsigned cflags)
 {
-  const int c = 2 * m + 1;
-  const long g1 = no[1] + 2 * m, g2 = no[2] + 2 * m;
   const int use_f = (cflags & C_COMPUTE_F) && f;
   const int use_g = (cflags & C_COMPUTE_GRAD_F) && grad;
-  const int cplx = is_real ? 1 : 2;
+  for (long j = 0; j < M; j++) scatter_one(n, no, m, window, b, j, x, f, grad, is_real, grid, use_f, use_g);
+}
+
+/* The same scatter on all OpenMP threads (CPU-baseline timing only).  Nodes
+ * are grouped by the block (u0 / c, u1 / c) of their first stencil cell,
+ * c = 2m+1; a node of block (s0, s1) writes padded planes [s0 c, s0 c + 2c - 1)
+ * x [s1 c, s1 c + 2c - 1) only, so blocks whose indices have the same parity
+ * in both axes never touch the same cell and run concurrently, four phases in
+ * all.  Within a block the nodes keep their input order. */
+void orc_B_adj_blocks(const long *n, const long *no, int m, int window, const double *b, long M,
+                      const double *x, const double *f, const double *grad, int is_real, double *grid,
+                      unsigned cflags)
+{
+  const int c = 2 * m + 1;
+  const int use_f = (cflags & C_COMPUTE_F) && f;
+  const int use_g = (cflags & C_COMPUTE_GRAD_F) && grad;
+  const long nb0 = (no[0] + 2 * m + c - 1) / c, nb1 = (no[1] + 2 * m + c - 1) / c;
+  const long nblk = nb0 * nb1;
+  long *blk = malloc(sizeof(long) * (M > 0 ? M : 1));
+  long *start = calloc((size_t)nblk + 1, sizeof(long));
+  long *order = malloc(sizeof(long) * (M > 0 ? M : 1));
   for (long j = 0; j < M; j++) {
-    double psi[3 * 64], dpsi[3 * 64];
-    long u[3];
-    node_windows(window, n, b, m, x + 3 * j, psi, use_g ? dpsi : NULL, u, no);
-    double fr = 0, fi = 0, gv[6] = {0, 0, 0, 0, 0, 0};
-    if (use_f) { fr = is_real ? f[j] : f[2 * j]; fi = is_real ? 0.0 : f[2 * j + 1]; }
-    if (use_g) for (int t = 0; t < 3; t++) {
-      gv[2 * t] = is_real ? grad[3 * j + t] : grad[6 * j + 2 * t];
-      gv[2 * t + 1] = is_real ? 0.0 : grad[6 * j + 2 * t + 1];
-    }
-    for (int l0 = 0; l0 < c; l0++)
-      for (int l1 = 0; l1 < c; l1++) {
-        double pxy = psi[l0] * psi[c + l1];
-        double pdxy = use_g ? dpsi[l0] * psi[c + l1] : 0.0;
-        double pxdy = use_g ? psi[l0] * dpsi[c + l1] : 0.0;
-        long row = ((u[0] + l0) * g1 + (u[1] + l1)) * g2 + u[2];
-        for (int l2 = 0; l2 < c; l2++) {
-          double *gp = grid + cplx * (row + l2);
-          double pz = psi[2 * c + l2];
-          double ar = 0, ai = 0;
-          if (use_f) { double w = pxy * pz; ar += w * fr; ai += w * fi; }
-          if (use_g) {
-            double w0 = pdxy * pz, w1 = pxdy * pz, w2 = pxy * dpsi[2 * c + l2];
-            ar += w0 * gv[0] + w1 * gv[2] + w2 * gv[4];
-            ai += w0 * gv[1] + w1 * gv[3] + w2 * gv[5];
-          }
-          gp[0] += ar;
-          if (!is_real) gp[1] += ai;
-        }
-      }
+    long u[2];
+    for (int t = 0; t < 2; t++) u[t] = (long)floor((double)n[t] * x[3 * j + t]) - m + no[t] / 2 + m;
+    blk[j] = (u[0] / c) * nb1 + u[1] / c;
+    start[blk[j] + 1]++;
   }
+  for (long k = 0; k < nblk; k++) start[k + 1] += start[k];
+  {
+    long *fill = malloc(sizeof(long) * (size_t)(nblk > 0 ? nblk : 1));
+    memcpy(fill, start, sizeof(long) * (size_t)nblk);
+    for (long j = 0; j < M; j++) order[fill[blk[j]]++] = j;
+    free(fill);
+  }
+  for (int ph = 0; ph < 4; ph++) {
+    const long p0 = ph >> 1, p1 = ph & 1;
+    const long h0 = (nb0 - p0 + 1) / 2, h1 = (nb1 - p1 + 1) / 2;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long q = 0; q < h0 * h1; q++) {
+      const long k = (p0 + 2 * (q / h1)) * nb1 + p1 + 2 * (q % h1);
+      for (long i = start[k]; i < start[k + 1]; i++)
+        scatter_one(n, no, m, window, b, order[i], x, f, grad, is_real, grid, use_f, use_g);
+    }
+  }
+  free(blk);
+  free(start);
+  free(order);
 }
 
 /* ---------------------------------------------------------------------------

@@ -28,6 +28,20 @@
 #include "nfft_b.hpp"
 #include "window_math.hpp"
 
+// kernel classes of pnfft_kernel_times (the PNFFT_KT_* values of include/pnfft.h)
+enum : int {
+  PNFFT_KT_FFT_ROWS = 0,
+  PNFFT_KT_FFT_COLS1 = 1,
+  PNFFT_KT_FFT_COLS0 = 2,
+  PNFFT_KT_GATHER_TILED = 3,
+  PNFFT_KT_GATHER_DIRECT = 4,
+  PNFFT_KT_SCATTER_TILED = 5,
+  PNFFT_KT_SCATTER_DIRECT = 6,
+  PNFFT_KT_BINNING = 7,
+  PNFFT_KT_GRID_ZERO = 8,
+  PNFFT_KT_COUNT = 9
+};
+
 // flag values (api/pnfft.h:302-390)
 namespace fl {
 constexpr unsigned PRE_PHI_HAT = 1u << 0;
@@ -160,8 +174,10 @@ static bool is_device_ptr(const void* p) {
 // ---------------------------------------------------------------------------
 // plan and nodes
 // ---------------------------------------------------------------------------
+// Event slots: 0/1 binning, 2/3 FFT (8, 9 after its first and second pass),
+// 4/5 B (10 between the tiled and the direct kernel), 11/12 grid clearing.
 struct EventTimer {
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[16] = {};
   bool ok = false;
   void init() {
     ok = true;
@@ -175,6 +191,7 @@ struct EventTimer {
   void rec(int i, hipStream_t s) {
     if (ok) (void)hipEventRecord(ev[i], s);
   }
+  hipEvent_t mark(int i) const { return ok ? ev[i] : nullptr; }
   double ms(int a, int b) {
     float t = 0;
     if (!ok || hipEventElapsedTime(&t, ev[a], ev[b]) != hipSuccess) {
@@ -215,6 +232,8 @@ struct PlanT {
   EventTimer evt;
   double timer_trafo[T_LEN] = {0};
   double timer_adj[T_LEN] = {0};
+  double ktime[PNFFT_KT_COUNT] = {0};  // per-kernel-class device ms (pnfft_kernel_times)
+  long kcount[PNFFT_KT_COUNT] = {0};
 };
 
 template <class R>
@@ -570,17 +589,35 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx) {
 }
 
 template <class R>
-static void finish_timers(PlanT<R>* P, double* timer, bool had_sort, bool had_f, bool had_b) {
-  if (had_sort) timer[T_SORT] += 1e-3 * P->evt.ms(0, 1);
+static void kt_add(PlanT<R>* P, int k, int a, int b) {
+  P->ktime[k] += P->evt.ms(a, b);
+  P->kcount[k] += 1;
+}
+
+// Accumulates the reference's stage timers (seconds) and the per-kernel times.
+// fwd: trafo (FFT passes rows, cols1, cols0; gather) or adj (cols0, cols1, rows; scatter).
+template <class R>
+static void finish_timers(PlanT<R>* P, double* timer, bool fwd, bool had_sort, bool had_f, bool had_b,
+                          bool had_zero = false) {
+  if (had_sort) {
+    timer[T_SORT] += 1e-3 * P->evt.ms(0, 1);
+    kt_add(P, PNFFT_KT_BINNING, 0, 1);
+  }
   if (had_f) {
     const double t = 1e-3 * P->evt.ms(2, 3);
     timer[T_F] += t;
+    kt_add(P, fwd ? PNFFT_KT_FFT_ROWS : PNFFT_KT_FFT_COLS0, 2, 8);
+    kt_add(P, PNFFT_KT_FFT_COLS1, 8, 9);
+    kt_add(P, fwd ? PNFFT_KT_FFT_COLS0 : PNFFT_KT_FFT_ROWS, 9, 3);
   }
   if (had_b) {
     const double t = 1e-3 * P->evt.ms(4, 5);
     timer[T_B] += t;
     timer[T_LOOP_B] += t;
+    kt_add(P, fwd ? PNFFT_KT_GATHER_TILED : PNFFT_KT_SCATTER_TILED, 4, 10);
+    kt_add(P, fwd ? PNFFT_KT_GATHER_DIRECT : PNFFT_KT_SCATTER_DIRECT, 10, 5);
   }
+  if (had_zero) kt_add(P, PNFFT_KT_GRID_ZERO, 11, 12);
 }
 
 // api/api-basic.c:199-244
@@ -631,7 +668,8 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     if (!(cf & fl::OMIT_FFT)) {
       P->evt.rec(2, s);
       const double* tab = (cf & fl::OMIT_DECONV) ? nullptr : P->d_tab;
-      if (fft3d_forward<R>(P->fft, dfh, tab, P->d_buf1, P->d_buf2, P->d_grid, s, P->c2r) != kOk)
+      if (fft3d_forward<R>(P->fft, dfh, tab, P->d_buf1, P->d_buf2, P->d_grid, s, P->c2r, P->evt.mark(8),
+                           P->evt.mark(9)) != kOk)
         set_error("FFT launch failed");
       P->evt.rec(3, s);
       had_f = true;
@@ -658,6 +696,7 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
       A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
       A.c2r = P->c2r;
       set_tiles(A, D);
+      A.mid = P->evt.mark(10);
       int mode = (want_f && want_g) ? B_FGRAD : (want_f ? B_F : B_GRAD);
       P->evt.rec(4, s);
       if (gather<R>(A, mode, s) != kOk) set_error("gather launch failed");
@@ -671,7 +710,7 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     set_error("trafo: HIP error %s", hipGetErrorString(hipGetLastError()));
     return;
   }
-  finish_timers(P, P->timer_trafo, had_sort, had_f, had_b);
+  finish_timers(P, P->timer_trafo, true, had_sort, had_f, had_b);
   P->timer_trafo[T_ITER] += 1;
   P->timer_trafo[T_WHOLE] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -701,7 +740,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
   // zero f_hat unless accumulating (:355-356)
   if (!accumulate) fill_zero<R>(dfh, P->fhat_elems, s);
 
-  bool had_sort = false, had_f = false, had_b = false;
+  bool had_sort = false, had_f = false, had_b = false, had_zero = false;
   if (cf & fl::COMPUTE_DIRECT) {
     NdftArgs<R> A{};
     for (int t = 0; t < 3; t++) A.N[t] = (int)P->N[t];
@@ -717,7 +756,10 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
   } else {
     if (!(cf & fl::OMIT_CONV)) {
       // B^T into the zeroed grid (:2629-2635)
+      P->evt.rec(11, s);
       fill_zero<R>(P->d_grid, P->n_total, s);
+      P->evt.rec(12, s);
+      had_zero = true;
       if (M > 0 && (use_f || use_g)) {
         const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x &&
                            same_geom(D->bin_geom, P->geom);
@@ -737,6 +779,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
         A.grad = dg;
         A.c2r = P->c2r;
         set_tiles(A, D, true);
+        A.mid = P->evt.mark(10);
         const int mode = (use_f && use_g) ? B_FGRAD : (use_f ? B_F : B_GRAD);
         P->evt.rec(4, s);
         if (scatter<R>(A, mode, s) != kOk) set_error("scatter launch failed");
@@ -747,7 +790,8 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     // F^H and D^T (fused), f_hat += D^T F^H grid
     if (!(cf & fl::OMIT_FFT) && !(cf & fl::OMIT_DECONV)) {
       P->evt.rec(2, s);
-      if (fft3d_backward<R>(P->fft, P->d_grid, P->d_tab, P->d_buf1, P->d_buf2, dfh, 1, s, P->c2r) != kOk)
+      if (fft3d_backward<R>(P->fft, P->d_grid, P->d_tab, P->d_buf1, P->d_buf2, dfh, 1, s, P->c2r,
+                            P->evt.mark(8), P->evt.mark(9)) != kOk)
         set_error("FFT launch failed");
       P->evt.rec(3, s);
       had_f = true;
@@ -758,7 +802,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     set_error("adj: HIP error %s", hipGetErrorString(hipGetLastError()));
     return;
   }
-  finish_timers(P, P->timer_adj, had_sort, had_f, had_b);
+  finish_timers(P, P->timer_adj, false, had_sort, had_f, had_b, had_zero);
   P->timer_adj[T_ITER] += 1;
   P->timer_adj[T_WHOLE] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -1097,6 +1141,16 @@ using namespace pnfft_amd;
   void PREFIX##reset_timer(void* p) {                                                                       \
     auto* P = static_cast<PlanT<R>*>(p);                                                                    \
     for (int i = 0; i < T_LEN; i++) P->timer_trafo[i] = P->timer_adj[i] = 0;                                 \
+    for (int i = 0; i < PNFFT_KT_COUNT; i++) { P->ktime[i] = 0; P->kcount[i] = 0; }                          \
+  }                                                                                                        \
+  int PREFIX##kernel_times(void* p, double* ms, long* launches, int n) {                                     \
+    auto* P = static_cast<PlanT<R>*>(p);                                                                    \
+    if (!P) return PNFFT_KT_COUNT;                                                                          \
+    for (int i = 0; i < n && i < PNFFT_KT_COUNT; i++) {                                                     \
+      if (ms) ms[i] = P->ktime[i];                                                                          \
+      if (launches) launches[i] = P->kcount[i];                                                             \
+    }                                                                                                      \
+    return PNFFT_KT_COUNT;                                                                                  \
   }                                                                                                        \
   static void PREFIX##print_timer(const double* tm, const char* name) {                                      \
     double t[T_LEN]; memcpy(t, tm, sizeof(t));                                                              \

@@ -30,6 +30,14 @@ namespace {
 
 constexpr int kB = 256;  // threads per block, direct kernels
 
+// Kernel bodies with compile-time stencil loops (window arrays in registers)
+// are used for FP32 at every m and for FP64 up to m = 6; FP64 at m = 7, 8 keeps
+// the runtime-width loops (the static bodies there need more than 256 VGPRs).
+template <class R, int MT>
+constexpr bool static_body() {
+  return MT > 0 && (sizeof(R) == 4 || MT <= 6);
+}
+
 template <class R, int CM>
 struct NodeWin {
   int start[3];  // natural index of stencil point 0 per axis
@@ -37,7 +45,10 @@ struct NodeWin {
   R dx[CM], dy[CM], dz[CM];
 };
 
-template <class R, int CM, bool WG>
+// Window values of one node.  MT > 0: the width CM = 2 MT + 1 is a compile-time
+// constant and every array index is static, so NodeWin lives in registers.
+// MT == 0 (generic m): runtime width m, arrays of kMaxC entries.
+template <class R, int MT, int CM, bool WG>
 __device__ inline void node_windows(const BArgs<R>& A, long j, int m, NodeWin<R, CM>& w) {
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
   const NodeAxis<R> a0 = node_axis<R>(A.x[3 * j + 0], n0, m);
@@ -46,16 +57,82 @@ __device__ inline void node_windows(const BArgs<R>& A, long j, int m, NodeWin<R,
   w.start[0] = a0.start;
   w.start[1] = a1.start;
   w.start[2] = a2.start;
-  window_axis<R>(A.g.win, m, a0.frac, (R)n0, (R)A.g.b[0], w.px, WG ? w.dx : nullptr);
-  window_axis<R>(A.g.win, m, a1.frac, (R)n1, (R)A.g.b[1], w.py, WG ? w.dy : nullptr);
-  window_axis<R>(A.g.win, m, a2.frac, (R)n2, (R)A.g.b[2], w.pz, WG ? w.dz : nullptr);
+  if constexpr (MT > 0) {
+    window_axis_s<R, CM, WG>(A.g.win, a0.frac, (R)n0, (R)A.g.b[0], w.px, w.dx);
+    window_axis_s<R, CM, WG>(A.g.win, a1.frac, (R)n1, (R)A.g.b[1], w.py, w.dy);
+    window_axis_s<R, CM, WG>(A.g.win, a2.frac, (R)n2, (R)A.g.b[2], w.pz, w.dz);
+  } else {
+    window_axis<R>(A.g.win, m, a0.frac, (R)n0, (R)A.g.b[0], w.px, WG ? w.dx : nullptr);
+    window_axis<R>(A.g.win, m, a1.frac, (R)n1, (R)A.g.b[1], w.py, WG ? w.dy : nullptr);
+    window_axis<R>(A.g.win, m, a2.frac, (R)n2, (R)A.g.b[2], w.pz, WG ? w.dz : nullptr);
+  }
+}
+
+// Drop the x-window entry of plane a: after the shift, px[0] / dx[0] belong to
+// plane a + 1.  Lets the plane loop stay rolled while every index is static.
+template <class R, int CM, bool WG>
+__device__ inline void shift_x(NodeWin<R, CM>& w) {
+#pragma unroll
+  for (int k = 0; k + 1 < CM; k++) {
+    w.px[k] = w.px[k + 1];
+    if (WG) w.dx[k] = w.dx[k + 1];
+  }
 }
 
 // Separable gather sum f = sum_a px sum_b py sum_c pz G (and the three
-// gradient sums with one factor replaced by its derivative).
+// gradient sums with one factor replaced by its derivative), static width CM.
 // ROW(a, b) returns a pointer p with p[IDX(c)] the grid value at stencil (a,b,c).
 template <class R, int CM, int MODE, class RowFn, class IdxFn>
-__device__ inline void gather_sum(const NodeWin<R, CM>& w, int c, RowFn row_of, IdxFn idx, Cplx<R>* out) {
+__device__ inline void gather_sum(NodeWin<R, CM> w, RowFn row_of, IdxFn idx, Cplx<R>* out) {
+  constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
+  Cplx<R> f = cmk<R>(0, 0), g0 = cmk<R>(0, 0), g1 = cmk<R>(0, 0), g2 = cmk<R>(0, 0);
+#pragma unroll 1
+  for (int a = 0; a < CM; a++) {
+    Cplx<R> ha = cmk<R>(0, 0), hay = cmk<R>(0, 0), haz = cmk<R>(0, 0);
+#pragma unroll
+    for (int b = 0; b < CM; b++) {
+      const Cplx<R>* row = row_of(a, b);
+      Cplx<R> hb = cmk<R>(0, 0), hbz = cmk<R>(0, 0);
+#pragma unroll
+      for (int s = 0; s < CM; s++) {
+        const Cplx<R> gv = row[idx(s)];
+        hb.x += w.pz[s] * gv.x;
+        hb.y += w.pz[s] * gv.y;
+        if (WG) {
+          hbz.x += w.dz[s] * gv.x;
+          hbz.y += w.dz[s] * gv.y;
+        }
+      }
+      ha.x += w.py[b] * hb.x;
+      ha.y += w.py[b] * hb.y;
+      if (WG) {
+        hay.x += w.dy[b] * hb.x;
+        hay.y += w.dy[b] * hb.y;
+        haz.x += w.py[b] * hbz.x;
+        haz.y += w.py[b] * hbz.y;
+      }
+    }
+    f.x += w.px[0] * ha.x;
+    f.y += w.px[0] * ha.y;
+    if (WG) {
+      g0.x += w.dx[0] * ha.x;
+      g0.y += w.dx[0] * ha.y;
+      g1.x += w.px[0] * hay.x;
+      g1.y += w.px[0] * hay.y;
+      g2.x += w.px[0] * haz.x;
+      g2.y += w.px[0] * haz.y;
+    }
+    shift_x<R, CM, WG>(w);
+  }
+  out[0] = f;
+  out[1] = g0;
+  out[2] = g1;
+  out[3] = g2;
+}
+
+// The same sum for a runtime width c <= CM (generic m); same operation order.
+template <class R, int CM, int MODE, class RowFn, class IdxFn>
+__device__ inline void gather_sum_rt(const NodeWin<R, CM>& w, int c, RowFn row_of, IdxFn idx, Cplx<R>* out) {
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
   Cplx<R> f = cmk<R>(0, 0), g0 = cmk<R>(0, 0), g1 = cmk<R>(0, 0), g2 = cmk<R>(0, 0);
   for (int a = 0; a < c; a++) {
@@ -63,16 +140,13 @@ __device__ inline void gather_sum(const NodeWin<R, CM>& w, int c, RowFn row_of, 
     for (int b = 0; b < c; b++) {
       const Cplx<R>* row = row_of(a, b);
       Cplx<R> hb = cmk<R>(0, 0), hbz = cmk<R>(0, 0);
-#pragma unroll
-      for (int s = 0; s < CM; s++) {
-        if (s < c) {
-          const Cplx<R> gv = row[idx(s)];
-          hb.x += w.pz[s] * gv.x;
-          hb.y += w.pz[s] * gv.y;
-          if (WG) {
-            hbz.x += w.dz[s] * gv.x;
-            hbz.y += w.dz[s] * gv.y;
-          }
+      for (int s = 0; s < c; s++) {
+        const Cplx<R> gv = row[idx(s)];
+        hb.x += w.pz[s] * gv.x;
+        hb.y += w.pz[s] * gv.y;
+        if (WG) {
+          hbz.x += w.dz[s] * gv.x;
+          hbz.y += w.dz[s] * gv.y;
         }
       }
       ha.x += w.py[b] * hb.x;
@@ -150,8 +224,9 @@ __device__ inline void gather_store(const BArgs<R>& A, long j, Cplx<R>* v) {
   }
 }
 
-// gather one node straight from the grid (periodic wrap on every axis)
-template <class R, int CM, int MODE>
+// gather one node straight from the grid (periodic wrap on every axis).
+// ST: the stencil width is the compile-time CM (else the runtime c <= CM).
+template <class R, int CM, int MODE, bool ST>
 __device__ inline void gather_node_global(const BArgs<R>& A, long j, const NodeWin<R, CM>& w, int c) {
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
   int i2s[CM];
@@ -170,12 +245,34 @@ __device__ inline void gather_node_global(const BArgs<R>& A, long j, const NodeW
   };
   auto idx = [&](int s) { return i2s[s]; };
   Cplx<R> v[4];
-  gather_sum<R, CM, MODE>(w, c, row_of, idx, v);
+  if constexpr (ST)
+    gather_sum<R, CM, MODE>(w, row_of, idx, v);
+  else
+    gather_sum_rt<R, CM, MODE>(w, c, row_of, idx, v);
   gather_store<R, MODE>(A, j, v);
 }
 
-template <class R, int CM, int MODE>
-__device__ inline void scatter_node_global(const BArgs<R>& A, long j, const NodeWin<R, CM>& w, int c) {
+// value added at one stencil point: f psi(a) psi(b) psi(c) and/or
+// grad . (psi'(a) psi(b) psi(c), psi(a) psi'(b) psi(c), psi(a) psi(b) psi'(c))
+template <class R, bool WF, bool WG>
+__device__ inline Cplx<R> scatter_value(R pxy, R pdxy, R pxdy, R pz, R dz, Cplx<R> fv, Cplx<R> gv0, Cplx<R> gv1,
+                                        Cplx<R> gv2) {
+  R vr = 0, vi = 0;
+  if (WF) {
+    const R q = pxy * pz;
+    vr += q * fv.x;
+    vi += q * fv.y;
+  }
+  if (WG) {
+    const R q0 = pdxy * pz, q1 = pxdy * pz, q2 = pxy * dz;
+    vr += q0 * gv0.x + q1 * gv1.x + q2 * gv2.x;
+    vi += q0 * gv0.y + q1 * gv1.y + q2 * gv2.y;
+  }
+  return cmk<R>(vr, vi);
+}
+
+template <class R, int CM, int MODE, bool ST>
+__device__ inline void scatter_node_global(const BArgs<R>& A, long j, NodeWin<R, CM> w, int c) {
   constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
@@ -186,35 +283,52 @@ __device__ inline void scatter_node_global(const BArgs<R>& A, long j, const Node
     gv1 = load_g(A, j, 1);
     gv2 = load_g(A, j, 2);
   }
-  for (int a = 0; a < c; a++) {
-    int i0 = w.start[0] + a;
-    if (i0 >= n0) i0 -= n0;
-    for (int b = 0; b < c; b++) {
-      int i1 = w.start[1] + b;
-      if (i1 >= n1) i1 -= n1;
-      Cplx<R>* row = A.grid + ((long)i0 * n1 + i1) * n2;
-      const R pxy = w.px[a] * w.py[b];
-      R pdxy = 0, pxdy = 0;
-      if (WG) {
-        pdxy = w.dx[a] * w.py[b];
-        pxdy = w.px[a] * w.dy[b];
+  if constexpr (ST) {
+    int i2s[CM];
+#pragma unroll
+    for (int s = 0; s < CM; s++) {
+      const int v = w.start[2] + s;
+      i2s[s] = v >= n2 ? v - n2 : v;
+    }
+#pragma unroll 1
+    for (int a = 0; a < CM; a++) {
+      int i0 = w.start[0] + a;
+      if (i0 >= n0) i0 -= n0;
+#pragma unroll
+      for (int b = 0; b < CM; b++) {
+        int i1 = w.start[1] + b;
+        if (i1 >= n1) i1 -= n1;
+        Cplx<R>* row = A.grid + ((long)i0 * n1 + i1) * n2;
+        const R pxy = w.px[0] * w.py[b];
+        const R pdxy = WG ? w.dx[0] * w.py[b] : R(0);
+        const R pxdy = WG ? w.px[0] * w.dy[b] : R(0);
+#pragma unroll
+        for (int s = 0; s < CM; s++) {
+          const Cplx<R> v = scatter_value<R, WF, WG>(pxy, pdxy, pxdy, w.pz[s], WG ? w.dz[s] : R(0), fv, gv0, gv1, gv2);
+          atomicAdd(&row[i2s[s]].x, v.x);
+          atomicAdd(&row[i2s[s]].y, v.y);
+        }
       }
-      for (int s = 0; s < c; s++) {
-        int i2 = w.start[2] + s;
-        if (i2 >= n2) i2 -= n2;
-        R vr = 0, vi = 0;
-        if (WF) {
-          const R q = pxy * w.pz[s];
-          vr += q * fv.x;
-          vi += q * fv.y;
+      shift_x<R, CM, WG>(w);
+    }
+  } else {
+    for (int a = 0; a < c; a++) {
+      int i0 = w.start[0] + a;
+      if (i0 >= n0) i0 -= n0;
+      for (int b = 0; b < c; b++) {
+        int i1 = w.start[1] + b;
+        if (i1 >= n1) i1 -= n1;
+        Cplx<R>* row = A.grid + ((long)i0 * n1 + i1) * n2;
+        const R pxy = w.px[a] * w.py[b];
+        const R pdxy = WG ? w.dx[a] * w.py[b] : R(0);
+        const R pxdy = WG ? w.px[a] * w.dy[b] : R(0);
+        for (int s = 0; s < c; s++) {
+          int i2 = w.start[2] + s;
+          if (i2 >= n2) i2 -= n2;
+          const Cplx<R> v = scatter_value<R, WF, WG>(pxy, pdxy, pxdy, w.pz[s], WG ? w.dz[s] : R(0), fv, gv0, gv1, gv2);
+          atomicAdd(&row[i2].x, v.x);
+          atomicAdd(&row[i2].y, v.y);
         }
-        if (WG) {
-          const R q0 = pdxy * w.pz[s], q1 = pxdy * w.pz[s], q2 = pxy * w.dz[s];
-          vr += q0 * gv0.x + q1 * gv1.x + q2 * gv2.x;
-          vi += q0 * gv0.y + q1 * gv1.y + q2 * gv2.y;
-        }
-        atomicAdd(&row[i2].x, vr);
-        atomicAdd(&row[i2].y, vi);
       }
     }
   }
@@ -241,8 +355,9 @@ __global__ __launch_bounds__(kB) void gather_direct_k(BArgs<R> A) {
   constexpr int CM = MT > 0 ? 2 * MT + 1 : kMaxC;
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
   NodeWin<R, CM> w;
-  node_windows<R, CM, WG>(A, j, m, w);
-  gather_node_global<R, CM, MODE>(A, j, w, 2 * m + 1);
+  constexpr bool ST = static_body<R, MT>();
+  node_windows<R, ST ? MT : 0, CM, WG>(A, j, m, w);
+  gather_node_global<R, CM, MODE, ST>(A, j, w, 2 * m + 1);
 }
 
 template <class R, int MT, int MODE>
@@ -255,8 +370,9 @@ __global__ __launch_bounds__(kB) void scatter_direct_k(BArgs<R> A) {
   constexpr int CM = MT > 0 ? 2 * MT + 1 : kMaxC;
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
   NodeWin<R, CM> w;
-  node_windows<R, CM, WG>(A, j, m, w);
-  scatter_node_global<R, CM, MODE>(A, j, w, 2 * m + 1);
+  constexpr bool ST = static_body<R, MT>();
+  node_windows<R, ST ? MT : 0, CM, WG>(A, j, m, w);
+  scatter_node_global<R, CM, MODE, ST>(A, j, w, 2 * m + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -317,6 +433,7 @@ __global__ __launch_bounds__(kTileThreads) void gather_tiled_k(BArgs<R> A) {
   constexpr int m = MT;
   constexpr int CM = 2 * MT + 1;
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
+  constexpr bool ST = static_body<R, MT>();
   const TileItem it = tile_item(A, m);
   const SubgridPitch sp = subgrid_pitch(it);
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
@@ -331,19 +448,22 @@ __global__ __launch_bounds__(kTileThreads) void gather_tiled_k(BArgs<R> A) {
   for (int p = it.beg + threadIdx.x; p < it.end; p += kTileThreads) {
     const long j = A.perm[p];
     NodeWin<R, CM> w;
-    node_windows<R, CM, WG>(A, j, m, w);
+    node_windows<R, ST ? MT : 0, CM, WG>(A, j, m, w);
     const int l0 = local_off(w.start[0], it.o[0], A.ts[0], n0);
     const int l1 = local_off(w.start[1], it.o[1], A.ts[1], n1);
     const int l2 = local_off(w.start[2], it.o[2], A.ts[2], n2);
     if (l0 < 0 || l1 < 0 || l2 < 0) {
-      gather_node_global<R, CM, MODE>(A, j, w, CM);
+      gather_node_global<R, CM, MODE, ST>(A, j, w, CM);
       continue;
     }
     const Cplx<R>* base = tile + l0 * sp.PP + l1 * sp.RP + l2;
     auto row_of = [&](int a, int b) { return base + a * sp.PP + b * sp.RP; };
     auto idx = [](int s) { return s; };
     Cplx<R> v[4];
-    gather_sum<R, CM, MODE>(w, CM, row_of, idx, v);
+    if constexpr (ST)
+      gather_sum<R, CM, MODE>(w, row_of, idx, v);
+    else
+      gather_sum_rt<R, CM, MODE>(w, CM, row_of, idx, v);
     gather_store<R, MODE>(A, j, v);
   }
 }
@@ -356,6 +476,7 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
   constexpr int CM = 2 * MT + 1;
   constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
+  constexpr bool ST = static_body<R, MT>();
   const TileItem it = tile_item(A, m);
   const SubgridPitch sp = subgrid_pitch(it);
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
@@ -372,12 +493,12 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
     const int p = it.beg + q;
     const long j = A.perm[p];
     NodeWin<R, CM> w;
-    node_windows<R, CM, WG>(A, j, m, w);
+    node_windows<R, ST ? MT : 0, CM, WG>(A, j, m, w);
     const int l0 = local_off(w.start[0], it.o[0], A.ts[0], n0);
     const int l1 = local_off(w.start[1], it.o[1], A.ts[1], n1);
     const int l2 = local_off(w.start[2], it.o[2], A.ts[2], n2);
     if (l0 < 0 || l1 < 0 || l2 < 0) {
-      scatter_node_global<R, CM, MODE>(A, j, w, CM);
+      scatter_node_global<R, CM, MODE, ST>(A, j, w, CM);
       continue;
     }
     Cplx<R> fv = cmk<R>(0, 0), gv0 = cmk<R>(0, 0), gv1 = cmk<R>(0, 0), gv2 = cmk<R>(0, 0);
@@ -387,31 +508,40 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
       gv1 = load_g(A, j, 1);
       gv2 = load_g(A, j, 2);
     }
-    double* base = acc + 2 * (l0 * sp.PP + l1 * sp.RP + l2);
-    for (int a = 0; a < CM; a++) {
-      for (int b = 0; b < CM; b++) {
-        double* row = base + 2 * (a * sp.PP + b * sp.RP);
-        const R pxy = w.px[a] * w.py[b];
-        R pdxy = 0, pxdy = 0;
-        if (WG) {
-          pdxy = w.dx[a] * w.py[b];
-          pxdy = w.px[a] * w.dy[b];
-        }
+    double* plane = acc + 2 * (l0 * sp.PP + l1 * sp.RP + l2);
+    if constexpr (ST) {
+#pragma unroll 1
+      for (int a = 0; a < CM; a++, plane += 2 * sp.PP) {
 #pragma unroll
-        for (int s = 0; s < CM; s++) {
-          R vr = 0, vi = 0;
-          if (WF) {
-            const R q = pxy * w.pz[s];
-            vr += q * fv.x;
-            vi += q * fv.y;
+        for (int b = 0; b < CM; b++) {
+          double* row = plane + 2 * b * sp.RP;
+          const R pxy = w.px[0] * w.py[b];
+          const R pdxy = WG ? w.dx[0] * w.py[b] : R(0);
+          const R pxdy = WG ? w.px[0] * w.dy[b] : R(0);
+#pragma unroll
+          for (int s = 0; s < CM; s++) {
+            const Cplx<R> v =
+                scatter_value<R, WF, WG>(pxy, pdxy, pxdy, w.pz[s], WG ? w.dz[s] : R(0), fv, gv0, gv1, gv2);
+            atomicAdd(&row[2 * s], (double)v.x);
+            atomicAdd(&row[2 * s + 1], (double)v.y);
           }
-          if (WG) {
-            const R q0 = pdxy * w.pz[s], q1 = pxdy * w.pz[s], q2 = pxy * w.dz[s];
-            vr += q0 * gv0.x + q1 * gv1.x + q2 * gv2.x;
-            vi += q0 * gv0.y + q1 * gv1.y + q2 * gv2.y;
+        }
+        shift_x<R, CM, WG>(w);
+      }
+    } else {
+      for (int a = 0; a < CM; a++) {
+        for (int b = 0; b < CM; b++) {
+          double* row = plane + 2 * (a * sp.PP + b * sp.RP);
+          const R pxy = w.px[a] * w.py[b];
+          const R pdxy = WG ? w.dx[a] * w.py[b] : R(0);
+          const R pxdy = WG ? w.px[a] * w.dy[b] : R(0);
+#pragma unroll
+          for (int s = 0; s < CM; s++) {
+            const Cplx<R> v =
+                scatter_value<R, WF, WG>(pxy, pdxy, pxdy, w.pz[s], WG ? w.dz[s] : R(0), fv, gv0, gv1, gv2);
+            atomicAdd(&row[2 * s], (double)v.x);
+            atomicAdd(&row[2 * s + 1], (double)v.y);
           }
-          atomicAdd(&row[2 * s], (double)vr);
-          atomicAdd(&row[2 * s + 1], (double)vi);
         }
       }
     }
@@ -457,6 +587,7 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                  \
         hipLaunchKernelGGL((gather_tiled_k<R, MV, MODE>), tgrid, tblock, lds, s, A);                      \
       }                                                                                                   \
+      if (A.mid) (void)hipEventRecord(A.mid, s);                                                          \
       hipLaunchKernelGGL((gather_direct_k<R, MV, MODE>), grid, block, 0, s, D);                           \
     } else {                                                                                              \
       if (tiled) {                                                                                        \
@@ -464,6 +595,7 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                  \
         hipLaunchKernelGGL((scatter_tiled_k<R, MV, SMODE>), tgrid, tblock, lds, s, A);                    \
       }                                                                                                   \
+      if (A.mid) (void)hipEventRecord(A.mid, s);                                                          \
       hipLaunchKernelGGL((scatter_direct_k<R, MV, SMODE>), grid, block, 0, s, D);                         \
     }                                                                                                     \
     break;
@@ -477,6 +609,7 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
 #undef PNFFT_B_CASE
     default: {
       D.tile_dense = nullptr;
+      if (A.mid) (void)hipEventRecord(A.mid, s);
       if constexpr (KIND == 0) hipLaunchKernelGGL((gather_direct_k<R, 0, MODE>), grid, block, 0, s, D);
       else hipLaunchKernelGGL((scatter_direct_k<R, 0, SMODE>), grid, block, 0, s, D);
       break;

@@ -5,9 +5,12 @@
 // evaluated for every l mod n (natural DFT order on the device grid).
 // Backward (adj): ghat[k] = sum_l g[l] e^{+2 pi i k.l / n} for k in [-N/2,N/2)^3.
 //
-// One HIP kernel per axis pass.  A block stages C pencils of length L in LDS
-// (layout [position][pencil], pitch C + 1), runs a Stockham radix-8/4/2 FFT in
-// place with one barrier pair per stage, and writes the pencils back.  The
+// One HIP kernel per axis pass, instantiated per pencil length L so that every
+// index is a compile-time shift or mask.  A block holds C pencils of length L
+// (layout [position][pencil], pitch C + 1) and runs a Stockham radix-8/4/2 FFT
+// with one barrier pair per LDS stage.  The column passes compute their first
+// stage on the loaded registers and their last stage straight into the stores,
+// so a 512-point pencil makes one LDS round trip instead of four.  The
 // zero padding N -> n and the k -> k mod n shift are folded into the loads
 // (forward) or the stores (backward), so the padded input never exists in HBM.
 // Matrix D is fused into the first forward pass and D^T into the last backward
@@ -22,6 +25,7 @@
 #include "fft.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace pnfft_amd {
 
@@ -42,8 +46,6 @@ struct PassArgs {
   int n_inner;               // pencils along the contiguous batch index
   int L, Lin, Lout;
   int C;                     // pencils per block
-  int nst;
-  int radix[16];
   const Cplx<R>* tw;         // tw[i] = e^{-2 pi i i / L}
   int backward;
   // fused deconvolution: factor tab0[k0]*tab1[k1]*tab2[k2]; row r = i0*N1 + i1
@@ -99,55 +101,83 @@ __device__ inline void dft8(Cplx<R>* v) {
   v[7] = csub(e3, o3);
 }
 
-// One Stockham stage of radix RR over the C pencils held in LDS.  Every thread
-// owns exactly EPT/RR butterflies (C*L == EPT*blockDim by construction).
-template <class R, int RR, int EPT>
-__device__ inline void stage(Cplx<R>* sh, int LD, int C, int L, int Ns, const Cplx<R>* __restrict__ tw) {
-  constexpr int NB = EPT / RR;
-  const int nbf = L / RR;
+// Block shape of a pass over pencils of length L: C pencils per block with
+// C * L = EPT * kThreads, LDS layout [position][pencil] with pitch LD.
+template <class R, int L>
+struct PassShape {
+  static constexpr int EPT = FftEpt<R>::value;
+  static constexpr int C = EPT * kThreads / L;
+  static constexpr int LD = C == 1 ? 1 : C + 1;
+};
+
+// radix of the stage that starts at span Ns (8 while possible, then 4 or 2)
+template <int L, int Ns>
+struct StageRadix {
+  static constexpr int value = (L / Ns) >= 8 ? 8 : (L / Ns);
+};
+
+template <class R, int RR>
+__device__ inline void dft(Cplx<R>* w) {
+  if constexpr (RR == 2) dft2(w[0], w[1]);
+  if constexpr (RR == 4) dft4(w[0], w[1], w[2], w[3]);
+  if constexpr (RR == 8) dft8(w);
+}
+
+// One Stockham stage LDS -> LDS of radix RR at span Ns.  Butterfly bf of the
+// block covers pencil c = bf % C, index j = bf / C; it reads positions
+// j + q L/RR and writes (j - k) RR + k + q Ns with k = j mod Ns.
+template <class R, int L, int Ns>
+__device__ inline void stage_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
+  using S = PassShape<R, L>;
+  constexpr int RR = StageRadix<L, Ns>::value;
+  constexpr int NB = S::EPT / RR;
+  constexpr int nbf = L / RR;
+  constexpr int twstep = L / (Ns * RR);
   const int tid = threadIdx.x;
-  Cplx<R> v[EPT];
+  Cplx<R> v[S::EPT];
 #pragma unroll
   for (int i = 0; i < NB; i++) {
     const int bf = tid + i * kThreads;
-    const int c = bf % C, j = bf / C;
+    const int c = bf % S::C, j = bf / S::C;
 #pragma unroll
-    for (int q = 0; q < RR; q++) v[i * RR + q] = sh[(j + q * nbf) * LD + c];
+    for (int q = 0; q < RR; q++) v[i * RR + q] = sh[(j + q * nbf) * S::LD + c];
   }
   __syncthreads();
-  const int twstep = L / (Ns * RR);
 #pragma unroll
   for (int i = 0; i < NB; i++) {
     const int bf = tid + i * kThreads;
-    const int c = bf % C, j = bf / C;
+    const int c = bf % S::C, j = bf / S::C;
     const int k = j % Ns;
     Cplx<R>* w = v + i * RR;
-    if (Ns > 1) {
+    if constexpr (Ns > 1) {
 #pragma unroll
       for (int q = 1; q < RR; q++) w[q] = cmul(w[q], tw[k * q * twstep]);
     }
-    if (RR == 2) dft2(w[0], w[1]);
-    if (RR == 4) dft4(w[0], w[1], w[2], w[3]);
-    if (RR == 8) dft8(w);
+    dft<R, RR>(w);
     const int base = (j - k) * RR + k;
 #pragma unroll
-    for (int q = 0; q < RR; q++) sh[(base + q * Ns) * LD + c] = w[q];
+    for (int q = 0; q < RR; q++) sh[(base + q * Ns) * S::LD + c] = w[q];
   }
   __syncthreads();
 }
 
-template <class R>
-__device__ inline void run_stages(Cplx<R>* sh, int LD, const PassArgs<R>& P) {
-  constexpr int EPT = FftEpt<R>::value;
-  int Ns = 1;
-  for (int st = 0; st < P.nst; st++) {
-    const int rr = P.radix[st];
-    if (rr == 8) stage<R, 8, EPT>(sh, LD, P.C, P.L, Ns, P.tw);
-    else if (rr == 4) stage<R, 4, EPT>(sh, LD, P.C, P.L, Ns, P.tw);
-    else stage<R, 2, EPT>(sh, LD, P.C, P.L, Ns, P.tw);
-    Ns *= rr;
+// stages LDS -> LDS from span Ns up to, but not including, span END
+template <class R, int L, int Ns, int END>
+__device__ inline void stages_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
+  if constexpr (Ns < END) {
+    stage_lds<R, L, Ns>(sh, tw);
+    stages_lds<R, L, Ns * StageRadix<L, Ns>::value, END>(sh, tw);
   }
 }
+
+// span of the last stage: Ns with Ns * radix(Ns) == L
+template <int L, int Ns = 1>
+constexpr int last_span() {
+  if constexpr (Ns * StageRadix<L, Ns>::value >= L) return Ns;
+  else return last_span<L, Ns * StageRadix<L, Ns>::value>();
+}
+
+// ---- per-element transforms of the loads and stores ----------------------
 
 // position p of a length-L pencil holds frequency k = p (p small) or p - L;
 // input index e = k + h when k lies in [-h, pos).
@@ -157,100 +187,180 @@ __device__ inline int pruned_src(int p, int L, int h, int pos) {
   return -1;
 }
 
-// ROWS: pencils are contiguous rows (axis 2); else pencils are strided columns
-// whose batch index is contiguous (axes 0 and 1).
-template <class R, bool ROWS>
-__global__ __launch_bounds__(kThreads) void fft_pass(PassArgs<R> P) {
+// forward load of position p: the pruned input index e (or -1 for a zero)
+template <class R>
+__device__ inline int load_index(const PassArgs<R>& P, int p, int L) {
+  return P.backward ? p : pruned_src(p, L, P.h, P.pos);
+}
+
+// value as it enters the transform: D factor and c2r weight (forward rows),
+// conjugation (backward)
+template <class R>
+__device__ inline Cplx<R> on_load(const PassArgs<R>& P, Cplx<R> w, int col, int e) {
+  const bool fwd = !P.backward;
+  if (fwd && P.tab0 != nullptr) {
+    // row = i0*N1 + i1 (rows pass over f_hat), e = i2
+    const int i0 = col / P.N1, i1 = col % P.N1;
+    const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
+    w = cmk<R>(R((double)w.x * f), R((double)w.y * f));
+  }
+  if (P.half && fwd && e < P.h) w = cmk<R>(R(2) * w.x, R(2) * w.y);
+  if (!fwd) w = cconj(w);
+  return w;
+}
+
+// value as it leaves: conjugation and D^T factor (backward), accumulation
+template <class R>
+__device__ inline void on_store(const PassArgs<R>& P, Cplx<R> w, int col, int e, long addr) {
+  if (P.backward) {
+    w = cconj(w);
+    if (P.tab0 != nullptr) {
+      const int i0 = col / P.N1, i1 = col % P.N1;
+      const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
+      w = cmk<R>(R((double)w.x * f), R((double)w.y * f));
+    }
+  }
+  if (P.accumulate) w = cadd(P.out[addr], w);
+  P.out[addr] = w;
+}
+
+// output position p of a pencil -> stored index e, or -1 (pruned away)
+template <class R>
+__device__ inline int store_index(const PassArgs<R>& P, int p, int L) {
+  if (!P.backward) return p;
+  int e = p + P.h;
+  if (e >= L) e -= L;
+  return e < P.Lout ? e : -1;
+}
+
+// Rows pass (axis 2, contiguous pencils): loads and stores walk each row with
+// consecutive lanes; all stages run in LDS.
+template <class R, int L>
+__global__ __launch_bounds__(kThreads) void fft_rows(PassArgs<R> P) {
+  using S = PassShape<R, L>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<R>* sh = reinterpret_cast<Cplx<R>*>(smem_raw);
-  const int C = P.C, L = P.L, LD = C + 1;
   const int tid = threadIdx.x;
-  const long outer = blockIdx.y;
-  const int c0 = blockIdx.x * C;
-  const bool fwd = !P.backward;
-  const bool deconv = (P.tab0 != nullptr);
-
-  // Every thread owns exactly EPT = L*C/blockDim positions of the block's
-  // pencils; all of its global loads are issued before the LDS writes (and all
-  // LDS reads before the global stores) to keep many requests in flight.
-  constexpr int EPT = FftEpt<R>::value;
-  Cplx<R> v[EPT];
-
-  // ---- load
+  const int c0 = blockIdx.x * S::C;
+  Cplx<R> v[S::EPT];
 #pragma unroll
-  for (int i = 0; i < EPT; i++) {
+  for (int i = 0; i < S::EPT; i++) {
     const int idx = tid + i * kThreads;
-    int p, c;
-    if (ROWS) { c = idx / L; p = idx % L; }
-    else { p = idx / C; c = idx % C; }
+    const int c = idx / L, p = idx % L;
     const int col = c0 + c;
+    const int e = load_index(P, p, L);
     v[i] = cmk<R>(R(0), R(0));
-    const int e = fwd ? pruned_src(p, L, P.h, P.pos) : p;
-    if (col < P.n_inner && e >= 0) {
-      long addr;
-      if (ROWS) addr = (long)col * P.in_outer + (long)e * P.in_elem;
-      else addr = outer * P.in_outer + col + (long)e * P.in_elem;
-      v[i] = P.in[addr];
-    }
+    if (col < P.n_inner && e >= 0) v[i] = P.in[(long)col * P.in_outer + e];
   }
 #pragma unroll
-  for (int i = 0; i < EPT; i++) {
+  for (int i = 0; i < S::EPT; i++) {
     const int idx = tid + i * kThreads;
-    int p, c;
-    if (ROWS) { c = idx / L; p = idx % L; }
-    else { p = idx / C; c = idx % C; }
-    Cplx<R> w = v[i];
+    const int c = idx / L, p = idx % L;
     const int col = c0 + c;
-    const int e = fwd ? pruned_src(p, L, P.h, P.pos) : p;
-    if (deconv && fwd && col < P.n_inner && e >= 0) {
-      // row = i0*N1 + i1 (ROWS pass over f_hat), e = i2
-      const int i0 = col / P.N1, i1 = col % P.N1;
-      const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
-      w = cmk<R>(R((double)w.x * f), R((double)w.y * f));
-    }
-    if (P.half && fwd && e >= 0 && e < P.h) w = cmk<R>(R(2) * w.x, R(2) * w.y);
-    if (!fwd) w = cconj(w);
-    sh[p * LD + c] = w;
+    const int e = load_index(P, p, L);
+    Cplx<R> w = v[i];
+    if (col < P.n_inner && e >= 0) w = on_load(P, w, col, e);
+    sh[p * S::LD + c] = w;
   }
   __syncthreads();
-
-  run_stages<R>(sh, LD, P);
-
-  // ---- store
-  const int nstore = P.Lout * C;
+  stages_lds<R, L, 1, L>(sh, P.tw);
 #pragma unroll
-  for (int i = 0; i < EPT; i++) {
+  for (int i = 0; i < S::EPT; i++) {
     const int idx = tid + i * kThreads;
-    if (idx < nstore) {
-      int e, c;
-      if (ROWS) { c = idx / P.Lout; e = idx % P.Lout; }
-      else { e = idx / C; c = idx % C; }
-      int p = e;
-      if (!fwd) p = (e - P.h + L) % L;
-      v[i] = sh[p * LD + c];
-    }
+    const int c = idx / L, p = idx % L;
+    v[i] = sh[p * S::LD + c];
   }
 #pragma unroll
-  for (int i = 0; i < EPT; i++) {
+  for (int i = 0; i < S::EPT; i++) {
     const int idx = tid + i * kThreads;
-    if (idx >= nstore) continue;
-    int e, c;
-    if (ROWS) { c = idx / P.Lout; e = idx % P.Lout; }
-    else { e = idx / C; c = idx % C; }
+    const int c = idx / L, p = idx % L;
+    const int col = c0 + c;
+    const int e = store_index(P, p, L);
+    if (col < P.n_inner && e >= 0) on_store(P, v[i], col, e, (long)col * P.out_outer + e);
+  }
+}
+
+// Columns pass (axes 0 and 1): pencil c of the block is column c0 + c of
+// plane blockIdx.y, consecutive lanes take consecutive columns.  The first
+// stage runs on the loaded registers and the last stage stores straight to
+// HBM, so only the middle stages touch LDS.
+template <class R, int L>
+__global__ __launch_bounds__(kThreads) void fft_cols(PassArgs<R> P) {
+  using S = PassShape<R, L>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<R>* sh = reinterpret_cast<Cplx<R>*>(smem_raw);
+  constexpr int R1 = StageRadix<L, 1>::value;
+  constexpr int NB1 = S::EPT / R1, nbf1 = L / R1;
+  constexpr int NsL = last_span<L>();
+  constexpr int RL = StageRadix<L, NsL>::value;
+  constexpr int NBL = S::EPT / RL, nbfL = L / RL;
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * S::C;
+  const long in_base = (long)blockIdx.y * P.in_outer;
+  const long out_base = (long)blockIdx.y * P.out_outer;
+  Cplx<R> v[S::EPT];
+  // ---- first stage (span 1): butterfly (c, j) takes positions j + q L/R1
+#pragma unroll
+  for (int i = 0; i < NB1; i++) {
+    const int bf = tid + i * kThreads;
+    const int c = bf % S::C, j = bf / S::C;
+    const int col = c0 + c;
+#pragma unroll
+    for (int q = 0; q < R1; q++) {
+      const int e = load_index(P, j + q * nbf1, L);
+      v[i * R1 + q] = cmk<R>(R(0), R(0));
+      if (col < P.n_inner && e >= 0) v[i * R1 + q] = P.in[in_base + col + (long)e * P.in_elem];
+    }
+  }
+  if (P.backward) {
+#pragma unroll
+    for (int i = 0; i < S::EPT; i++) v[i] = cconj(v[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < NB1; i++) dft<R, R1>(v + i * R1);
+  if constexpr (NsL > 1) {
+#pragma unroll
+    for (int i = 0; i < NB1; i++) {
+      const int bf = tid + i * kThreads;
+      const int c = bf % S::C, j = bf / S::C;
+#pragma unroll
+      for (int q = 0; q < R1; q++) sh[(j * R1 + q) * S::LD + c] = v[i * R1 + q];
+    }
+    __syncthreads();
+    stages_lds<R, L, R1, NsL>(sh, P.tw);
+    // ---- last stage (span NsL): outputs at positions j + q NsL
+#pragma unroll
+    for (int i = 0; i < NBL; i++) {
+      const int bf = tid + i * kThreads;
+      const int c = bf % S::C, j = bf / S::C;
+#pragma unroll
+      for (int q = 0; q < RL; q++) v[i * RL + q] = sh[(j + q * nbfL) * S::LD + c];
+    }
+    constexpr int twstep = L / (NsL * RL);
+#pragma unroll
+    for (int i = 0; i < NBL; i++) {
+      const int bf = tid + i * kThreads;
+      const int j = bf / S::C;
+#pragma unroll
+      for (int q = 1; q < RL; q++) v[i * RL + q] = cmul(v[i * RL + q], P.tw[j * q * twstep]);
+      dft<R, RL>(v + i * RL);
+    }
+  }
+  // positions of the outputs: j + q NsL (a single stage: NsL = 1, j + q nbf1)
+  constexpr int RO = NsL > 1 ? RL : R1;
+  constexpr int NBO = S::EPT / RO;
+  constexpr int stepO = NsL > 1 ? NsL : nbf1;
+#pragma unroll
+  for (int i = 0; i < NBO; i++) {
+    const int bf = tid + i * kThreads;
+    const int c = bf % S::C, j = bf / S::C;
     const int col = c0 + c;
     if (col >= P.n_inner) continue;
-    Cplx<R> w = v[i];
-    if (!fwd) w = cconj(w);
-    long addr;
-    if (ROWS) addr = (long)col * P.out_outer + (long)e * P.out_elem;
-    else addr = outer * P.out_outer + col + (long)e * P.out_elem;
-    if (deconv && !fwd) {
-      const int i0 = col / P.N1, i1 = col % P.N1;
-      const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
-      w = cmk<R>(R((double)w.x * f), R((double)w.y * f));
+#pragma unroll
+    for (int q = 0; q < RO; q++) {
+      const int e = store_index(P, j + q * stepO, L);
+      if (e >= 0) on_store(P, v[i * RO + q], col, e, out_base + col + (long)e * P.out_elem);
     }
-    if (P.accumulate) w = cadd(P.out[addr], w);
-    P.out[addr] = w;
   }
 }
 
@@ -269,14 +379,6 @@ template <class R>
 int FftAxis<R>::init(int L_) {
   L = L_;
   if (!fft_length_supported(L, sizeof(R) == 8)) return kUnsupported;
-  // radices: as many 8s as possible, then one 4 or 2
-  int e = 0;
-  while ((1 << e) < L) e++;
-  nst = 0;
-  int r3 = e % 3;
-  for (int i = 0; i < e / 3; i++) radix[nst++] = 8;
-  if (r3 == 2) radix[nst++] = 4;
-  if (r3 == 1) radix[nst++] = 2;
   C = FftEpt<R>::value * kThreads / L;
   std::vector<Cplx<R>> h(L);
   for (int i = 0; i < L; i++) {
@@ -300,20 +402,73 @@ static PassArgs<R> make_args(const FftAxis<R>& ax) {
   PassArgs<R> P{};
   P.L = ax.L;
   P.C = ax.C;
-  P.nst = ax.nst;
-  for (int i = 0; i < ax.nst; i++) P.radix[i] = ax.radix[i];
   P.tw = ax.tw;
   return P;
 }
 
+template <class R, int L>
+constexpr size_t pass_lds() {
+  return sizeof(Cplx<R>) * (size_t)L * (size_t)PassShape<R, L>::LD;
+}
+
+template <class R, int L>
+static int pass_attr() {
+  constexpr size_t lds = pass_lds<R, L>();
+  if (lds <= 64 * 1024) return kOk;
+  if (hipFuncSetAttribute((const void*)fft_rows<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+          hipSuccess ||
+      hipFuncSetAttribute((const void*)fft_cols<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+          hipSuccess)
+    return kHipError;
+  return kOk;
+}
+
+template <class R, int L>
+static void pass_launch(bool rows, dim3 grid, const PassArgs<R>& P, hipStream_t s) {
+  constexpr size_t lds = pass_lds<R, L>();
+  if (rows) hipLaunchKernelGGL((fft_rows<R, L>), grid, dim3(kThreads), lds, s, P);
+  else hipLaunchKernelGGL((fft_cols<R, L>), grid, dim3(kThreads), lds, s, P);
+}
+
+// Runs F(L) for the supported lengths; F is a generic lambda taking the
+// length as std::integral_constant.
+template <class R, class F>
+static int with_length(int L, F&& f) {
+  switch (L) {
+#define PNFFT_FFT_L(V) \
+  case V: return f(std::integral_constant<int, V>{});
+    PNFFT_FFT_L(2)
+    PNFFT_FFT_L(4)
+    PNFFT_FFT_L(8)
+    PNFFT_FFT_L(16)
+    PNFFT_FFT_L(32)
+    PNFFT_FFT_L(64)
+    PNFFT_FFT_L(128)
+    PNFFT_FFT_L(256)
+    PNFFT_FFT_L(512)
+    PNFFT_FFT_L(1024)
+    PNFFT_FFT_L(2048)
+    PNFFT_FFT_L(4096)
+#undef PNFFT_FFT_L
+    case 8192:
+      if constexpr (sizeof(R) == 4) return f(std::integral_constant<int, 8192>{});
+      return kUnsupported;
+    default:
+      return kUnsupported;
+  }
+}
+
 template <class R>
-static size_t lds_bytes(const FftAxis<R>& ax) {
-  return sizeof(Cplx<R>) * (size_t)ax.L * (size_t)(ax.C + 1);
+static void launch_pass(bool rows, dim3 grid, const PassArgs<R>& P, hipStream_t s) {
+  (void)with_length<R>(P.L, [&](auto Lc) {
+    pass_launch<R, decltype(Lc)::value>(rows, grid, P, s);
+    return kOk;
+  });
 }
 
 template <class R>
 int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
-                  Cplx<R>* grid, hipStream_t s, bool half) {
+                  Cplx<R>* grid, hipStream_t s, bool half, hipEvent_t mark1, hipEvent_t mark2) {
   const int N0 = F.N[0], N1 = F.N[1], N2 = F.N[2];
   const int n0 = F.n[0], n1 = F.n[1], n2 = F.n[2];
   const int N2in = half ? N2 / 2 + 1 : N2;  // stored length of f_hat rows
@@ -331,7 +486,8 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     if (tab) { P.tab0 = tab; P.tab1 = tab + N0; P.tab2 = tab + N0 + N1; }
     P.N1 = N1;
     dim3 grid_dim((P.n_inner + P.C - 1) / P.C, 1);
-    hipLaunchKernelGGL((fft_pass<R, true>), grid_dim, dim3(kThreads), lds_bytes(F.ax[2]), s, P);
+    launch_pass<R>(true, grid_dim, P, s);
+    if (mark1) (void)hipEventRecord(mark1, s);
   }
   // axis 1: pencils (i0, p2) stride n2 -> buf2[i0][p1][p2]
   {
@@ -343,7 +499,8 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.Lin = N1; P.Lout = n1;
     P.h = N1 / 2; P.pos = N1 - N1 / 2;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);
-    hipLaunchKernelGGL((fft_pass<R, false>), grid_dim, dim3(kThreads), lds_bytes(F.ax[1]), s, P);
+    launch_pass<R>(false, grid_dim, P, s);
+    if (mark2) (void)hipEventRecord(mark2, s);
   }
   // axis 0: pencils (p1, p2) stride n1*n2 -> grid[p0][p1][p2]
   {
@@ -355,14 +512,15 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.Lin = N0; P.Lout = n0;
     P.h = N0 / 2; P.pos = N0 - N0 / 2;
     dim3 grid_dim((n2 + P.C - 1) / P.C, n1);
-    hipLaunchKernelGGL((fft_pass<R, false>), grid_dim, dim3(kThreads), lds_bytes(F.ax[0]), s, P);
+    launch_pass<R>(false, grid_dim, P, s);
   }
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
 
 template <class R>
 int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
-                   Cplx<R>* fhat, int accumulate, hipStream_t s, bool half) {
+                   Cplx<R>* fhat, int accumulate, hipStream_t s, bool half, hipEvent_t mark1,
+                   hipEvent_t mark2) {
   const int N0 = F.N[0], N1 = F.N[1], N2 = F.N[2];
   const int n0 = F.n[0], n1 = F.n[1], n2 = F.n[2];
   const int N2out = half ? N2 / 2 + 1 : N2;
@@ -376,7 +534,8 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.Lin = n0; P.Lout = N0; P.h = N0 / 2;
     P.backward = 1;
     dim3 grid_dim((n2 + P.C - 1) / P.C, n1);
-    hipLaunchKernelGGL((fft_pass<R, false>), grid_dim, dim3(kThreads), lds_bytes(F.ax[0]), s, P);
+    launch_pass<R>(false, grid_dim, P, s);
+    if (mark1) (void)hipEventRecord(mark1, s);
   }
   // axis 1: buf2[i0][p1][p2] -> buf1[i0][i1][p2]
   {
@@ -388,7 +547,8 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.Lin = n1; P.Lout = N1; P.h = N1 / 2;
     P.backward = 1;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);
-    hipLaunchKernelGGL((fft_pass<R, false>), grid_dim, dim3(kThreads), lds_bytes(F.ax[1]), s, P);
+    launch_pass<R>(false, grid_dim, P, s);
+    if (mark2) (void)hipEventRecord(mark2, s);
   }
   // axis 2: rows buf1[i0][i1][:] -> f_hat[i0][i1][:] (D^T fused; c2r keeps k2 in [-N2/2, 0])
   {
@@ -403,7 +563,7 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     if (tab) { P.tab0 = tab; P.tab1 = tab + N0; P.tab2 = tab + N0 + N1; }
     P.N1 = N1;
     dim3 grid_dim((P.n_inner + P.C - 1) / P.C, 1);
-    hipLaunchKernelGGL((fft_pass<R, true>), grid_dim, dim3(kThreads), lds_bytes(F.ax[2]), s, P);
+    launch_pass<R>(true, grid_dim, P, s);
   }
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
@@ -418,15 +578,8 @@ int Fft3d<R>::init(const int* N_, const int* n_) {
     if (st != kOk) return st;
   }
   for (int t = 0; t < 3; t++) {
-    const size_t bytes = sizeof(Cplx<R>) * (size_t)ax[t].L * (size_t)(ax[t].C + 1);
-    if (bytes > 64 * 1024) {
-      if (hipFuncSetAttribute((const void*)fft_pass<R, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes) != hipSuccess)
-        return kHipError;
-      if (hipFuncSetAttribute((const void*)fft_pass<R, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes) != hipSuccess)
-        return kHipError;
-    }
+    const int st = with_length<R>(ax[t].L, [](auto Lc) { return pass_attr<R, decltype(Lc)::value>(); });
+    if (st != kOk) return st;
   }
   return kOk;
 }
@@ -441,12 +594,12 @@ template struct FftAxis<double>;
 template struct Fft3d<float>;
 template struct Fft3d<double>;
 template int fft3d_forward<float>(const Fft3d<float>&, const Cplx<float>*, const double*, Cplx<float>*,
-                                  Cplx<float>*, Cplx<float>*, hipStream_t, bool);
+                                  Cplx<float>*, Cplx<float>*, hipStream_t, bool, hipEvent_t, hipEvent_t);
 template int fft3d_forward<double>(const Fft3d<double>&, const Cplx<double>*, const double*, Cplx<double>*,
-                                   Cplx<double>*, Cplx<double>*, hipStream_t, bool);
+                                   Cplx<double>*, Cplx<double>*, hipStream_t, bool, hipEvent_t, hipEvent_t);
 template int fft3d_backward<float>(const Fft3d<float>&, const Cplx<float>*, const double*, Cplx<float>*,
-                                   Cplx<float>*, Cplx<float>*, int, hipStream_t, bool);
+                                   Cplx<float>*, Cplx<float>*, int, hipStream_t, bool, hipEvent_t, hipEvent_t);
 template int fft3d_backward<double>(const Fft3d<double>&, const Cplx<double>*, const double*, Cplx<double>*,
-                                    Cplx<double>*, Cplx<double>*, int, hipStream_t, bool);
+                                    Cplx<double>*, Cplx<double>*, int, hipStream_t, bool, hipEvent_t, hipEvent_t);
 
 }  // namespace pnfft_amd

@@ -17,8 +17,6 @@ template <class R>
 struct FftAxis {
   int L = 0;
   int C = 0;          // pencils per block
-  int nst = 0;
-  int radix[16] = {0};
   Cplx<R>* tw = nullptr;  // device twiddles e^{-2 pi i j / L}
   int init(int L);
   void release();
@@ -37,15 +35,18 @@ struct Fft3d {
 };
 
 // grid <- F D fhat  (tab = concatenated FP64 D factors, or null to skip D).
+// mark1 / mark2 (optional) are recorded on s after the first and second pass.
 // half: fhat holds the c2r half spectrum k2 in [-N2/2, 0]; the grid's real part is then the
 // c2r transform (the k2 < 0 coefficients count twice, the k2 = 0 plane once).
 template <class R>
 int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
-                  Cplx<R>* grid, hipStream_t s, bool half = false);
+                  Cplx<R>* grid, hipStream_t s, bool half = false, hipEvent_t mark1 = nullptr,
+                  hipEvent_t mark2 = nullptr);
 
 // fhat (+)= D^T F^H grid; half: only k2 in [-N2/2, 0] is produced (r2c of a real grid).
 template <class R>
 int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
-                   Cplx<R>* fhat, int accumulate, hipStream_t s, bool half = false);
+                   Cplx<R>* fhat, int accumulate, hipStream_t s, bool half = false, hipEvent_t mark1 = nullptr,
+                   hipEvent_t mark2 = nullptr);
 
 }  // namespace pnfft_amd

@@ -32,6 +32,7 @@ struct BArgs {
   int n_items;
   int ts[3];               // tile size per axis
   int ntile[3];            // tiles per axis
+  hipEvent_t mid;          // host only: recorded on the stream between the tiled and the direct launch
 };
 
 constexpr int kTileThreads = 512;  // threads per block of the tiled kernels

@@ -213,4 +213,89 @@ __host__ __device__ inline void window_axis(int win, int m, R frac, R nR, R b, R
   }
 }
 
+// bspline_values with the order K fixed at compile time: same recursion and
+// operation order, every index static so V stays in registers.
+template <int K, class R>
+__host__ __device__ inline void bspline_values_s(R u, R (&V)[K]) {
+  V[0] = R(1);
+#pragma unroll
+  for (int r = 2; r <= K; r++) {
+    const R inv = R(1) / R(r - 1);
+    V[r - 1] = R(0);
+#pragma unroll
+    for (int d = r - 1; d >= 0; d--) {
+      const R left = (d > 0) ? V[d - 1] : R(0);
+      V[d] = ((u + R(d)) * V[d] + (R(r) - u - R(d)) * left) * inv;
+    }
+  }
+}
+
+// window_axis for a stencil width CM = 2m+1 known at compile time (the device
+// kernels for m = 2..8).  Values are identical to window_axis.
+template <class R, int CM, bool WD>
+__host__ __device__ inline void window_axis_s(int win, R frac, R nR, R b, R (&psi)[CM], R (&dpsi)[CM]) {
+  constexpr int m = (CM - 1) / 2;
+  const R pi = pi_v(R(0));
+  switch (win) {
+    case WIN_GAUSS: {
+      const R norm = R(1) / m_sqrt(pi * b);
+#pragma unroll
+      for (int s = 0; s < CM; s++) {
+        const R u = R(s - m) - frac;
+        psi[s] = m_exp(-(u * u) / b) * norm;
+      }
+      if (WD) {
+#pragma unroll
+        for (int s = 0; s < CM; s++) dpsi[s] = R(-2) * nR / b * (frac + R(m - s)) * psi[s];
+      }
+      break;
+    }
+    case WIN_BSPLINE: {
+      R V[2 * m];
+      const R u = R(1) - frac;
+      bspline_values_s<2 * m, R>(u, V);
+      psi[0] = R(0);
+#pragma unroll
+      for (int s = 1; s < CM; s++) psi[s] = V[s - 1];
+      if (WD) {
+        R W[2 * m - 1];
+        bspline_values_s<2 * m - 1, R>(u, W);
+#pragma unroll
+        for (int s = 0; s < CM; s++) {
+          const R a = (s >= 1 && s - 1 < 2 * m - 1) ? W[s - 1] : R(0);
+          const R bb = (s >= 2 && s - 2 < 2 * m - 1) ? W[s - 2] : R(0);
+          dpsi[s] = -nR * (a - bb);
+        }
+      }
+      break;
+    }
+    case WIN_SINCPOW: {
+#pragma unroll
+      for (int s = 0; s < CM; s++) {
+        const R u = R(s - m) - frac;
+        psi[s] = ipow(sinc_ref<R>(pi * u / b), 2 * m) / b;
+      }
+      if (WD) {
+#pragma unroll
+        for (int s = 0; s < CM; s++) {
+          const R y = pi * (frac + R(m - s)) / b;
+          dpsi[s] = (m_fabs(y) > Consts<R>::eps)
+                        ? R(2 * m) * pi * nR / b * (R(1) / m_tan(y) - R(1) / y) * psi[s]
+                        : R(0);
+        }
+      }
+      break;
+    }
+    default: {  // Kaiser-Bessel
+#pragma unroll
+      for (int s = 0; s < CM; s++) psi[s] = kb_psi<R>(R(s - m) - frac, b, m);
+      if (WD) {
+#pragma unroll
+        for (int s = 0; s < CM; s++) dpsi[s] = kb_dpsi<R>(frac + R(m - s), nR, b, m, psi[s]);
+      }
+      break;
+    }
+  }
+}
+
 }  // namespace pnfft_amd

@@ -110,6 +110,20 @@ class DevicePlan:
         self.api.timer_free(a)
         return out
 
+    KT_NAMES = ("fft_rows", "fft_cols1", "fft_cols0", "gather_tiled", "gather_direct", "scatter_tiled",
+                "scatter_direct", "binning", "grid_zero")
+
+    def kernel_times(self):
+        """{kernel class: (device ms accumulated, launches)} since the plan was made or reset."""
+        n = len(self.KT_NAMES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_long * n)()
+        self.api.kernel_times(self.plan, ms, cnt, n)
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KT_NAMES)}
+
+    def reset_timers(self):
+        self.api.reset_timer(self.plan)
+
     def close(self):
         if self.nodes:
             self.api.free_nodes(self.nodes, 0)

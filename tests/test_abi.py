@@ -129,3 +129,21 @@ def test_no_cpu_fallback_in_product():
                 txt = open(os.path.join(dirpath, fn)).read()
                 assert not re.search(r"^\s*(from|import)\s+oracle", txt, re.M), fn
                 assert "liboracle" not in txt, fn
+
+
+def test_kernel_time_classes_match_library(tmp_path):
+    """PNFFT_KT_COUNT of the header equals what the library's kernel_times reports (null plan: no copy)."""
+    if not _cc_available("gcc"):
+        pytest.skip("gcc missing")
+    src = tmp_path / "kt.c"
+    src.write_text('#include <stdio.h>\n#include "pnfft.h"\nint main(void){ printf("%d\\n", PNFFT_KT_COUNT); '
+                   'return 0; }\n')
+    exe = tmp_path / "kt"
+    subprocess.run(["gcc", "-std=c99", "-I", INC, str(src), "-o", str(exe)], check=True)
+    count = int(subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout)
+    import pnfft_amd as PA
+    from pnfft_amd.device import DevicePlan
+
+    for prec in ("double", "float"):
+        assert PA.lib().api(prec).kernel_times(None, None, None, 0) == count
+    assert len(DevicePlan.KT_NAMES) == count

@@ -292,3 +292,29 @@ def test_reference_sort_matches_stable_argsort(oracle):
     order = np.argsort(k, kind="stable")
     assert np.array_equal(perm, order)
     assert np.array_equal(keys, k[order])
+
+
+# ---------------------------------------------------------------- threaded scatter (CPU baseline)
+def test_block_parallel_scatter_matches_serial(oracle):
+    """orc_B_adj_blocks (four-phase block-parallel scatter) equals the serial orc_B_adj."""
+    import ctypes
+
+    P = oracle.Plan([16, 8, 16], m=3)
+    rng = np.random.default_rng(12)
+    M = 3000
+    x = rng.uniform(-0.5, 0.5, (M, 3))
+    x[: M // 2] = 0.05 * rng.normal(size=(M // 2, 3))  # a dense cluster
+    f = rng.normal(size=M) + 1j * rng.normal(size=M)
+    g = rng.normal(size=3 * M) + 1j * rng.normal(size=3 * M)
+    L = oracle.lib()
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    lp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
+    n3, no3 = np.asarray(P.n, np.int64), np.asarray(P.no, np.int64)
+    xs = np.ascontiguousarray(x.reshape(-1))
+    out = []
+    for fn in (L.orc_B_adj, L.orc_B_adj_blocks):
+        grid = np.zeros([v + 2 * P.m for v in P.no], np.complex128)
+        fn(lp(n3), lp(no3), P.m, P.window, dp(P.b), M, dp(xs), dp(f.view(np.float64)), dp(g.view(np.float64)), 0,
+           dp(grid.view(np.float64)), 3)
+        out.append(grid)
+    assert oracle.rel_l2(out[1], out[0]) < 1e-14
