@@ -56,7 +56,6 @@ def algorithmic_bytes(w, precision, c2r=False):
         "gather_f": cs * nt + M * (node + cs),
         "gather_grad": cs * nt + M * (node + 3 * cs),
         "scatter_f": cs * nt + M * (node + cs),
-        "grid_zero": cs * nt,
     }
 
 
@@ -64,8 +63,9 @@ def roofline_from_kernels(kt, ab, steps):
     """Dominant kernel class of one step and its achieved HBM rate.
 
     kt: {class: (ms, launches)} over the timed steps (HIP events on the plan's stream).  One step
-    launches: FFT rows 2 forward + 1 backward, cols1 3, cols0 3; gather (tiled + direct pair) 2
-    (f, grad); scatter (tiled + direct pair) 1; grid clearing 1.
+    launches: FFT rows 2 forward + 1 backward, cols1 3, cols0 3; gather stage (tiled + direct
+    kernels) 2 (f, grad); scatter stage (tiled kernel, fold of the tile subgrids into the grid,
+    direct kernel for stale bins) 1.
     """
     def ms(k):
         return kt[k][0] / steps
@@ -75,13 +75,15 @@ def roofline_from_kernels(kt, ab, steps):
         "fft_cols1": (ms("fft_cols1"), 3 * ab["fft_cols1"], 3),
         "fft_cols0": (ms("fft_cols0"), 3 * ab["fft_cols0"], 3),
         "gather_tiled+direct": (ms("gather_tiled") + ms("gather_direct"), ab["gather_f"] + ab["gather_grad"], 2),
-        "scatter_tiled+direct": (ms("scatter_tiled") + ms("scatter_direct"), ab["scatter_f"], 1),
-        "grid_zero": (ms("grid_zero"), ab["grid_zero"], 1),
+        "scatter_tiled+fold+direct": (ms("scatter_tiled") + ms("scatter_fold") + ms("scatter_direct"),
+                                      ab["scatter_f"], 1),
     }
     dom = max(classes, key=lambda k: classes[k][0])
     t_ms, nbytes, nl = classes[dom]
     achieved = (nbytes / nl) / (t_ms / nl * 1e-3) / 1e9 if t_ms > 0 else 0.0
     per_kernel = {k: round(v[0], 3) for k, v in classes.items()}
+    per_kernel.update({k: round(ms(k), 3) for k in ("gather_tiled", "gather_direct", "scatter_tiled", "scatter_fold",
+                                                    "scatter_direct")})
     return dom, achieved, nbytes / nl, t_ms / nl, per_kernel
 
 
@@ -201,6 +203,22 @@ def cpu_baseline(w, sample_M, threads):
                 seconds=t)
 
 
+def copy_probe():
+    """Plain device copy of known size: 3 copies of 2^30 bytes (reads 2^30, writes 2^30 each)."""
+    import torch
+
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes_read_per_launch": 1 << 30, "copy_bytes_written_per_launch": 1 << 30,
+                      "launches": 3, "GB_per_s": round(3 * 2 * (1 << 30) / el / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,7 +229,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy a 1 GiB device buffer 3 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
               "described in DESIGN.md but not built", file=sys.stderr)

@@ -194,7 +194,7 @@ extern "C" {
 #define PNFFT_KT_SCATTER_TILED 5   /* adj B^T, LDS-tiled kernel */
 #define PNFFT_KT_SCATTER_DIRECT 6  /* adj B^T, direct kernel */
 #define PNFFT_KT_BINNING 7         /* node binning (keys, radix sort, work items) */
-#define PNFFT_KT_GRID_ZERO 8       /* clearing the oversampled grid before B^T */
+#define PNFFT_KT_SCATTER_FOLD 8    /* adj B^T, sum of the tiles' subgrids into the grid */
 #define PNFFT_KT_COUNT 9
 
 #define PNFFT_MANGLE_DOUBLE(name) pnfft_##name

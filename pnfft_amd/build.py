@@ -23,6 +23,12 @@ B_VARIANTS = [
                          ("scatter", 1, ((1, "f"), (2, "grad"))))
     for mi, m in modes
 ]
+# scatter_cols.hip is compiled once per (precision, mode)
+COLS_VARIANTS = [
+    (f"{r}_{m}", [f"-DPNFFT_B_REAL={'float' if r == 'f32' else 'double'}", f"-DPNFFT_B_MODE={mi}"])
+    for r in ("f32", "f64")
+    for mi, m in ((1, "f"), (2, "grad"), (3, "fgrad"))
+]
 ARCH = os.environ.get("PNFFT_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -54,7 +60,8 @@ def _compile(job, force=True):
 def build(verbose=True, force=True):
     """Compile every translation unit (force=False skips objects newer than their inputs)."""
     os.makedirs(BUILD, exist_ok=True)
-    jobs = [("b_kernels.hip", "_" + sfx, d) for sfx, d in B_VARIANTS] + [(s, "", []) for s in SOURCES]
+    jobs = ([("b_kernels.hip", "_" + sfx, d) for sfx, d in B_VARIANTS] +
+            [("scatter_cols.hip", "_" + sfx, d) for sfx, d in COLS_VARIANTS] + [(s, "", []) for s in SOURCES])
     with ThreadPoolExecutor(max_workers=max(1, min(len(jobs), os.cpu_count() or 4))) as ex:
         objs = list(ex.map(lambda j: _compile(j, force), jobs))
     tmp = LIB + ".tmp"

@@ -38,7 +38,7 @@ enum : int {
   PNFFT_KT_SCATTER_TILED = 5,
   PNFFT_KT_SCATTER_DIRECT = 6,
   PNFFT_KT_BINNING = 7,
-  PNFFT_KT_GRID_ZERO = 8,
+  PNFFT_KT_SCATTER_FOLD = 8,
   PNFFT_KT_COUNT = 9
 };
 
@@ -175,7 +175,7 @@ static bool is_device_ptr(const void* p) {
 // plan and nodes
 // ---------------------------------------------------------------------------
 // Event slots: 0/1 binning, 2/3 FFT (8, 9 after its first and second pass),
-// 4/5 B (10 between the tiled and the direct kernel), 11/12 grid clearing.
+// 4/5 B (10 and 13 between its kernels).
 struct EventTimer {
   hipEvent_t ev[16] = {};
   bool ok = false;
@@ -266,6 +266,13 @@ struct NodesT {
   int* d_items_s = nullptr;
   int n_items_s = 0;
   long items_cap = 0;
+  // column scatter: nonempty tiles by decreasing node count, tile -> slot map, slots
+  int* d_torder = nullptr;
+  int* d_tile_slot = nullptr;
+  unsigned* d_twork = nullptr;
+  Cplx<R>* d_slots = nullptr;
+  size_t slots_cap = 0;  // complex elements
+  int n_torder = 0;
   SortWork sw;
   bool bins_valid = false;
   GridGeom bin_geom{};
@@ -424,6 +431,12 @@ static void nodes_release_device(NodesT<R>* D) {
   dfree(D->d_items_s);
   D->items_cap = 0;
   D->n_items = D->n_items_s = 0;
+  dfree(D->d_torder);
+  dfree(D->d_tile_slot);
+  dfree(D->d_twork);
+  dfree(D->d_slots);
+  D->slots_cap = 0;
+  D->n_torder = 0;
   dfree(D->sw.keys[0]);
   dfree(D->sw.keys[1]);
   dfree(D->sw.vals[0]);
@@ -511,12 +524,20 @@ static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
     dfree(D->d_tpartial);
     dfree(D->d_dense);
     dfree(D->d_dense_s);
+    dfree(D->d_torder);
+    dfree(D->d_tile_slot);
+    dfree(D->d_twork);
     D->d_tile_start = dalloc<int>(T.ntiles() + 1);
     D->d_chunks = dalloc<unsigned>(2 * T.ntiles());
     D->d_tpartial = dalloc<unsigned>((T.ntiles() + 4095) / 4096 + 1);
     D->d_dense = dalloc<unsigned char>(T.ntiles());
     D->d_dense_s = dalloc<unsigned char>(T.ntiles());
-    if (!D->d_tile_start || !D->d_chunks || !D->d_tpartial || !D->d_dense || !D->d_dense_s) return false;
+    D->d_torder = dalloc<int>(T.ntiles());
+    D->d_tile_slot = dalloc<int>(T.ntiles());
+    D->d_twork = dalloc<unsigned>(65);
+    if (!D->d_tile_start || !D->d_chunks || !D->d_tpartial || !D->d_dense || !D->d_dense_s || !D->d_torder ||
+        !D->d_tile_slot || !D->d_twork)
+      return false;
     D->tile_cap = T.ntiles() + 1;
   }
   const long icap = T.ntiles() + (M + kTileNodes - 1) / kTileNodes + 1;
@@ -529,6 +550,16 @@ static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
     D->items_cap = icap;
   }
   return true;
+}
+
+// Scatter path: the column scatter (default) or the LDS-atomic kernels
+// (PNFFT_AMD_SCATTER=atomic, for comparison).
+static bool cols_enabled() {
+  static bool v = [] {
+    const char* e = getenv("PNFFT_AMD_SCATTER");
+    return !(e && strcmp(e, "atomic") == 0);
+  }();
+  return v;
 }
 
 // Minimum node count for a tile to take the LDS path (tunable by PNFFT_AMD_DENSE_MIN).
@@ -547,6 +578,12 @@ static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false)
   A.tile_dense = for_scatter ? D->d_dense_s : D->d_dense;
   A.items = for_scatter ? D->d_items_s : D->d_items;
   A.n_items = for_scatter ? D->n_items_s : D->n_items;
+  if (for_scatter && D->n_torder > 0) {
+    A.slots = D->d_slots;
+    A.tile_slot = D->d_tile_slot;
+    A.torder = D->d_torder;
+    A.n_torder = D->n_torder;
+  }
   for (int t = 0; t < 3; t++) {
     A.ts[t] = D->bin_tiles.ts[t];
     A.ntile[t] = D->bin_tiles.ntile[t];
@@ -582,6 +619,26 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx) {
     D->n_items = ni;
     D->n_items_s = ns;
   }
+  D->n_torder = 0;
+  const int pad = cols_enabled() ? cols_pad(P->m, sizeof(R) == 8, P->tiles.ts) : 0;
+  if (pad > 0 && D->M > 0) {
+    const int nt = build_tile_order(D->d_tile_start, P->tiles.ntiles(), D->d_twork, D->d_torder, D->d_tile_slot,
+                                    P->stream);
+    if (nt < 0) {
+      set_error("building the tile order failed");
+      return kHipError;
+    }
+    const size_t need = (size_t)nt * (size_t)pad * pad * pad;
+    if (D->slots_cap < need) {
+      dfree(D->d_slots);
+      D->slots_cap = 0;
+      D->d_slots = dalloc<Cplx<R>>(need);
+      if (D->d_slots) D->slots_cap = need;
+      (void)hipGetLastError();
+    }
+    // without slot memory the scatter takes the LDS-atomic path
+    D->n_torder = D->d_slots ? nt : 0;
+  }
   D->bin_geom = P->geom;
   D->bin_tiles = P->tiles;
   D->bins_valid = true;
@@ -595,10 +652,11 @@ static void kt_add(PlanT<R>* P, int k, int a, int b) {
 }
 
 // Accumulates the reference's stage timers (seconds) and the per-kernel times.
-// fwd: trafo (FFT passes rows, cols1, cols0; gather) or adj (cols0, cols1, rows; scatter).
+// fwd: trafo (FFT passes rows, cols1, cols0; gather) or adj (cols0, cols1, rows;
+// scatter).  The B stage is up to three kernels separated by events 10 and 13:
+// tiled, fold (column scatter only) and direct.
 template <class R>
-static void finish_timers(PlanT<R>* P, double* timer, bool fwd, bool had_sort, bool had_f, bool had_b,
-                          bool had_zero = false) {
+static void finish_timers(PlanT<R>* P, double* timer, bool fwd, bool had_sort, bool had_f, bool had_b) {
   if (had_sort) {
     timer[T_SORT] += 1e-3 * P->evt.ms(0, 1);
     kt_add(P, PNFFT_KT_BINNING, 0, 1);
@@ -615,9 +673,9 @@ static void finish_timers(PlanT<R>* P, double* timer, bool fwd, bool had_sort, b
     timer[T_B] += t;
     timer[T_LOOP_B] += t;
     kt_add(P, fwd ? PNFFT_KT_GATHER_TILED : PNFFT_KT_SCATTER_TILED, 4, 10);
-    kt_add(P, fwd ? PNFFT_KT_GATHER_DIRECT : PNFFT_KT_SCATTER_DIRECT, 10, 5);
+    if (!fwd) kt_add(P, PNFFT_KT_SCATTER_FOLD, 10, 13);
+    kt_add(P, fwd ? PNFFT_KT_GATHER_DIRECT : PNFFT_KT_SCATTER_DIRECT, 13, 5);
   }
-  if (had_zero) kt_add(P, PNFFT_KT_GRID_ZERO, 11, 12);
 }
 
 // api/api-basic.c:199-244
@@ -697,6 +755,7 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
       A.c2r = P->c2r;
       set_tiles(A, D);
       A.mid = P->evt.mark(10);
+      A.mid2 = P->evt.mark(13);
       int mode = (want_f && want_g) ? B_FGRAD : (want_f ? B_F : B_GRAD);
       P->evt.rec(4, s);
       if (gather<R>(A, mode, s) != kOk) set_error("gather launch failed");
@@ -740,7 +799,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
   // zero f_hat unless accumulating (:355-356)
   if (!accumulate) fill_zero<R>(dfh, P->fhat_elems, s);
 
-  bool had_sort = false, had_f = false, had_b = false, had_zero = false;
+  bool had_sort = false, had_f = false, had_b = false;
   if (cf & fl::COMPUTE_DIRECT) {
     NdftArgs<R> A{};
     for (int t = 0; t < 3; t++) A.N[t] = (int)P->N[t];
@@ -755,11 +814,8 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     if (ndft_adj<R>(A, s) != kOk) set_error("NDFT adj launch failed");
   } else {
     if (!(cf & fl::OMIT_CONV)) {
-      // B^T into the zeroed grid (:2629-2635)
-      P->evt.rec(11, s);
-      fill_zero<R>(P->d_grid, P->n_total, s);
-      P->evt.rec(12, s);
-      had_zero = true;
+      // grid = B^T (f, grad): the reference scatters into the zeroed grid (:2629-2635);
+      // scatter() writes every cell itself
       if (M > 0 && (use_f || use_g)) {
         const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x &&
                            same_geom(D->bin_geom, P->geom);
@@ -780,11 +836,14 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
         A.c2r = P->c2r;
         set_tiles(A, D, true);
         A.mid = P->evt.mark(10);
+        A.mid2 = P->evt.mark(13);
         const int mode = (use_f && use_g) ? B_FGRAD : (use_f ? B_F : B_GRAD);
         P->evt.rec(4, s);
         if (scatter<R>(A, mode, s) != kOk) set_error("scatter launch failed");
         P->evt.rec(5, s);
         had_b = true;
+      } else {
+        fill_zero<R>(P->d_grid, P->n_total, s);
       }
     }
     // F^H and D^T (fused), f_hat += D^T F^H grid
@@ -802,7 +861,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     set_error("adj: HIP error %s", hipGetErrorString(hipGetLastError()));
     return;
   }
-  finish_timers(P, P->timer_adj, false, had_sort, had_f, had_b, had_zero);
+  finish_timers(P, P->timer_adj, false, had_sort, had_f, had_b);
   P->timer_adj[T_ITER] += 1;
   P->timer_adj[T_WHOLE] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

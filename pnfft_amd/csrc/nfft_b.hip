@@ -215,6 +215,44 @@ __global__ __launch_bounds__(kB) void tile_items_k(const unsigned* __restrict__ 
   }
 }
 
+// nonempty tiles by node-count class (leading zeros of the count: more nodes,
+// smaller class), then placed class by class: most nodes first
+__global__ __launch_bounds__(kB) void tile_class_k(const int* __restrict__ tile_start, long ntiles, unsigned* work,
+                                                   int* tile_slot) {
+  const long t = (long)blockIdx.x * kB + threadIdx.x;
+  if (t >= ntiles) return;
+  const int cnt = tile_start[t + 1] - tile_start[t];
+  tile_slot[t] = -1;
+  if (cnt > 0) atomicAdd(&work[__clz(cnt)], 1u);
+}
+
+// work[0..31] class sizes -> work[32..63] class offsets, work[64] total
+__global__ void tile_class_scan_k(unsigned* work) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned acc = 0;
+  for (int k = 0; k < 32; k++) {
+    work[32 + k] = acc;
+    acc += work[k];
+  }
+  work[64] = acc;
+}
+
+__global__ __launch_bounds__(kB) void tile_place_k(const int* __restrict__ tile_start, long ntiles, unsigned* work,
+                                                   int* order, int* tile_slot) {
+  const long t = (long)blockIdx.x * kB + threadIdx.x;
+  if (t >= ntiles) return;
+  const int cnt = tile_start[t + 1] - tile_start[t];
+  if (cnt <= 0) return;
+  const unsigned pos = atomicAdd(&work[32 + __clz(cnt)], 1u);
+  order[pos] = (int)t;
+  tile_slot[t] = (int)pos;
+}
+
+__global__ void fill_uint_k(unsigned* p, long n, unsigned v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 template <class R>
 __global__ void fill_zero_k(Cplx<R>* p, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -292,6 +330,25 @@ int build_items(const int* tile_start, long ntiles, int dense_min, unsigned* chu
   return (int)total;
 }
 
+int cols_pad(int m, bool is_double, const int* ts) {
+  if (m < 2 || m > 8 || (is_double && m > 6)) return 0;
+  const int TS = m <= 6 ? 8 : 4;
+  if (ts[0] != TS || ts[1] != TS || ts[2] != TS) return 0;
+  return TS + 2 * m;
+}
+
+int build_tile_order(const int* tile_start, long ntiles, unsigned* work, int* order, int* tile_slot, hipStream_t s) {
+  hipLaunchKernelGGL(fill_uint_k, dim3(1), dim3(128), 0, s, work, 65L, 0u);
+  hipLaunchKernelGGL(tile_class_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, tile_slot);
+  hipLaunchKernelGGL(tile_class_scan_k, dim3(1), dim3(64), 0, s, work);
+  hipLaunchKernelGGL(tile_place_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, order,
+                     tile_slot);
+  unsigned total = 0;
+  if (hipMemcpyAsync(&total, work + 64, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  return (int)total;
+}
+
 template <class R>
 int gather(const BArgs<R>& A, int mode, hipStream_t s) {
   if (A.M == 0) return kOk;
@@ -309,6 +366,19 @@ template <class R>
 int scatter(const BArgs<R>& A, int mode, hipStream_t s) {
   if (A.M == 0) return kOk;
   if (A.g.m < 1 || A.g.m > kMaxM) return kUnsupported;
+  if (A.slots) {  // column path: writes every grid cell through the fold
+    int st = kUnsupported;
+    switch (mode) {
+      case B_F: st = cols_launch<R, B_F>(A, s); break;
+      case B_GRAD: st = cols_launch<R, B_GRAD>(A, s); break;
+      case B_FGRAD: st = cols_launch<R, B_FGRAD>(A, s); break;
+      default: break;
+    }
+    if (st != kUnsupported) return st;  // kUnsupported: nothing was launched, take the LDS-atomic path
+  }
+  // LDS-atomic path: accumulates into the cleared grid
+  const long ntot = (long)A.g.n[0] * A.g.n[1] * A.g.n[2];
+  hipLaunchKernelGGL((fill_zero_k<R>), dim3(grid1d(ntot, 256)), dim3(256), 0, s, A.grid, (size_t)ntot);
   switch (mode) {
     case B_F: return b_launch<R, 1, B_F>(A, s);
     case B_GRAD: return b_launch<R, 1, B_GRAD>(A, s);

@@ -32,10 +32,34 @@ struct BArgs {
   int n_items;
   int ts[3];               // tile size per axis
   int ntile[3];            // tiles per axis
-  hipEvent_t mid;          // host only: recorded on the stream between the tiled and the direct launch
+  // column scatter (scatter_cols.hip): when slots is set, scatter() runs it instead of the LDS-atomic path
+  Cplx<R>* slots;          // [n_torder][P^3] padded subgrid of every nonempty tile
+  const int* tile_slot;    // [ntiles] slot of each tile, -1 if empty
+  const int* torder;       // [n_torder] nonempty tiles, most nodes first
+  int n_torder;
+  hipEvent_t mid;          // host only: recorded between the first and the second kernel of the stage
+  hipEvent_t mid2;         // host only: recorded between the second and the third kernel of the stage
 };
 
 constexpr int kTileThreads = 512;  // threads per block of the tiled kernels
+
+// Column scatter: applies for m = 2..8 (FP64: m <= 6) when all three tile
+// sizes equal the layout's TS (8 for m <= 6, 4 above); P = TS + 2m cells per
+// axis of a tile's padded subgrid.  Returns P, or 0 if the path does not apply.
+int cols_pad(int m, bool is_double, const int* ts);
+
+// Nonempty tiles ordered by decreasing node count (power-of-two classes, most
+// nodes first) and tile_slot[t] = position of tile t in that order (-1 if
+// empty).  work holds 64 unsigned.  Returns the number of nonempty tiles (host
+// value; synchronizes s), or -1.
+int build_tile_order(const int* tile_start, long ntiles, unsigned* work, int* order, int* tile_slot, hipStream_t s);
+
+// Launch of the column scatter for MODE (B_F, B_GRAD, B_FGRAD): column kernel
+// into the slots, fold of the slots into the grid (which it overwrites), then
+// the nodes whose bins are older than their coordinates through the direct
+// path (global atomics).
+template <class R, int MODE>
+int cols_launch(const BArgs<R>& A, hipStream_t s);
 constexpr int kTileNodes = 512;    // nodes per work item
 
 // Work items of the tiled kernels: every tile with at least dense_min nodes is
@@ -78,6 +102,8 @@ int b_launch(const BArgs<R>& A, hipStream_t s);
 template <class R>
 int gather(const BArgs<R>& A, int mode, hipStream_t s);
 
+// grid = B^T (f, grad): the whole grid is written (the column path overwrites
+// it through the fold; the LDS-atomic path clears it first).
 template <class R>
 int scatter(const BArgs<R>& A, int mode, hipStream_t s);
 

@@ -1,0 +1,299 @@
+// scatter_cols.hip -- adjoint window scatter B^T with register accumulators.
+//
+// This file is compiled once per (precision, mode); the build passes
+// PNFFT_B_REAL (float|double) and PNFFT_B_MODE (B_F, B_GRAD or B_FGRAD).
+//
+// Reference: the scatter node loop of kernel/ndft-parallel.c:2890-3009 with
+// the stencil kernels spread_f_c2c / spread_grad_f_c2c of
+// kernel/assign.c:478-640 (cutoff 2m+1, kernel/ndft-parallel.c:910).
+//
+// One block per nonempty tile, tiles with more nodes first.  The tile's padded
+// subgrid (P = TS + 2m cells per axis from the tile origin; every node binned to
+// the tile has its stencil inside it) lives in registers: thread (b, c) owns
+// the column (., b, c), and for FP32 f with m <= 4 also (., b + P/2, c), and keeps
+// their cells as FP64 (re, im) accumulators.
+// The tile's nodes are taken in sorted order, in batches: one thread per node
+// evaluates the window values into LDS, then each node is applied by the
+// threads whose columns lie in its stencil; the node's x-window is stored
+// shifted to its plane offset in a zeroed row of P planes, so every thread
+// updates its accumulators with fixed register indices (two-plane blocks the
+// window does not reach are skipped by a uniform branch).
+// Each thread then writes its column to the tile's slot, and the fold kernel
+// sums, for every grid cell, the slots of the tiles whose subgrids cover it in
+// a fixed order and writes the cell.  Accumulation and fold use no atomics, so
+// the adjoint is deterministic; only nodes whose bins are older than their
+// coordinates go through the direct path (global atomics) afterwards.
+#include "b_device.hpp"
+
+namespace pnfft_amd {
+
+namespace {
+
+using namespace bdev;
+
+template <class R, int MT, int MODE>
+struct ColsShape {
+  static constexpr int CM = 2 * MT + 1;
+  static constexpr int TS = MT <= 6 ? 8 : 4;
+  static constexpr int P = TS + 2 * MT;  // even
+  static constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
+  static constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
+  // columns per thread: two (rows b and b + P/2) where their FP64 accumulators
+  // fit in 256 VGPRs with the rest of the kernel (FP32, f only, m <= 4), else one
+  static constexpr int NC = (sizeof(R) == 4 && MT <= 4 && !WG) ? 2 : 1;
+  static constexpr int H = P / NC;       // rows covered by the first column of each thread
+  static constexpr int NCOL = H * P;     // threads that own columns
+  static constexpr int NT = (NCOL + 63) / 64 * 64;
+  static constexpr int NB = WG ? 64 : 128;  // nodes per batch
+};
+
+// Planes are updated in blocks of two; block k (planes 2k, 2k+1) is skipped by
+// a uniform branch when the node's window [l0, l0 + CM) does not reach it.
+template <int NC, int P, int CM, bool WG, int K>
+__device__ __attribute__((always_inline)) inline void plane_blocks(int l0, double (&ar)[NC][P], double (&ai)[NC][P],
+                                                                   const double* __restrict__ px,
+                                                                   const double* __restrict__ dx,
+                                                                   const double (&w)[NC][4]) {
+  if constexpr (2 * K < P) {
+    if (2 * K + 1 >= l0 && 2 * K < l0 + CM) {
+#pragma unroll
+      for (int a = 2 * K; a < 2 * K + 2; a++) {
+        const double x = px[a];
+        const double d = WG ? dx[a] : 0.0;
+#pragma unroll
+        for (int r = 0; r < NC; r++) {
+          ar[r][a] = fma(x, w[r][0], ar[r][a]);
+          ai[r][a] = fma(x, w[r][1], ai[r][a]);
+          if (WG) {
+            ar[r][a] = fma(d, w[r][2], ar[r][a]);
+            ai[r][a] = fma(d, w[r][3], ai[r][a]);
+          }
+        }
+      }
+    }
+    plane_blocks<NC, P, CM, WG, K + 1>(l0, ar, ai, px, dx, w);
+  }
+}
+
+template <class R, int MT, int MODE>
+__global__ __launch_bounds__((ColsShape<R, MT, MODE>::NT)) void scatter_cols_k(BArgs<R> A) {
+  using S = ColsShape<R, MT, MODE>;
+  constexpr int CM = S::CM, P = S::P, H = S::H, NB = S::NB, TS = S::TS, NC = S::NC;
+  constexpr bool WF = S::WF, WG = S::WG;
+  __shared__ int s_loc[NB];
+  __shared__ Cplx<R> s_f[WF ? NB : 1];
+  __shared__ Cplx<R> s_g[WG ? NB : 1][3];
+  // x-windows placed at the node's plane offset in a zeroed row of P planes, so
+  // that every accumulator has a fixed register (an index l0 + a would not)
+  __shared__ __align__(16) double s_px[NB][P];
+  __shared__ __align__(16) double s_dx[WG ? NB : 1][P];
+  __shared__ R s_py[NB][CM], s_pz[NB][CM];
+  __shared__ R s_dy[WG ? NB : 1][CM], s_dz[WG ? NB : 1][CM];
+
+  const int tile = A.torder[blockIdx.x];
+  const int t2 = tile % A.ntile[2];
+  const int t1 = (tile / A.ntile[2]) % A.ntile[1];
+  const int t0 = tile / (A.ntile[2] * A.ntile[1]);
+  const int o0 = t0 * TS, o1 = t1 * TS, o2 = t2 * TS;
+  const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
+  const int beg = A.tile_start[tile], end = A.tile_start[tile + 1];
+  const int tid = threadIdx.x;
+  const int b = tid / P, c = tid % P;  // rows b (and b + H when NC = 2) of column c
+  const bool own = tid < S::NCOL;
+
+  double ar[NC][P], ai[NC][P];
+#pragma unroll
+  for (int r = 0; r < NC; r++)
+#pragma unroll
+    for (int a = 0; a < P; a++) ar[r][a] = ai[r][a] = 0.0;
+
+  for (int base = beg; base < end; base += NB) {
+    const int nb = min(NB, end - base);
+    if (tid < nb) {
+      const long j = A.perm[base + tid];
+      NodeWin<R, CM> w;
+      node_windows<R, MT, CM, WG>(A, j, MT, w);
+      const int l0 = local_off(w.start[0], o0, TS, n0);
+      const int l1 = local_off(w.start[1], o1, TS, n1);
+      const int l2 = local_off(w.start[2], o2, TS, n2);
+      const bool in = l0 >= 0 && l1 >= 0 && l2 >= 0;
+      s_loc[tid] = in ? (l0 | (l1 << 8) | (l2 << 16)) : -1;
+#pragma unroll
+      for (int a = 0; a < P; a++) {
+        s_px[tid][a] = 0.0;
+        if (WG) s_dx[tid][a] = 0.0;
+      }
+      const int o = in ? l0 : 0;
+#pragma unroll
+      for (int a = 0; a < CM; a++) {
+        s_px[tid][o + a] = (double)w.px[a];
+        s_py[tid][a] = w.py[a];
+        s_pz[tid][a] = w.pz[a];
+        if (WG) {
+          s_dx[tid][o + a] = (double)w.dx[a];
+          s_dy[tid][a] = w.dy[a];
+          s_dz[tid][a] = w.dz[a];
+        }
+      }
+      if (WF) s_f[tid] = load_f(A, j);
+      if (WG) {
+        s_g[tid][0] = load_g(A, j, 0);
+        s_g[tid][1] = load_g(A, j, 1);
+        s_g[tid][2] = load_g(A, j, 2);
+      }
+    }
+    __syncthreads();
+    int loc_next = s_loc[0];
+    for (int i = 0; i < nb; i++) {
+      const int loc = __builtin_amdgcn_readfirstlane(loc_next);
+      if (i + 1 < nb) loc_next = s_loc[i + 1];
+      if (loc < 0) continue;  // stencil start outside the tile: the stale pass takes the node
+      const int l0 = loc & 255, l1 = (loc >> 8) & 255, l2 = (loc >> 16) & 255;
+      const int dc = c - l2;
+      if (!own || (unsigned)dc >= (unsigned)CM) continue;
+      // per row r (b, b + H): weights of the f and d/dx terms (re, im), zero outside the stencil
+      double w[NC][4];
+      const R pz = s_pz[i][dc];
+      const R dz = WG ? s_dz[i][dc] : R(0);
+#pragma unroll
+      for (int r = 0; r < NC; r++) {
+        const int db = b + r * H - l1;
+        const bool in_b = (unsigned)db < (unsigned)CM;
+        const int dbc = in_b ? db : 0;
+        const R py = in_b ? s_py[i][dbc] : R(0);
+        w[r][0] = w[r][1] = w[r][2] = w[r][3] = 0.0;
+        if (WF) {
+          const Cplx<R> f = s_f[i];
+          const R q = py * pz;
+          w[r][0] = (double)(q * f.x);
+          w[r][1] = (double)(q * f.y);
+        }
+        if (WG) {
+          const R dy = in_b ? s_dy[i][dbc] : R(0);
+          const Cplx<R> g0 = s_g[i][0], g1 = s_g[i][1], g2 = s_g[i][2];
+          const R q0 = py * pz, q1 = dy * pz, q2 = py * dz;
+          w[r][2] = (double)(q0 * g0.x);
+          w[r][3] = (double)(q0 * g0.y);
+          w[r][0] += (double)(q1 * g1.x + q2 * g2.x);
+          w[r][1] += (double)(q1 * g1.y + q2 * g2.y);
+        }
+      }
+      plane_blocks<NC, P, CM, WG, 0>(l0, ar, ai, &s_px[i][0], WG ? &s_dx[i][0] : nullptr, w);
+    }
+    __syncthreads();
+  }
+
+  if (own) {
+    Cplx<R>* slot = A.slots + (size_t)blockIdx.x * (size_t)(P * P * P);
+#pragma unroll
+    for (int r = 0; r < NC; r++)
+#pragma unroll
+      for (int a = 0; a < P; a++) slot[(a * P + b + r * H) * P + c] = cmk<R>((R)ar[r][a], (R)ai[r][a]);
+  }
+}
+
+// grid[x] = sum over the tiles whose padded subgrids cover x of their slot
+// value at x, tiles taken in a fixed order; cells no nonempty tile covers get 0.
+// One block per tile of the grid (empty ones included), one thread per cell;
+// the neighbour tile (T - d) is the same for the whole block, so empty
+// neighbours are skipped by a uniform branch.
+template <class R, int TS, int P>
+__global__ __launch_bounds__(TS* TS* TS) void fold_k(BArgs<R> A) {
+  constexpr int D = (P + TS - 1) / TS;  // tiles per axis whose subgrids can cover a cell
+  constexpr size_t VOL = (size_t)P * P * P;
+  const int nt0 = A.ntile[0], nt1 = A.ntile[1], nt2 = A.ntile[2];
+  const int T = blockIdx.x;
+  const int T2 = T % nt2, T1 = (T / nt2) % nt1, T0 = T / (nt2 * nt1);
+  const int q2 = threadIdx.x % TS, q1 = (threadIdx.x / TS) % TS, q0 = threadIdx.x / (TS * TS);
+  double sr = 0.0, si = 0.0;
+#pragma unroll
+  for (int d0 = 0; d0 < D; d0++) {
+    const int s0 = (T0 - d0 % nt0 + nt0) % nt0;
+#pragma unroll
+    for (int d1 = 0; d1 < D; d1++) {
+      const int s1 = (T1 - d1 % nt1 + nt1) % nt1;
+#pragma unroll
+      for (int d2 = 0; d2 < D; d2++) {
+        const int s2 = (T2 - d2 % nt2 + nt2) % nt2;
+        const int slot = A.tile_slot[((long)s0 * nt1 + s1) * nt2 + s2];
+        if (slot < 0) continue;
+        const int p0 = q0 + d0 * TS, p1 = q1 + d1 * TS, p2 = q2 + d2 * TS;
+        if (p0 < P && p1 < P && p2 < P) {
+          const Cplx<R> v = A.slots[(size_t)slot * VOL + ((size_t)p0 * P + p1) * P + p2];
+          sr += (double)v.x;
+          si += (double)v.y;
+        }
+      }
+    }
+  }
+  const long n1 = A.g.n[1], n2 = A.g.n[2];
+  const long x0 = (long)T0 * TS + q0, x1 = (long)T1 * TS + q1, x2 = (long)T2 * TS + q2;
+  A.grid[(x0 * n1 + x1) * n2 + x2] = cmk<R>((R)sr, (R)si);
+}
+
+// nodes whose stencil start left the tile they were binned to (bins older
+// than the coordinates): added to the folded grid through global atomics
+template <class R, int MT, int MODE>
+__global__ __launch_bounds__(256) void scatter_stale_k(BArgs<R> A) {
+  using S = ColsShape<R, MT, MODE>;
+  constexpr int CM = S::CM;
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= A.M) return;
+  const long j = A.perm[p];
+  unsigned long long tile = 0;
+  for (int t = 0; t < 3; t++) {
+    const NodeAxis<R> a = node_axis<R>(A.x[3 * j + t], A.g.n[t], MT);
+    tile = tile * (unsigned long long)A.ntile[t] + (unsigned long long)(a.start / A.ts[t]);
+  }
+  const unsigned vol = (unsigned)(A.ts[0] * A.ts[1] * A.ts[2]);
+  if (tile == (unsigned long long)(A.keys[p] / vol)) return;
+  NodeWin<R, CM> w;
+  node_windows<R, MT, CM, S::WG>(A, j, MT, w);
+  scatter_node_global<R, CM, MODE, true>(A, j, w, CM);
+}
+
+inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <class R, int MT, int MODE>
+void launch_m(const BArgs<R>& A, hipStream_t s) {
+  using S = ColsShape<R, MT, MODE>;
+  hipLaunchKernelGGL((scatter_cols_k<R, MT, MODE>), dim3((unsigned)A.n_torder), dim3(S::NT), 0, s, A);
+  if (A.mid) (void)hipEventRecord(A.mid, s);
+  const long ntiles = (long)A.ntile[0] * A.ntile[1] * A.ntile[2];
+  hipLaunchKernelGGL((fold_k<R, S::TS, S::P>), dim3((unsigned)ntiles), dim3(S::TS * S::TS * S::TS), 0, s, A);
+  if (A.mid2) (void)hipEventRecord(A.mid2, s);
+  hipLaunchKernelGGL((scatter_stale_k<R, MT, MODE>), dim3(grid1d(A.M, 256)), dim3(256), 0, s, A);
+}
+
+}  // namespace
+
+template <class R, int MODE>
+int cols_launch(const BArgs<R>& A, hipStream_t s) {
+  if (A.M == 0 || A.n_torder <= 0) return kUnsupported;
+  // FP64 gradient modes would need more than 256 VGPRs per lane here: they keep the LDS-atomic path
+  if constexpr (sizeof(R) == 8 && MODE != B_F) {
+    return kUnsupported;
+  } else {
+    switch (A.g.m) {
+      case 2: launch_m<R, 2, MODE>(A, s); break;
+      case 3: launch_m<R, 3, MODE>(A, s); break;
+      case 4: launch_m<R, 4, MODE>(A, s); break;
+      case 5: launch_m<R, 5, MODE>(A, s); break;
+      case 6: launch_m<R, 6, MODE>(A, s); break;
+      case 7:
+        if constexpr (sizeof(R) != 4) return kUnsupported;
+        else launch_m<R, 7, MODE>(A, s);
+        break;
+      case 8:
+        if constexpr (sizeof(R) != 4) return kUnsupported;
+        else launch_m<R, 8, MODE>(A, s);
+        break;
+      default: return kUnsupported;
+    }
+    return hipGetLastError() == hipSuccess ? kOk : kHipError;
+  }
+}
+
+template int cols_launch<PNFFT_B_REAL, PNFFT_B_MODE>(const BArgs<PNFFT_B_REAL>&, hipStream_t);
+
+}  // namespace pnfft_amd

@@ -111,7 +111,7 @@ class DevicePlan:
         return out
 
     KT_NAMES = ("fft_rows", "fft_cols1", "fft_cols0", "gather_tiled", "gather_direct", "scatter_tiled",
-                "scatter_direct", "binning", "grid_zero")
+                "scatter_direct", "binning", "scatter_fold")
 
     def kernel_times(self):
         """{kernel class: (device ms accumulated, launches)} since the plan was made or reset."""
