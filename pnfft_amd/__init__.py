@@ -38,6 +38,8 @@ PNFFT_FREE_ALL = 0xF
 PNFFT_PRE_TENSOR = 0
 PNFFT_PRE_FULL = 1 << 0
 PNFFT_PRE_PSI = 1 << 1
+PNFFT_PRE_GRAD_PSI = 1 << 2
+PNFFT_PRE_HESSIAN_PSI = 1 << 3
 PNFFT_COMPUTE_F = 1 << 0
 PNFFT_COMPUTE_GRAD_F = 1 << 1
 PNFFT_COMPUTE_HESSIAN_F = 1 << 2

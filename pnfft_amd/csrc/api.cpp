@@ -273,6 +273,15 @@ struct NodesT {
   Cplx<R>* d_slots = nullptr;
   size_t slots_cap = 0;  // complex elements
   int n_torder = 0;
+  int* d_stale = nullptr;            // [M] sorted positions of nodes outside their tile
+  unsigned* d_stale_count = nullptr;
+  // window tables of precompute_psi (sorted order, see precompute_tables)
+  R* d_pre_psi = nullptr;
+  R* d_pre_dpsi = nullptr;
+  R* d_pre_x = nullptr;
+  size_t pre_psi_cap = 0, pre_dpsi_cap = 0, pre_x_cap = 0;
+  bool pre_valid = false, pre_grad = false;
+  GridGeom pre_geom{};
   SortWork sw;
   bool bins_valid = false;
   GridGeom bin_geom{};
@@ -281,11 +290,19 @@ struct NodesT {
 
 template <class R>
 static void tables_compute(PlanT<R>* P) {
+  // window scale 2^-e per axis, e = exponent of psi(0) (the window's peak); the
+  // D factors carry 2^e, so B F D is unchanged (kernel/matrix_D.c:227-280)
+  int e[3];
+  for (int t = 0; t < 3; t++) {
+    const double peak = std::fabs(host::psi(P->win, P->n[t], P->b[t], P->m, 0.0));
+    e[t] = (std::isfinite(peak) && peak > 0.0) ? std::ilogb(peak) : 0;
+    P->geom.ws[t] = std::ldexp(1.0, -e[t]);
+  }
   P->h_tab.resize(P->N[0] + P->N[1] + P->N[2]);
   long off = 0;
   for (int t = 0; t < 3; t++)
     for (long i = 0; i < P->N[t]; i++)
-      P->h_tab[off++] = host::inv_phi_hat(P->win, i - P->N[t] / 2, P->n[t], P->b[t], P->m);
+      P->h_tab[off++] = std::ldexp(host::inv_phi_hat(P->win, i - P->N[t] / 2, P->n[t], P->b[t], P->m), e[t]);
   if (P->d_tab)
     (void)hipMemcpy(P->d_tab, P->h_tab.data(), sizeof(double) * P->h_tab.size(), hipMemcpyHostToDevice);
 }
@@ -437,6 +454,13 @@ static void nodes_release_device(NodesT<R>* D) {
   dfree(D->d_slots);
   D->slots_cap = 0;
   D->n_torder = 0;
+  dfree(D->d_stale);
+  dfree(D->d_stale_count);
+  dfree(D->d_pre_psi);
+  dfree(D->d_pre_dpsi);
+  dfree(D->d_pre_x);
+  D->pre_psi_cap = D->pre_dpsi_cap = D->pre_x_cap = 0;
+  D->pre_valid = false;
   dfree(D->sw.keys[0]);
   dfree(D->sw.keys[1]);
   dfree(D->sw.vals[0]);
@@ -493,13 +517,18 @@ static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
     dfree(D->sw.vals[1]);
     dfree(D->d_perm);
     dfree(D->d_keys);
+    dfree(D->d_stale);
+    dfree(D->d_stale_count);
     D->sw.keys[0] = dalloc<unsigned>(M);
     D->sw.keys[1] = dalloc<unsigned>(M);
     D->sw.vals[0] = dalloc<int>(M);
     D->sw.vals[1] = dalloc<int>(M);
     D->d_perm = dalloc<int>(M);
     D->d_keys = dalloc<unsigned>(M);
-    if (!D->sw.keys[0] || !D->sw.keys[1] || !D->sw.vals[0] || !D->sw.vals[1] || !D->d_perm || !D->d_keys)
+    D->d_stale = dalloc<int>(M);
+    D->d_stale_count = dalloc<unsigned>(1);
+    if (!D->sw.keys[0] || !D->sw.keys[1] || !D->sw.vals[0] || !D->sw.vals[1] || !D->d_perm || !D->d_keys ||
+        !D->d_stale || !D->d_stale_count)
       return false;
     D->sw.cap = M;
   }
@@ -566,7 +595,7 @@ static bool cols_enabled() {
 static int dense_min() {
   static int v = [] {
     const char* e = getenv("PNFFT_AMD_DENSE_MIN");
-    return e ? atoi(e) : 8;
+    return e ? std::max(1, atoi(e)) : 1;
   }();
   return v;
 }
@@ -578,11 +607,15 @@ static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false)
   A.tile_dense = for_scatter ? D->d_dense_s : D->d_dense;
   A.items = for_scatter ? D->d_items_s : D->d_items;
   A.n_items = for_scatter ? D->n_items_s : D->n_items;
+  // the scatter's list holds every nonempty tile; the gather's when dense_min is 1
+  A.all_dense = (for_scatter || dense_min() == 1) ? 1 : 0;
   if (for_scatter && D->n_torder > 0) {
     A.slots = D->d_slots;
     A.tile_slot = D->d_tile_slot;
     A.torder = D->d_torder;
     A.n_torder = D->n_torder;
+    A.stale_count = D->d_stale_count;
+    A.stale_list = D->d_stale;
   }
   for (int t = 0; t < 3; t++) {
     A.ts[t] = D->bin_tiles.ts[t];
@@ -754,6 +787,7 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
       A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
       A.c2r = P->c2r;
       set_tiles(A, D);
+      if (reuse) set_tables(A, P, D);
       A.mid = P->evt.mark(10);
       A.mid2 = P->evt.mark(13);
       int mode = (want_f && want_g) ? B_FGRAD : (want_f ? B_F : B_GRAD);
@@ -835,6 +869,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
         A.grad = dg;
         A.c2r = P->c2r;
         set_tiles(A, D, true);
+        if (reuse) set_tables(A, P, D);
         A.mid = P->evt.mark(10);
         A.mid2 = P->evt.mark(13);
         const int mode = (use_f && use_g) ? B_FGRAD : (use_f ? B_F : B_GRAD);
@@ -901,25 +936,75 @@ static void vjp_impl(PlanT<R>* P, NodesT<R>* D, const Cplx<R>* w, R* gx) {
   A.gx = dgx;
   A.c2r = P->c2r;
   set_tiles(A, D);
+  if (reuse) set_tables(A, P, D);
   if (gather<R>(A, B_VJP, s) != kOk) set_error("gather launch failed");
   if (host_gx) (void)hipMemcpyAsync(gx, dgx, sizeof(R) * 3 * M, hipMemcpyDeviceToHost, s);
   if (hipStreamSynchronize(s) != hipSuccess) set_error("vjp: HIP error %s", hipGetErrorString(hipGetLastError()));
 }
 
-// kernel/ndft-parallel.c:1144-1250: here the node precomputation is the
-// binning of the nodes (sort permutation and tile bounds) for the plan's grid.
+static bool same_full_geom(const GridGeom& a, const GridGeom& b) {
+  for (int t = 0; t < 3; t++)
+    if (a.n[t] != b.n[t] || a.b[t] != b.b[t] || a.ws[t] != b.ws[t]) return false;
+  return a.m == b.m && a.win == b.win;
+}
+
+template <class T>
+static T* grow(T*& p, size_t& cap, size_t need) {
+  if (cap < need) {
+    dfree(p);
+    cap = 0;
+    p = dalloc<T>(need);
+    if (p) cap = need;
+    (void)hipGetLastError();
+  }
+  return p;
+}
+
+// kernel/ndft-parallel.c:1144-1245: bins the nodes and, for PNFFT_PRE_PSI,
+// tabulates their window factors (PNFFT_PRE_GRAD_PSI: also the derivatives) in
+// the bin order; set_x discards both.  Without device memory for the tables the
+// B kernels evaluate the windows as without PRE_PSI.
 template <class R>
 static void precompute_impl(PlanT<R>* P, NodesT<R>* D, unsigned flags) {
   if (!P || !D) return;
   D->precompute_flags = flags;
+  D->pre_valid = false;
   if (!(flags & fl::PRE_PSI) || D->M == 0) return;
   bool host_x = false;
   R* dx = stage_in<R>(D->x, D->d_x, 3 * D->M, true, P->stream, &host_x);
   if (!dx) return;
   if (bin(P, D, dx) != kOk) return;
+  if (!host_x && P->m >= 2 && P->m <= 8) {
+    const size_t M = (size_t)D->M, cm = 2 * (size_t)P->m + 1;
+    const bool want_grad = (flags & fl::PRE_GRAD_PSI) != 0;
+    R* psi = grow(D->d_pre_psi, D->pre_psi_cap, 3 * cm * M);
+    R* dpsi = want_grad ? grow(D->d_pre_dpsi, D->pre_dpsi_cap, 3 * cm * M) : nullptr;
+    R* xo = grow(D->d_pre_x, D->pre_x_cap, 3 * M);
+    if (psi && xo && (dpsi || !want_grad)) {
+      BArgs<R> A{};
+      A.g = P->geom;
+      A.M = D->M;
+      A.x = dx;
+      A.perm = D->d_perm;
+      if (precompute_tables<R>(A, psi, dpsi, xo, P->stream) == kOk) {
+        D->pre_valid = true;
+        D->pre_grad = dpsi != nullptr;
+        D->pre_geom = P->geom;
+      }
+    }
+  }
   (void)hipStreamSynchronize(P->stream);
   // host x is re-uploaded on every call, so its bins are not reused
   if (host_x) D->bins_valid = false;
+}
+
+// window tables for a B stage that reuses the precomputed bins
+template <class R>
+static void set_tables(BArgs<R>& A, const PlanT<R>* P, const NodesT<R>* D) {
+  if (!D->pre_valid || !same_full_geom(D->pre_geom, P->geom)) return;
+  A.pre_psi = D->d_pre_psi;
+  A.pre_dpsi = D->pre_grad ? D->d_pre_dpsi : nullptr;
+  A.pre_x = D->d_pre_x;
 }
 
 template <class R>
@@ -1060,7 +1145,7 @@ using namespace pnfft_amd;
   void PREFIX##set_x(R* x, void* h) {                                                                       \
     if (!h) return;                                                                                        \
     auto* D = static_cast<NodesT<R>*>(h);                                                                   \
-    D->x = x; D->bins_valid = false;                                                                        \
+    D->x = x; D->bins_valid = false; D->pre_valid = false;                                                  \
   }                                                                                                        \
   void PREFIX##set_f_hat(void* fh, void* p) { if (p) static_cast<PlanT<R>*>(p)->f_hat = fh; }               \
   void PREFIX##set_f_hat_real(R* fh, void* p) { if (p) static_cast<PlanT<R>*>(p)->f_hat = fh; }             \

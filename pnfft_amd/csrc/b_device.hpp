@@ -28,26 +28,77 @@ struct NodeWin {
   R dx[CM], dy[CM], dz[CM];
 };
 
-// Window values of one node.  MT > 0: the width CM = 2 MT + 1 is a compile-time
-// constant and every array index is static, so NodeWin lives in registers.
-// MT == 0 (generic m): runtime width m, arrays of kMaxC entries.
+// Multiply the window values of each axis by the plan's power-of-two scale
+// (GridGeom::ws; exact), c <= CM entries.
+template <class R, int CM, bool WG>
+__device__ inline void scale_windows(const GridGeom& g, int c, NodeWin<R, CM>& w) {
+  const R s0 = (R)g.ws[0], s1 = (R)g.ws[1], s2 = (R)g.ws[2];
+#pragma unroll
+  for (int k = 0; k < CM; k++) {
+    if (k < c) {
+      w.px[k] *= s0;
+      w.py[k] *= s1;
+      w.pz[k] *= s2;
+      if (WG) {
+        w.dx[k] *= s0;
+        w.dy[k] *= s1;
+        w.dz[k] *= s2;
+      }
+    }
+  }
+}
+
+// Window values of one node (node id j, position p in the sorted order).
+// MT > 0: the width CM = 2 MT + 1 is a compile-time constant and every array
+// index is static, so NodeWin lives in registers; the values come from the
+// precomputed tables (pnfft_precompute_psi) when they exist (with the
+// derivatives, if WG) and the node's coordinates still equal the ones they were
+// computed from.  MT == 0 (generic
+// m): runtime width m, arrays of kMaxC entries, always evaluated.
 template <class R, int MT, int CM, bool WG>
-__device__ inline void node_windows(const BArgs<R>& A, long j, int m, NodeWin<R, CM>& w) {
+__device__ inline void node_windows(const BArgs<R>& A, long p, long j, int m, NodeWin<R, CM>& w) {
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
-  const NodeAxis<R> a0 = node_axis<R>(A.x[3 * j + 0], n0, m);
-  const NodeAxis<R> a1 = node_axis<R>(A.x[3 * j + 1], n1, m);
-  const NodeAxis<R> a2 = node_axis<R>(A.x[3 * j + 2], n2, m);
+  const R x0 = A.x[3 * j + 0], x1 = A.x[3 * j + 1], x2 = A.x[3 * j + 2];
+  const NodeAxis<R> a0 = node_axis<R>(x0, n0, m);
+  const NodeAxis<R> a1 = node_axis<R>(x1, n1, m);
+  const NodeAxis<R> a2 = node_axis<R>(x2, n2, m);
   w.start[0] = a0.start;
   w.start[1] = a1.start;
   w.start[2] = a2.start;
   if constexpr (MT > 0) {
-    window_axis_s<R, CM, WG>(A.g.win, a0.frac, (R)n0, (R)A.g.b[0], w.px, w.dx);
-    window_axis_s<R, CM, WG>(A.g.win, a1.frac, (R)n1, (R)A.g.b[1], w.py, w.dy);
-    window_axis_s<R, CM, WG>(A.g.win, a2.frac, (R)n2, (R)A.g.b[2], w.pz, w.dz);
+    // table values are loaded before the coordinate check, so that both round
+    // trips overlap; they are kept when x is unchanged since the tables were made
+    bool have = false;
+    if (A.pre_psi && p >= 0) {
+      const long M = A.M;
+#pragma unroll
+      for (int s = 0; s < CM; s++) {
+        w.px[s] = A.pre_psi[(long)s * M + p];
+        w.py[s] = A.pre_psi[(long)(CM + s) * M + p];
+        w.pz[s] = A.pre_psi[(long)(2 * CM + s) * M + p];
+      }
+      if (WG && A.pre_dpsi) {
+#pragma unroll
+        for (int s = 0; s < CM; s++) {
+          w.dx[s] = A.pre_dpsi[(long)s * M + p];
+          w.dy[s] = A.pre_dpsi[(long)(CM + s) * M + p];
+          w.dz[s] = A.pre_dpsi[(long)(2 * CM + s) * M + p];
+        }
+      }
+      const bool same = x0 == A.pre_x[p] && x1 == A.pre_x[M + p] && x2 == A.pre_x[2 * M + p];
+      have = same && (!WG || A.pre_dpsi != nullptr);
+    }
+    if (!have) {
+      window_axis_s<R, CM, WG>(A.g.win, a0.frac, (R)n0, (R)A.g.b[0], w.px, w.dx);
+      window_axis_s<R, CM, WG>(A.g.win, a1.frac, (R)n1, (R)A.g.b[1], w.py, w.dy);
+      window_axis_s<R, CM, WG>(A.g.win, a2.frac, (R)n2, (R)A.g.b[2], w.pz, w.dz);
+      scale_windows<R, CM, WG>(A.g, CM, w);
+    }
   } else {
     window_axis<R>(A.g.win, m, a0.frac, (R)n0, (R)A.g.b[0], w.px, WG ? w.dx : nullptr);
     window_axis<R>(A.g.win, m, a1.frac, (R)n1, (R)A.g.b[1], w.py, WG ? w.dy : nullptr);
     window_axis<R>(A.g.win, m, a2.frac, (R)n2, (R)A.g.b[2], w.pz, WG ? w.dz : nullptr);
+    scale_windows<R, CM, WG>(A.g, 2 * m + 1, w);
   }
 }
 

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kB) void gather_direct_k(BArgs<R> A) {
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
   NodeWin<R, CM> w;
   constexpr bool ST = static_body<R, MT>();
-  node_windows<R, ST ? MT : 0, CM, WG>(A, j, m, w);
+  node_windows<R, ST ? MT : 0, CM, WG>(A, A.perm ? p : -1, j, m, w);
   gather_node_global<R, CM, MODE, ST>(A, j, w, 2 * m + 1);
 }
 
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kB) void scatter_direct_k(BArgs<R> A) {
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
   NodeWin<R, CM> w;
   constexpr bool ST = static_body<R, MT>();
-  node_windows<R, ST ? MT : 0, CM, WG>(A, j, m, w);
+  node_windows<R, ST ? MT : 0, CM, WG>(A, A.perm ? p : -1, j, m, w);
   scatter_node_global<R, CM, MODE, ST>(A, j, w, 2 * m + 1);
 }
 
@@ -110,8 +110,16 @@ __device__ inline SubgridPitch subgrid_pitch(const TileItem& it) {
   return s;
 }
 
+// Waves per SIMD the FP32 gather kernels are compiled for (register budget
+// 512 / w): 4 for f, 3 for the gradient modes; FP64 keeps the compiler's
+// choice (a budget would make it spill).
+template <class R, int MODE>
+constexpr int gather_waves() {
+  return sizeof(R) == 4 ? (MODE == B_F ? 4 : 3) : 1;
+}
+
 template <class R, int MT, int MODE>
-__global__ __launch_bounds__(kTileThreads) void gather_tiled_k(BArgs<R> A) {
+__global__ __launch_bounds__(kTileThreads, (gather_waves<R, MODE>())) void gather_tiled_k(BArgs<R> A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<R>* tile = reinterpret_cast<Cplx<R>*>(smem_raw);
   constexpr int m = MT;
@@ -132,7 +140,7 @@ __global__ __launch_bounds__(kTileThreads) void gather_tiled_k(BArgs<R> A) {
   for (int p = it.beg + threadIdx.x; p < it.end; p += kTileThreads) {
     const long j = A.perm[p];
     NodeWin<R, CM> w;
-    node_windows<R, ST ? MT : 0, CM, WG>(A, j, m, w);
+    node_windows<R, ST ? MT : 0, CM, WG>(A, A.perm ? p : -1, j, m, w);
     const int l0 = local_off(w.start[0], it.o[0], A.ts[0], n0);
     const int l1 = local_off(w.start[1], it.o[1], A.ts[1], n1);
     const int l2 = local_off(w.start[2], it.o[2], A.ts[2], n2);
@@ -177,7 +185,7 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
     const int p = it.beg + q;
     const long j = A.perm[p];
     NodeWin<R, CM> w;
-    node_windows<R, ST ? MT : 0, CM, WG>(A, j, m, w);
+    node_windows<R, ST ? MT : 0, CM, WG>(A, A.perm ? p : -1, j, m, w);
     const int l0 = local_off(w.start[0], it.o[0], A.ts[0], n0);
     const int l1 = local_off(w.start[1], it.o[1], A.ts[1], n1);
     const int l2 = local_off(w.start[2], it.o[2], A.ts[2], n2);
@@ -273,7 +281,7 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
       }                                                                                                   \
       if (A.mid) (void)hipEventRecord(A.mid, s);                                                          \
       if (A.mid2) (void)hipEventRecord(A.mid2, s);                                                        \
-      hipLaunchKernelGGL((gather_direct_k<R, MV, MODE>), grid, block, 0, s, D);                           \
+      if (!(tiled && A.all_dense)) hipLaunchKernelGGL((gather_direct_k<R, MV, MODE>), grid, block, 0, s, D);  \
     } else {                                                                                              \
       if (tiled) {                                                                                        \
         (void)hipFuncSetAttribute((const void*)scatter_tiled_k<R, MV, SMODE>,                             \
@@ -282,7 +290,7 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
       }                                                                                                   \
       if (A.mid) (void)hipEventRecord(A.mid, s);                                                          \
       if (A.mid2) (void)hipEventRecord(A.mid2, s);                                                        \
-      hipLaunchKernelGGL((scatter_direct_k<R, MV, SMODE>), grid, block, 0, s, D);                         \
+      if (!(tiled && A.all_dense)) hipLaunchKernelGGL((scatter_direct_k<R, MV, SMODE>), grid, block, 0, s, D); \
     }                                                                                                     \
     break;
     PNFFT_B_CASE(2)

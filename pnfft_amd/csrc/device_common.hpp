@@ -37,6 +37,10 @@ struct GridGeom {
   int m;         // window cutoff parameter (stencil 2m+1)
   int win;       // WindowId
   double b[3];   // window shape parameter per axis
+  // power-of-two factor applied to the window values of each axis in the B
+  // kernels (2^-e with e the exponent of the window's peak) and undone in the
+  // D factors, so that FP32 intermediates stay near 1; exact in both precisions
+  double ws[3];
 };
 
 // Launch helpers report errors through this code (0 = ok).

@@ -32,16 +32,24 @@ struct BArgs {
   int n_items;
   int ts[3];               // tile size per axis
   int ntile[3];            // tiles per axis
+  int all_dense;           // every nonempty tile is a work item: the direct kernel has no nodes to take
   // column scatter (scatter_cols.hip): when slots is set, scatter() runs it instead of the LDS-atomic path
   Cplx<R>* slots;          // [n_torder][P^3] padded subgrid of every nonempty tile
   const int* tile_slot;    // [ntiles] slot of each tile, -1 if empty
   const int* torder;       // [n_torder] nonempty tiles, most nodes first
   int n_torder;
+  // window tables of pnfft_precompute_psi in the sorted order, [3 CM][M] each (or null)
+  const R* pre_psi;
+  const R* pre_dpsi;
+  const R* pre_x;          // [3][M] coordinates the tables were computed from
+  unsigned* stale_count;   // nodes found outside the tile they are binned to (bins older than x) ...
+  int* stale_list;         // ... and their sorted positions [M]
   hipEvent_t mid;          // host only: recorded between the first and the second kernel of the stage
   hipEvent_t mid2;         // host only: recorded between the second and the third kernel of the stage
 };
 
 constexpr int kTileThreads = 512;  // threads per block of the tiled kernels
+constexpr int kTileNodes = 512;    // nodes per work item
 
 // Column scatter: applies for m = 2..8 (FP64: m <= 6) when all three tile
 // sizes equal the layout's TS (8 for m <= 6, 4 above); P = TS + 2m cells per
@@ -56,11 +64,10 @@ int build_tile_order(const int* tile_start, long ntiles, unsigned* work, int* or
 
 // Launch of the column scatter for MODE (B_F, B_GRAD, B_FGRAD): column kernel
 // into the slots, fold of the slots into the grid (which it overwrites), then
-// the nodes whose bins are older than their coordinates through the direct
-// path (global atomics).
+// the nodes the column kernel listed as outside their tile (bins older than
+// their coordinates) through the direct path (global atomics).
 template <class R, int MODE>
 int cols_launch(const BArgs<R>& A, hipStream_t s);
-constexpr int kTileNodes = 512;    // nodes per work item
 
 // Work items of the tiled kernels: every tile with at least dense_min nodes is
 // cut into chunks of kTileNodes sorted nodes.  Writes tile_dense[ntiles],
@@ -109,6 +116,12 @@ int scatter(const BArgs<R>& A, int mode, hipStream_t s);
 
 template <class R>
 int fill_zero(Cplx<R>* p, size_t n, hipStream_t s);
+
+// Window tables for pnfft_precompute_psi: for sorted position p (node perm[p]),
+// psi[(t CM + s) M + p] = psi_t(s) (and dpsi likewise if non-null), x_out[t M + p]
+// = x_t.  Applies for m = 2..8 (FP64: m <= 6); returns kUnsupported otherwise.
+template <class R>
+int precompute_tables(const BArgs<R>& A, R* psi, R* dpsi, R* x_out, hipStream_t s);
 
 // Number of sort passes for keys below key_limit.
 int sort_passes(unsigned long long key_limit);

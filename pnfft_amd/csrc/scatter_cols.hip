@@ -22,7 +22,8 @@
 // sums, for every grid cell, the slots of the tiles whose subgrids cover it in
 // a fixed order and writes the cell.  Accumulation and fold use no atomics, so
 // the adjoint is deterministic; only nodes whose bins are older than their
-// coordinates go through the direct path (global atomics) afterwards.
+// coordinates (listed by the column kernel) go through the direct path
+// (global atomics) afterwards.
 #include "b_device.hpp"
 
 namespace pnfft_amd {
@@ -75,6 +76,48 @@ __device__ __attribute__((always_inline)) inline void plane_blocks(int l0, doubl
   }
 }
 
+// Weights of node i for the thread's rows r (b, b + H): w[r] = (re, im) of the
+// f term and of the d/dx term, zero outside the stencil.  Returns false when
+// the thread's column is outside the node's stencil (or the node is outside
+// the tile, loc < 0).
+template <class R, int NC, int CM, bool WF, bool WG, int NB, int NBG>
+__device__ __attribute__((always_inline)) inline bool node_weights(int loc, int i, int b, int c, int H, bool own,
+                                                                   const R (&s_py)[NB][CM], const R (&s_pz)[NB][CM],
+                                                                   const R (&s_dy)[NBG][CM], const R (&s_dz)[NBG][CM],
+                                                                   const Cplx<R> (&s_f)[WF ? NB : 1],
+                                                                   const Cplx<R> (&s_g)[NBG][3], double (&w)[NC][4]) {
+  if (loc < 0) return false;
+  const int l1 = (loc >> 8) & 255, l2 = (loc >> 16) & 255;
+  const int dc = c - l2;
+  if (!own || (unsigned)dc >= (unsigned)CM) return false;
+  const R pz = s_pz[i][dc];
+  const R dz = WG ? s_dz[i][dc] : R(0);
+#pragma unroll
+  for (int r = 0; r < NC; r++) {
+    const int db = b + r * H - l1;
+    const bool in_b = (unsigned)db < (unsigned)CM;
+    const int dbc = in_b ? db : 0;
+    const R py = in_b ? s_py[i][dbc] : R(0);
+    w[r][0] = w[r][1] = w[r][2] = w[r][3] = 0.0;
+    if (WF) {
+      const Cplx<R> f = s_f[i];
+      const R q = py * pz;
+      w[r][0] = (double)(q * f.x);
+      w[r][1] = (double)(q * f.y);
+    }
+    if (WG) {
+      const R dy = in_b ? s_dy[i][dbc] : R(0);
+      const Cplx<R> g0 = s_g[i][0], g1 = s_g[i][1], g2 = s_g[i][2];
+      const R q0 = py * pz, q1 = dy * pz, q2 = py * dz;
+      w[r][2] = (double)(q0 * g0.x);
+      w[r][3] = (double)(q0 * g0.y);
+      w[r][0] += (double)(q1 * g1.x + q2 * g2.x);
+      w[r][1] += (double)(q1 * g1.y + q2 * g2.y);
+    }
+  }
+  return true;
+}
+
 template <class R, int MT, int MODE>
 __global__ __launch_bounds__((ColsShape<R, MT, MODE>::NT)) void scatter_cols_k(BArgs<R> A) {
   using S = ColsShape<R, MT, MODE>;
@@ -112,12 +155,13 @@ __global__ __launch_bounds__((ColsShape<R, MT, MODE>::NT)) void scatter_cols_k(B
     if (tid < nb) {
       const long j = A.perm[base + tid];
       NodeWin<R, CM> w;
-      node_windows<R, MT, CM, WG>(A, j, MT, w);
+      node_windows<R, MT, CM, WG>(A, base + tid, j, MT, w);
       const int l0 = local_off(w.start[0], o0, TS, n0);
       const int l1 = local_off(w.start[1], o1, TS, n1);
       const int l2 = local_off(w.start[2], o2, TS, n2);
       const bool in = l0 >= 0 && l1 >= 0 && l2 >= 0;
       s_loc[tid] = in ? (l0 | (l1 << 8) | (l2 << 16)) : -1;
+      if (!in) A.stale_list[atomicAdd(A.stale_count, 1u)] = base + tid;
 #pragma unroll
       for (int a = 0; a < P; a++) {
         s_px[tid][a] = 0.0;
@@ -143,42 +187,29 @@ __global__ __launch_bounds__((ColsShape<R, MT, MODE>::NT)) void scatter_cols_k(B
       }
     }
     __syncthreads();
-    int loc_next = s_loc[0];
-    for (int i = 0; i < nb; i++) {
-      const int loc = __builtin_amdgcn_readfirstlane(loc_next);
-      if (i + 1 < nb) loc_next = s_loc[i + 1];
-      if (loc < 0) continue;  // stencil start outside the tile: the stale pass takes the node
-      const int l0 = loc & 255, l1 = (loc >> 8) & 255, l2 = (loc >> 16) & 255;
-      const int dc = c - l2;
-      if (!own || (unsigned)dc >= (unsigned)CM) continue;
-      // per row r (b, b + H): weights of the f and d/dx terms (re, im), zero outside the stencil
-      double w[NC][4];
-      const R pz = s_pz[i][dc];
-      const R dz = WG ? s_dz[i][dc] : R(0);
-#pragma unroll
-      for (int r = 0; r < NC; r++) {
-        const int db = b + r * H - l1;
-        const bool in_b = (unsigned)db < (unsigned)CM;
-        const int dbc = in_b ? db : 0;
-        const R py = in_b ? s_py[i][dbc] : R(0);
-        w[r][0] = w[r][1] = w[r][2] = w[r][3] = 0.0;
-        if (WF) {
-          const Cplx<R> f = s_f[i];
-          const R q = py * pz;
-          w[r][0] = (double)(q * f.x);
-          w[r][1] = (double)(q * f.y);
-        }
-        if (WG) {
-          const R dy = in_b ? s_dy[i][dbc] : R(0);
-          const Cplx<R> g0 = s_g[i][0], g1 = s_g[i][1], g2 = s_g[i][2];
-          const R q0 = py * pz, q1 = dy * pz, q2 = py * dz;
-          w[r][2] = (double)(q0 * g0.x);
-          w[r][3] = (double)(q0 * g0.y);
-          w[r][0] += (double)(q1 * g1.x + q2 * g2.x);
-          w[r][1] += (double)(q1 * g1.y + q2 * g2.y);
-        }
+    if constexpr (sizeof(R) == 4) {
+      // two nodes per iteration: both nodes' LDS loads are issued before either
+      // node's accumulation, so their latencies overlap
+      for (int i = 0; i < nb; i += 2) {
+        const bool two = i + 1 < nb;
+        const int locA = __builtin_amdgcn_readfirstlane(s_loc[i]);
+        const int locB = two ? __builtin_amdgcn_readfirstlane(s_loc[i + 1]) : -1;
+        double wA[NC][4], wB[NC][4];
+        const bool onA =
+            node_weights<R, NC, CM, WF, WG>(locA, i, b, c, H, own, s_py, s_pz, s_dy, s_dz, s_f, s_g, wA);
+        const bool onB =
+            node_weights<R, NC, CM, WF, WG>(locB, i + 1, b, c, H, own, s_py, s_pz, s_dy, s_dz, s_f, s_g, wB);
+        if (onA) plane_blocks<NC, P, CM, WG, 0>(locA & 255, ar, ai, &s_px[i][0], WG ? &s_dx[i][0] : nullptr, wA);
+        if (onB)
+          plane_blocks<NC, P, CM, WG, 0>(locB & 255, ar, ai, &s_px[i + 1][0], WG ? &s_dx[i + 1][0] : nullptr, wB);
       }
-      plane_blocks<NC, P, CM, WG, 0>(l0, ar, ai, &s_px[i][0], WG ? &s_dx[i][0] : nullptr, w);
+    } else {  // FP64: one node per iteration (two would exceed 256 VGPRs)
+      for (int i = 0; i < nb; i++) {
+        const int loc = __builtin_amdgcn_readfirstlane(s_loc[i]);
+        double w[NC][4];
+        if (node_weights<R, NC, CM, WF, WG>(loc, i, b, c, H, own, s_py, s_pz, s_dy, s_dz, s_f, s_g, w))
+          plane_blocks<NC, P, CM, WG, 0>(loc & 255, ar, ai, &s_px[i][0], WG ? &s_dx[i][0] : nullptr, w);
+      }
     }
     __syncthreads();
   }
@@ -232,24 +263,24 @@ __global__ __launch_bounds__(TS* TS* TS) void fold_k(BArgs<R> A) {
 }
 
 // nodes whose stencil start left the tile they were binned to (bins older
-// than the coordinates): added to the folded grid through global atomics
+// than the coordinates), listed by the column kernel: added to the folded grid
+// through global atomics
 template <class R, int MT, int MODE>
 __global__ __launch_bounds__(256) void scatter_stale_k(BArgs<R> A) {
   using S = ColsShape<R, MT, MODE>;
   constexpr int CM = S::CM;
-  const long p = (long)blockIdx.x * 256 + threadIdx.x;
-  if (p >= A.M) return;
-  const long j = A.perm[p];
-  unsigned long long tile = 0;
-  for (int t = 0; t < 3; t++) {
-    const NodeAxis<R> a = node_axis<R>(A.x[3 * j + t], A.g.n[t], MT);
-    tile = tile * (unsigned long long)A.ntile[t] + (unsigned long long)(a.start / A.ts[t]);
+  const unsigned count = *A.stale_count;
+  for (unsigned k = blockIdx.x * 256 + threadIdx.x; k < count; k += gridDim.x * 256) {
+    const int p = A.stale_list[k];
+    const long j = A.perm[p];
+    NodeWin<R, CM> w;
+    node_windows<R, MT, CM, S::WG>(A, p, j, MT, w);
+    scatter_node_global<R, CM, MODE, true>(A, j, w, CM);
   }
-  const unsigned vol = (unsigned)(A.ts[0] * A.ts[1] * A.ts[2]);
-  if (tile == (unsigned long long)(A.keys[p] / vol)) return;
-  NodeWin<R, CM> w;
-  node_windows<R, MT, CM, S::WG>(A, j, MT, w);
-  scatter_node_global<R, CM, MODE, true>(A, j, w, CM);
+}
+
+__global__ void zero_count_k(unsigned* c) {
+  if (threadIdx.x == 0) *c = 0u;
 }
 
 inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -257,19 +288,20 @@ inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
 template <class R, int MT, int MODE>
 void launch_m(const BArgs<R>& A, hipStream_t s) {
   using S = ColsShape<R, MT, MODE>;
+  hipLaunchKernelGGL(zero_count_k, dim3(1), dim3(64), 0, s, A.stale_count);
   hipLaunchKernelGGL((scatter_cols_k<R, MT, MODE>), dim3((unsigned)A.n_torder), dim3(S::NT), 0, s, A);
   if (A.mid) (void)hipEventRecord(A.mid, s);
   const long ntiles = (long)A.ntile[0] * A.ntile[1] * A.ntile[2];
   hipLaunchKernelGGL((fold_k<R, S::TS, S::P>), dim3((unsigned)ntiles), dim3(S::TS * S::TS * S::TS), 0, s, A);
   if (A.mid2) (void)hipEventRecord(A.mid2, s);
-  hipLaunchKernelGGL((scatter_stale_k<R, MT, MODE>), dim3(grid1d(A.M, 256)), dim3(256), 0, s, A);
+  hipLaunchKernelGGL((scatter_stale_k<R, MT, MODE>), dim3(256), dim3(256), 0, s, A);
 }
 
 }  // namespace
 
 template <class R, int MODE>
 int cols_launch(const BArgs<R>& A, hipStream_t s) {
-  if (A.M == 0 || A.n_torder <= 0) return kUnsupported;
+  if (A.M == 0 || A.n_torder <= 0 || !A.stale_count || !A.stale_list) return kUnsupported;
   // FP64 gradient modes would need more than 256 VGPRs per lane here: they keep the LDS-atomic path
   if constexpr (sizeof(R) == 8 && MODE != B_F) {
     return kUnsupported;

@@ -335,3 +335,64 @@ def test_adjoint_is_deterministic(gpu, precision):
     dp.close()
     assert err is None, err
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_precomputed_tables_equal_evaluated_windows(gpu, oracle, precision):
+    """precompute_psi(PRE_PSI | PRE_GRAD_PSI) tables give the same trafo and grad bit for bit as
+    evaluating the windows in the kernels (same code computes the tables), and the same adj up to
+    summation order (the FP64 f+grad adjoint accumulates with atomics); both match the oracle."""
+    w = make(1, M=3000)
+    say("case precompute tables", precision)
+    x = w["x"] if precision == "double" else w["x"].astype(np.float32).astype(np.float64)
+    outs = []
+    for pre in (0, PA.PNFFT_PRE_PSI | PA.PNFFT_PRE_GRAD_PSI):
+        dp = DevicePlan(w["N"], w["n"], w["m"], 0, w["M"], precision=precision)
+        dp.set_x(x)
+        dp.set_f_hat(w["f_hat"])
+        if pre:
+            dp.precompute(pre)
+        dp.trafo(PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_GRAD_F)
+        f = dp.f.cpu().numpy().copy()
+        g = dp.grad.cpu().numpy().copy()
+        dp.set_f(w["f"])
+        dp.set_grad(g)
+        dp.adj(PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_GRAD_F)
+        fh = dp.f_hat.cpu().numpy().copy()
+        err = dp.api.last_error()
+        dp.close()
+        assert err is None, err
+        outs.append((f, g, fh))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert rel(outs[1][2], outs[0][2]) < (1e-14 if precision == "double" else 1e-7)
+    tol = 1e-12 if precision == "double" else 1e-5
+    P = oracle.Plan(w["N"], w["n"], w["m"], 0)
+    f_o, g_o = P.trafo(x, w["f_hat"], compute_grad=True)
+    assert rel(outs[1][0], f_o) < tol
+    assert rel(outs[1][1], g_o) < tol
+    assert rel(outs[1][2], P.adj(x, f=w["f"], grad=outs[1][1])) < tol
+
+
+def test_fp32_adjoint_of_large_gradients(gpu, oracle):
+    """FP32 adjoint of grad_f values from trafo (|g| ~ 1e4) with the Kaiser-Bessel window at m = 6,
+    whose unscaled values reach 4e10 per axis: the kernels' power-of-two window scaling keeps the
+    FP32 intermediates finite."""
+    w = make(1, M=3000)
+    say("case fp32 large gradients")
+    x = w["x"].astype(np.float32).astype(np.float64)
+    dp = DevicePlan(w["N"], w["n"], w["m"], 0, w["M"], precision="float")
+    dp.set_x(x)
+    dp.set_f_hat(w["f_hat"])
+    dp.trafo(PA.PNFFT_COMPUTE_GRAD_F)
+    g = dp.grad.cpu().numpy().copy()
+    dp.set_f(w["f"])
+    dp.adj(PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_GRAD_F)
+    fh = dp.f_hat.cpu().numpy().copy()
+    err = dp.api.last_error()
+    dp.close()
+    assert err is None, err
+    assert np.abs(g).max() > 1e3
+    P = oracle.Plan(w["N"], w["n"], w["m"], 0)
+    assert np.isfinite(fh).all()
+    assert rel(fh, P.adj(x, f=w["f"], grad=g)) < 1e-5
