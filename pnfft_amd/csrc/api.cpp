@@ -581,15 +581,15 @@ static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
   return true;
 }
 
-// Scatter path: the column scatter (default) or the LDS-atomic kernels
-// (PNFFT_AMD_SCATTER=atomic, for comparison).
-static bool cols_enabled() {
-  static bool v = [] {
-    const char* e = getenv("PNFFT_AMD_SCATTER");
-    return !(e && strcmp(e, "atomic") == 0);
-  }();
-  return v;
+// Scatter path: the column scatter (default; matrix-core kernel where it
+// applies) or, for comparison, the LDS-atomic kernels (PNFFT_AMD_SCATTER=atomic)
+// or the vector column kernel everywhere (PNFFT_AMD_SCATTER=cols).
+static const char* scatter_env() {
+  static const char* e = getenv("PNFFT_AMD_SCATTER");
+  return e ? e : "";
 }
+static bool cols_enabled() { return strcmp(scatter_env(), "atomic") != 0; }
+static bool cols_vector_only() { return strcmp(scatter_env(), "cols") == 0; }
 
 // Minimum node count for a tile to take the LDS path (tunable by PNFFT_AMD_DENSE_MIN).
 static int dense_min() {
@@ -616,6 +616,7 @@ static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false)
     A.n_torder = D->n_torder;
     A.stale_count = D->d_stale_count;
     A.stale_list = D->d_stale;
+    A.scatter_vec = cols_vector_only() ? 1 : 0;
   }
   for (int t = 0; t < 3; t++) {
     A.ts[t] = D->bin_tiles.ts[t];

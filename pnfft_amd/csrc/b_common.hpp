@@ -13,15 +13,18 @@ struct NodeAxis {
   R frac;     // n x - floor(n x)
 };
 
+// n is a power of two (plans reject other oversampled lengths): floor(n x) mod n
+// is fl - n floor(fl / n), exact in FP64 for |n x| < 2^53 (the scaling by 1/n is
+// an exponent shift), and the shift by -m wraps with a mask.
 template <class R>
 __device__ inline NodeAxis<R> node_axis(R xv, int n, int m) {
   const double nx = (double)n * (double)xv;
   const double fl = floor(nx);
   NodeAxis<R> a;
   a.frac = (R)(nx - fl);
-  long st = ((long)fl - m) % n;
-  if (st < 0) st += n;
-  a.start = (int)st;
+  const double q = floor(ldexp(fl, -__builtin_ctz((unsigned)n)));
+  const int r = (int)fma(-q, (double)n, fl);
+  a.start = (r - m) & (n - 1);
   return a;
 }
 

@@ -110,6 +110,43 @@ __device__ inline SubgridPitch subgrid_pitch(const TileItem& it) {
   return s;
 }
 
+// Copy of a tile's padded subgrid (P^3 cells from the tile origin, periodic
+// wrap) into LDS, for the standard layout (tile size TS on every axis, so P =
+// TS + 2m is a compile-time constant).  Each thread moves pairs of cells along
+// axis 2: the origin is a multiple of TS and n2 is a power of two, so a pair
+// starting at an even offset never straddles the wrap and is one aligned load.
+template <class R, int P>
+__device__ inline void stage_subgrid_static(const BArgs<R>& A, const TileItem& it, Cplx<R>* tile) {
+  static_assert(P % 2 == 0, "P = TS + 2m is even");
+  constexpr int H = P / 2, RP = P + 1, PP = P * RP + 1, NPAIR = P * P * H;
+  constexpr int K = (NPAIR + kTileThreads - 1) / kTileThreads;
+  using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;  // two cells
+  const unsigned n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
+  const V* src = reinterpret_cast<const V*>(A.grid);
+  // all loads first, then the LDS stores: the K round trips overlap
+  V v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int q = threadIdx.x + k * kTileThreads;
+    if (q < NPAIR) {
+      const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
+      const unsigned g0 = (unsigned)(it.o[0] + cx) & (n0 - 1), g1 = (unsigned)(it.o[1] + cy) & (n1 - 1);
+      const unsigned g2 = (unsigned)(it.o[2] + 2 * hz) & (n2 - 1);
+      v[k] = src[((g0 * n1 + g1) * n2 + g2) >> 1];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int q = threadIdx.x + k * kTileThreads;
+    if (q < NPAIR) {
+      const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
+      Cplx<R>* d = tile + cx * PP + cy * RP + 2 * hz;
+      d[0] = cmk<R>(v[k].x, v[k].y);
+      d[1] = cmk<R>(v[k].z, v[k].w);
+    }
+  }
+}
+
 // Waves per SIMD the FP32 gather kernels are compiled for (register budget
 // 512 / w): 4 for f, 3 for the gradient modes; FP64 keeps the compiler's
 // choice (a budget would make it spill).
@@ -131,10 +168,15 @@ __global__ __launch_bounds__(kTileThreads, (gather_waves<R, MODE>())) void gathe
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
   const int P1 = it.P[1], P2 = it.P[2];
   const int vol = it.P[0] * P1 * P2;
-  for (int idx = threadIdx.x; idx < vol; idx += kTileThreads) {
-    const int cz = idx % P2, cy = (idx / P2) % P1, cx = idx / (P1 * P2);
-    const int g0 = wrapi(it.o[0] + cx, n0), g1 = wrapi(it.o[1] + cy, n1), g2 = wrapi(it.o[2] + cz, n2);
-    tile[cx * sp.PP + cy * sp.RP + cz] = A.grid[((long)g0 * n1 + g1) * n2 + g2];
+  constexpr int TSL = MT <= 6 ? 8 : 4;  // tile size of the standard layout
+  if (ST && A.ts[0] == TSL && A.ts[1] == TSL && A.ts[2] == TSL) {
+    stage_subgrid_static<R, TSL + 2 * MT>(A, it, tile);
+  } else {
+    for (int idx = threadIdx.x; idx < vol; idx += kTileThreads) {
+      const int cz = idx % P2, cy = (idx / P2) % P1, cx = idx / (P1 * P2);
+      const int g0 = wrapi(it.o[0] + cx, n0), g1 = wrapi(it.o[1] + cy, n1), g2 = wrapi(it.o[2] + cz, n2);
+      tile[cx * sp.PP + cy * sp.RP + cz] = A.grid[((long)g0 * n1 + g1) * n2 + g2];
+    }
   }
   __syncthreads();
   for (int p = it.beg + threadIdx.x; p < it.end; p += kTileThreads) {

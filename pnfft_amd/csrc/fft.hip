@@ -101,14 +101,23 @@ __device__ inline void dft8(Cplx<R>* v) {
   v[7] = csub(e3, o3);
 }
 
-// Block shape of a pass over pencils of length L: C pencils per block with
-// C * L = EPT * kThreads, LDS layout [position][pencil] with pitch LD.
-template <class R, int L>
+// Block shape of a pass over pencils of length L with NT threads: C pencils per
+// block with C * L = EPT * NT, LDS layout [position][pencil] with pitch LD.
+template <class R, int L, int NT = kThreads>
 struct PassShape {
   static constexpr int EPT = FftEpt<R>::value;
-  static constexpr int C = EPT * kThreads / L;
+  static constexpr int C = EPT * NT / L;
   static constexpr int LD = C == 1 ? 1 : C + 1;
+  static constexpr size_t LDS = sizeof(Cplx<R>) * (size_t)L * (size_t)LD;
 };
+
+// Threads of a columns pass: 512 where the doubled tile still fits in 160 KB of
+// LDS (twice the pencils per block, so every row segment is twice as long:
+// 256 B at L = 512 in FP32), else 256.
+template <class R, int L>
+constexpr int cols_threads() {
+  return PassShape<R, L, 2 * kThreads>::LDS <= 160 * 1024 ? 2 * kThreads : kThreads;
+}
 
 // radix of the stage that starts at span Ns (8 while possible, then 4 or 2)
 template <int L, int Ns>
@@ -126,9 +135,9 @@ __device__ inline void dft(Cplx<R>* w) {
 // One Stockham stage LDS -> LDS of radix RR at span Ns.  Butterfly bf of the
 // block covers pencil c = bf % C, index j = bf / C; it reads positions
 // j + q L/RR and writes (j - k) RR + k + q Ns with k = j mod Ns.
-template <class R, int L, int Ns>
+template <class R, int L, int Ns, int NT>
 __device__ inline void stage_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
-  using S = PassShape<R, L>;
+  using S = PassShape<R, L, NT>;
   constexpr int RR = StageRadix<L, Ns>::value;
   constexpr int NB = S::EPT / RR;
   constexpr int nbf = L / RR;
@@ -137,7 +146,7 @@ __device__ inline void stage_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
   Cplx<R> v[S::EPT];
 #pragma unroll
   for (int i = 0; i < NB; i++) {
-    const int bf = tid + i * kThreads;
+    const int bf = tid + i * NT;
     const int c = bf % S::C, j = bf / S::C;
 #pragma unroll
     for (int q = 0; q < RR; q++) v[i * RR + q] = sh[(j + q * nbf) * S::LD + c];
@@ -145,7 +154,7 @@ __device__ inline void stage_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < NB; i++) {
-    const int bf = tid + i * kThreads;
+    const int bf = tid + i * NT;
     const int c = bf % S::C, j = bf / S::C;
     const int k = j % Ns;
     Cplx<R>* w = v + i * RR;
@@ -162,11 +171,11 @@ __device__ inline void stage_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
 }
 
 // stages LDS -> LDS from span Ns up to, but not including, span END
-template <class R, int L, int Ns, int END>
+template <class R, int L, int Ns, int END, int NT>
 __device__ inline void stages_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
   if constexpr (Ns < END) {
-    stage_lds<R, L, Ns>(sh, tw);
-    stages_lds<R, L, Ns * StageRadix<L, Ns>::value, END>(sh, tw);
+    stage_lds<R, L, Ns, NT>(sh, tw);
+    stages_lds<R, L, Ns * StageRadix<L, Ns>::value, END, NT>(sh, tw);
   }
 }
 
@@ -263,7 +272,7 @@ __global__ __launch_bounds__(kThreads) void fft_rows(PassArgs<R> P) {
     sh[p * S::LD + c] = w;
   }
   __syncthreads();
-  stages_lds<R, L, 1, L>(sh, P.tw);
+  stages_lds<R, L, 1, L, kThreads>(sh, P.tw);
 #pragma unroll
   for (int i = 0; i < S::EPT; i++) {
     const int idx = tid + i * kThreads;
@@ -285,8 +294,9 @@ __global__ __launch_bounds__(kThreads) void fft_rows(PassArgs<R> P) {
 // stage runs on the loaded registers and the last stage stores straight to
 // HBM, so only the middle stages touch LDS.
 template <class R, int L>
-__global__ __launch_bounds__(kThreads) void fft_cols(PassArgs<R> P) {
-  using S = PassShape<R, L>;
+__global__ __launch_bounds__((cols_threads<R, L>())) void fft_cols(PassArgs<R> P) {
+  constexpr int NT = cols_threads<R, L>();
+  using S = PassShape<R, L, NT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<R>* sh = reinterpret_cast<Cplx<R>*>(smem_raw);
   constexpr int R1 = StageRadix<L, 1>::value;
@@ -302,7 +312,7 @@ __global__ __launch_bounds__(kThreads) void fft_cols(PassArgs<R> P) {
   // ---- first stage (span 1): butterfly (c, j) takes positions j + q L/R1
 #pragma unroll
   for (int i = 0; i < NB1; i++) {
-    const int bf = tid + i * kThreads;
+    const int bf = tid + i * NT;
     const int c = bf % S::C, j = bf / S::C;
     const int col = c0 + c;
 #pragma unroll
@@ -321,17 +331,17 @@ __global__ __launch_bounds__(kThreads) void fft_cols(PassArgs<R> P) {
   if constexpr (NsL > 1) {
 #pragma unroll
     for (int i = 0; i < NB1; i++) {
-      const int bf = tid + i * kThreads;
+      const int bf = tid + i * NT;
       const int c = bf % S::C, j = bf / S::C;
 #pragma unroll
       for (int q = 0; q < R1; q++) sh[(j * R1 + q) * S::LD + c] = v[i * R1 + q];
     }
     __syncthreads();
-    stages_lds<R, L, R1, NsL>(sh, P.tw);
+    stages_lds<R, L, R1, NsL, NT>(sh, P.tw);
     // ---- last stage (span NsL): outputs at positions j + q NsL
 #pragma unroll
     for (int i = 0; i < NBL; i++) {
-      const int bf = tid + i * kThreads;
+      const int bf = tid + i * NT;
       const int c = bf % S::C, j = bf / S::C;
 #pragma unroll
       for (int q = 0; q < RL; q++) v[i * RL + q] = sh[(j + q * nbfL) * S::LD + c];
@@ -339,7 +349,7 @@ __global__ __launch_bounds__(kThreads) void fft_cols(PassArgs<R> P) {
     constexpr int twstep = L / (NsL * RL);
 #pragma unroll
     for (int i = 0; i < NBL; i++) {
-      const int bf = tid + i * kThreads;
+      const int bf = tid + i * NT;
       const int j = bf / S::C;
 #pragma unroll
       for (int q = 1; q < RL; q++) v[i * RL + q] = cmul(v[i * RL + q], P.tw[j * q * twstep]);
@@ -352,7 +362,7 @@ __global__ __launch_bounds__(kThreads) void fft_cols(PassArgs<R> P) {
   constexpr int stepO = NsL > 1 ? NsL : nbf1;
 #pragma unroll
   for (int i = 0; i < NBO; i++) {
-    const int bf = tid + i * kThreads;
+    const int bf = tid + i * NT;
     const int c = bf % S::C, j = bf / S::C;
     const int col = c0 + c;
     if (col >= P.n_inner) continue;
@@ -376,22 +386,6 @@ bool fft_length_supported(int L, bool is_double) {
 }
 
 template <class R>
-int FftAxis<R>::init(int L_) {
-  L = L_;
-  if (!fft_length_supported(L, sizeof(R) == 8)) return kUnsupported;
-  C = FftEpt<R>::value * kThreads / L;
-  std::vector<Cplx<R>> h(L);
-  for (int i = 0; i < L; i++) {
-    // angle -2 pi i / L evaluated in FP64
-    const double a = -2.0 * 3.14159265358979323846 * (double)i / (double)L;
-    h[i] = cmk<R>(R(cos(a)), R(sin(a)));
-  }
-  if (hipMalloc(&tw, sizeof(Cplx<R>) * L) != hipSuccess) return kHipError;
-  if (hipMemcpy(tw, h.data(), sizeof(Cplx<R>) * L, hipMemcpyHostToDevice) != hipSuccess) return kHipError;
-  return kOk;
-}
-
-template <class R>
 void FftAxis<R>::release() {
   if (tw) (void)hipFree(tw);
   tw = nullptr;
@@ -407,17 +401,15 @@ static PassArgs<R> make_args(const FftAxis<R>& ax) {
 }
 
 template <class R, int L>
-constexpr size_t pass_lds() {
-  return sizeof(Cplx<R>) * (size_t)L * (size_t)PassShape<R, L>::LD;
-}
-
-template <class R, int L>
 static int pass_attr() {
-  constexpr size_t lds = pass_lds<R, L>();
-  if (lds <= 64 * 1024) return kOk;
-  if (hipFuncSetAttribute((const void*)fft_rows<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-          hipSuccess ||
-      hipFuncSetAttribute((const void*)fft_cols<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+  constexpr size_t lr = PassShape<R, L>::LDS;
+  constexpr size_t lc = PassShape<R, L, cols_threads<R, L>()>::LDS;
+  if (lr > 64 * 1024 &&
+      hipFuncSetAttribute((const void*)fft_rows<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr) !=
+          hipSuccess)
+    return kHipError;
+  if (lc > 64 * 1024 &&
+      hipFuncSetAttribute((const void*)fft_cols<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lc) !=
           hipSuccess)
     return kHipError;
   return kOk;
@@ -425,9 +417,14 @@ static int pass_attr() {
 
 template <class R, int L>
 static void pass_launch(bool rows, dim3 grid, const PassArgs<R>& P, hipStream_t s) {
-  constexpr size_t lds = pass_lds<R, L>();
-  if (rows) hipLaunchKernelGGL((fft_rows<R, L>), grid, dim3(kThreads), lds, s, P);
-  else hipLaunchKernelGGL((fft_cols<R, L>), grid, dim3(kThreads), lds, s, P);
+  if (rows) {
+    constexpr size_t lds = PassShape<R, L>::LDS;
+    hipLaunchKernelGGL((fft_rows<R, L>), grid, dim3(kThreads), lds, s, P);
+  } else {
+    constexpr int nt = cols_threads<R, L>();
+    constexpr size_t lds = PassShape<R, L, nt>::LDS;
+    hipLaunchKernelGGL((fft_cols<R, L>), grid, dim3(nt), lds, s, P);
+  }
 }
 
 // Runs F(L) for the supported lengths; F is a generic lambda taking the
@@ -456,6 +453,29 @@ static int with_length(int L, F&& f) {
     default:
       return kUnsupported;
   }
+}
+
+template <class R>
+int FftAxis<R>::init(int L_) {
+  L = L_;
+  if (!fft_length_supported(L, sizeof(R) == 8)) return kUnsupported;
+  C = FftEpt<R>::value * kThreads / L;
+  Ccols = 0;
+  (void)with_length<R>(L, [&](auto Lc) {
+    constexpr int LL = decltype(Lc)::value;
+    Ccols = PassShape<R, LL, cols_threads<R, LL>()>::C;
+    return kOk;
+  });
+  if (Ccols <= 0) return kUnsupported;
+  std::vector<Cplx<R>> h(L);
+  for (int i = 0; i < L; i++) {
+    // angle -2 pi i / L evaluated in FP64
+    const double a = -2.0 * 3.14159265358979323846 * (double)i / (double)L;
+    h[i] = cmk<R>(R(cos(a)), R(sin(a)));
+  }
+  if (hipMalloc(&tw, sizeof(Cplx<R>) * L) != hipSuccess) return kHipError;
+  if (hipMemcpy(tw, h.data(), sizeof(Cplx<R>) * L, hipMemcpyHostToDevice) != hipSuccess) return kHipError;
+  return kOk;
 }
 
 template <class R>
@@ -496,6 +516,7 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.in_outer = (long)N1 * n2; P.in_elem = n2;
     P.out_outer = (long)n1 * n2; P.out_elem = n2;
     P.n_inner = n2;
+    P.C = F.ax[1].Ccols;
     P.Lin = N1; P.Lout = n1;
     P.h = N1 / 2; P.pos = N1 - N1 / 2;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);
@@ -509,6 +530,7 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.in_outer = n2; P.in_elem = (long)n1 * n2;
     P.out_outer = n2; P.out_elem = (long)n1 * n2;
     P.n_inner = n2;
+    P.C = F.ax[0].Ccols;
     P.Lin = N0; P.Lout = n0;
     P.h = N0 / 2; P.pos = N0 - N0 / 2;
     dim3 grid_dim((n2 + P.C - 1) / P.C, n1);
@@ -531,6 +553,7 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.in_outer = n2; P.in_elem = (long)n1 * n2;
     P.out_outer = n2; P.out_elem = (long)n1 * n2;
     P.n_inner = n2;
+    P.C = F.ax[0].Ccols;
     P.Lin = n0; P.Lout = N0; P.h = N0 / 2;
     P.backward = 1;
     dim3 grid_dim((n2 + P.C - 1) / P.C, n1);
@@ -544,6 +567,7 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.in_outer = (long)n1 * n2; P.in_elem = n2;
     P.out_outer = (long)N1 * n2; P.out_elem = n2;
     P.n_inner = n2;
+    P.C = F.ax[1].Ccols;
     P.Lin = n1; P.Lout = N1; P.h = N1 / 2;
     P.backward = 1;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);

@@ -16,7 +16,8 @@ bool fft_length_supported(int L, bool is_double);
 template <class R>
 struct FftAxis {
   int L = 0;
-  int C = 0;          // pencils per block
+  int C = 0;          // pencils per block, rows pass
+  int Ccols = 0;      // pencils per block, columns passes
   Cplx<R>* tw = nullptr;  // device twiddles e^{-2 pi i j / L}
   int init(int L);
   void release();

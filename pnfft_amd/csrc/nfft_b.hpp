@@ -38,6 +38,7 @@ struct BArgs {
   const int* tile_slot;    // [ntiles] slot of each tile, -1 if empty
   const int* torder;       // [n_torder] nonempty tiles, most nodes first
   int n_torder;
+  int scatter_vec;         // 1: vector column kernel also where the matrix-core kernel applies
   // window tables of pnfft_precompute_psi in the sorted order, [3 CM][M] each (or null)
   const R* pre_psi;
   const R* pre_dpsi;

@@ -187,29 +187,11 @@ __global__ __launch_bounds__((ColsShape<R, MT, MODE>::NT)) void scatter_cols_k(B
       }
     }
     __syncthreads();
-    if constexpr (sizeof(R) == 4) {
-      // two nodes per iteration: both nodes' LDS loads are issued before either
-      // node's accumulation, so their latencies overlap
-      for (int i = 0; i < nb; i += 2) {
-        const bool two = i + 1 < nb;
-        const int locA = __builtin_amdgcn_readfirstlane(s_loc[i]);
-        const int locB = two ? __builtin_amdgcn_readfirstlane(s_loc[i + 1]) : -1;
-        double wA[NC][4], wB[NC][4];
-        const bool onA =
-            node_weights<R, NC, CM, WF, WG>(locA, i, b, c, H, own, s_py, s_pz, s_dy, s_dz, s_f, s_g, wA);
-        const bool onB =
-            node_weights<R, NC, CM, WF, WG>(locB, i + 1, b, c, H, own, s_py, s_pz, s_dy, s_dz, s_f, s_g, wB);
-        if (onA) plane_blocks<NC, P, CM, WG, 0>(locA & 255, ar, ai, &s_px[i][0], WG ? &s_dx[i][0] : nullptr, wA);
-        if (onB)
-          plane_blocks<NC, P, CM, WG, 0>(locB & 255, ar, ai, &s_px[i + 1][0], WG ? &s_dx[i + 1][0] : nullptr, wB);
-      }
-    } else {  // FP64: one node per iteration (two would exceed 256 VGPRs)
-      for (int i = 0; i < nb; i++) {
-        const int loc = __builtin_amdgcn_readfirstlane(s_loc[i]);
-        double w[NC][4];
-        if (node_weights<R, NC, CM, WF, WG>(loc, i, b, c, H, own, s_py, s_pz, s_dy, s_dz, s_f, s_g, w))
-          plane_blocks<NC, P, CM, WG, 0>(loc & 255, ar, ai, &s_px[i][0], WG ? &s_dx[i][0] : nullptr, w);
-      }
+    for (int i = 0; i < nb; i++) {
+      const int loc = __builtin_amdgcn_readfirstlane(s_loc[i]);
+      double w[NC][4];
+      if (node_weights<R, NC, CM, WF, WG>(loc, i, b, c, H, own, s_py, s_pz, s_dy, s_dz, s_f, s_g, w))
+        plane_blocks<NC, P, CM, WG, 0>(loc & 255, ar, ai, &s_px[i][0], WG ? &s_dx[i][0] : nullptr, w);
     }
     __syncthreads();
   }
