@@ -205,6 +205,152 @@ __global__ __launch_bounds__((ColsShape<R, MT, MODE>::NT)) void scatter_cols_k(B
   }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core form (FP32 plans, P <= 16, i.e. m <= 4): the tile update
+//   G[a][b][c] += sum_v X_v[a] Y_v[b] W_v Z_v[c]
+// over rank-one terms v (one per node for f; three for grad, (psi'x, psi y,
+// psi z; g0), (psi x, psi'y, psi z; g1), (psi x, psi y, psi'z; g2); four for
+// f + grad) is, for every plane a, the product of the 16 x K matrix
+// A[b][v] = X_v[a] Y_v[b] and the K x 16 matrix B[v][c] = W_v Z_v[c]: one
+// v_mfma_f32_16x16x4f32 per plane, K = 4 terms, per real and imaginary part.
+// The four waves of a block own the planes a = w + 4q (q = 0..3, interleaved so
+// that every wave meets a node's stencil on 2-3 of its planes); a plane is
+// skipped when none of the quad's four terms reaches it (sorted nodes of a tile
+// share their plane offset most of the time).  The FP32 accumulators are added
+// into FP64 totals after every batch of terms; the totals go to the tile's slot.
+// ---------------------------------------------------------------------------
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+template <int MODE>
+struct MfmaShape {
+  static constexpr int V = MODE == B_F ? 1 : (MODE == B_GRAD ? 3 : 4);  // rank-one terms per node
+  static constexpr int NB = 64;                                          // nodes per batch
+  static constexpr int NV = NB * V;                                      // terms per batch
+  // waves per SIMD the kernel is compiled for: the gradient windows need the larger register budget
+  static constexpr int WAVES = V == 1 ? 3 : 2;
+};
+
+// row position of plane a: wave (a & 3) reads the float4 at 4 (a & 3), entry a >> 2
+__device__ __forceinline__ int xpos(int a) { return ((a & 3) << 2) | (a >> 2); }
+
+template <int MT, int MODE>
+__global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(BArgs<float> A) {
+  using S = ColsShape<float, MT, MODE>;
+  using MS = MfmaShape<MODE>;
+  constexpr int CM = S::CM, P = S::P, TS = S::TS, V = MS::V, NB = MS::NB, NV = MS::NV;
+  constexpr bool WF = S::WF, WG = S::WG;
+  static_assert(P <= 16, "one 16 x 16 block per plane");
+  __shared__ __align__(16) float s_x[NV][16];  // X_v by xpos
+  __shared__ __align__(16) float s_y[NV][16];  // Y_v[b], zero outside the stencil
+  __shared__ __align__(16) float s_z[NV][16];  // Z_v[c]
+  __shared__ float2 s_w[NV];                   // W_v
+
+  const int tile = A.torder[blockIdx.x];
+  const int t2 = tile % A.ntile[2];
+  const int t1 = (tile / A.ntile[2]) % A.ntile[1];
+  const int t0 = tile / (A.ntile[2] * A.ntile[1]);
+  const int o0 = t0 * TS, o1 = t1 * TS, o2 = t2 * TS;
+  const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
+  const int beg = A.tile_start[tile], end = A.tile_start[tile + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+
+  mfma_f4 cre[4], cim[4];  // plane wave + 4q: D[b = 4 lk + e][c = li]
+  double tre[4][4], tim[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      cre[q][e] = cim[q][e] = 0.f;
+      tre[q][e] = tim[q][e] = 0.0;
+    }
+
+  for (int base = beg; base < end; base += NB) {
+    const int nb = min(NB, end - base);
+    if (tid < NB) {
+      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int u = 0; u < V; u++) {
+        const int v = tid * V + u;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          reinterpret_cast<float4*>(s_x[v])[e] = z4;
+          reinterpret_cast<float4*>(s_y[v])[e] = z4;
+          reinterpret_cast<float4*>(s_z[v])[e] = z4;
+        }
+        s_w[v] = make_float2(0.f, 0.f);
+      }
+      if (tid < nb) {
+        const long j = A.perm[base + tid];
+        NodeWin<float, CM> w;
+        node_windows<float, MT, CM, WG>(A, base + tid, j, MT, w);
+        const int l0 = local_off(w.start[0], o0, TS, n0);
+        const int l1 = local_off(w.start[1], o1, TS, n1);
+        const int l2 = local_off(w.start[2], o2, TS, n2);
+        if (l0 >= 0 && l1 >= 0 && l2 >= 0) {
+          // term u: (x, y, z) factors ψ or ψ' and its weight
+          auto put = [&](int v, const float(&X)[CM], const float(&Y)[CM], const float(&Z)[CM], Cplx<float> wv) {
+#pragma unroll
+            for (int s = 0; s < CM; s++) {
+              s_x[v][xpos(l0 + s)] = X[s];
+              s_y[v][l1 + s] = Y[s];
+              s_z[v][l2 + s] = Z[s];
+            }
+            s_w[v] = make_float2(wv.x, wv.y);
+          };
+          int v = tid * V;
+          if (WF) put(v++, w.px, w.py, w.pz, load_f(A, j));
+          if (WG) {
+            put(v++, w.dx, w.py, w.pz, load_g(A, j, 0));
+            put(v++, w.px, w.dy, w.pz, load_g(A, j, 1));
+            put(v++, w.px, w.py, w.dz, load_g(A, j, 2));
+          }
+        } else {
+          A.stale_list[atomicAdd(A.stale_count, 1u)] = base + tid;
+        }
+      }
+    }
+    __syncthreads();
+    const int nq = (nb * V + 3) >> 2;
+    for (int q4 = 0; q4 < nq; q4++) {
+      const int v = 4 * q4 + lk;
+      const float4 x = reinterpret_cast<const float4*>(s_x[v])[wave];
+      const float y = s_y[v][li], z = s_z[v][li];
+      const float2 wv = s_w[v];
+      const float bre = wv.x * z, bim = wv.y * z;
+      const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (__any(xs[q] != 0.f)) {
+          const float a = xs[q] * y;
+          cre[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bre, cre[q], 0, 0, 0);
+          cim[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bim, cim[q], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        tre[q][e] += (double)cre[q][e];
+        tim[q][e] += (double)cim[q][e];
+        cre[q][e] = cim[q][e] = 0.f;
+      }
+    __syncthreads();
+  }
+
+  Cplx<float>* slot = A.slots + (size_t)blockIdx.x * (size_t)(P * P * P);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int a = wave + 4 * q;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int b = 4 * lk + e;
+      if (a < P && b < P && li < P) slot[(a * P + b) * P + li] = cmk<float>((float)tre[q][e], (float)tim[q][e]);
+    }
+  }
+}
+
 // grid[x] = sum over the tiles whose padded subgrids cover x of their slot
 // value at x, tiles taken in a fixed order; cells no nonempty tile covers get 0.
 // One block per tile of the grid (empty ones included), one thread per cell;
@@ -271,7 +417,14 @@ template <class R, int MT, int MODE>
 void launch_m(const BArgs<R>& A, hipStream_t s) {
   using S = ColsShape<R, MT, MODE>;
   hipLaunchKernelGGL(zero_count_k, dim3(1), dim3(64), 0, s, A.stale_count);
-  hipLaunchKernelGGL((scatter_cols_k<R, MT, MODE>), dim3((unsigned)A.n_torder), dim3(S::NT), 0, s, A);
+  bool done = false;
+  if constexpr (sizeof(R) == 4 && S::P <= 16) {
+    if (!A.scatter_vec) {
+      hipLaunchKernelGGL((scatter_mfma_k<MT, MODE>), dim3((unsigned)A.n_torder), dim3(256), 0, s, A);
+      done = true;
+    }
+  }
+  if (!done) hipLaunchKernelGGL((scatter_cols_k<R, MT, MODE>), dim3((unsigned)A.n_torder), dim3(S::NT), 0, s, A);
   if (A.mid) (void)hipEventRecord(A.mid, s);
   const long ntiles = (long)A.ntile[0] * A.ntile[1] * A.ntile[2];
   hipLaunchKernelGGL((fold_k<R, S::TS, S::P>), dim3((unsigned)ntiles), dim3(S::TS * S::TS * S::TS), 0, s, A);

@@ -396,3 +396,37 @@ def test_fp32_adjoint_of_large_gradients(gpu, oracle):
     P = oracle.Plan(w["N"], w["n"], w["m"], 0)
     assert np.isfinite(fh).all()
     assert rel(fh, P.adj(x, f=w["f"], grad=g)) < 1e-5
+
+
+@pytest.mark.parametrize("m", [2, 3, 4])
+def test_fp32_adjoint_modes_clustered(gpu, oracle, m):
+    """FP32 adjoint of f, of grad_f and of both (for m <= 4 the matrix-core tile scatter with one,
+    three and four rank-one terms per node) on clustered nodes: tiles from one node to several
+    hundred (several batches per tile), with and without precomputed window tables."""
+    from pnfft_amd.workloads import nodes
+    rng = SplitMix64(400 + m)
+    M = 20000
+    N, n = (32, 32, 32), (64, 64, 64)
+    x = nodes("gaussmix", M, rng).astype(np.float32).astype(np.float64)
+    fv = rng.complex_normal(M)
+    gv = rng.complex_normal(3 * M)
+    P = oracle.Plan(N, n, m, 0)
+    F, G = PA.PNFFT_COMPUTE_F, PA.PNFFT_COMPUTE_GRAD_F
+    refs = {F: P.adj(x, f=fv), G: P.adj(x, grad=gv), F | G: P.adj(x, f=fv, grad=gv)}
+    for pre in (0, PA.PNFFT_PRE_PSI | PA.PNFFT_PRE_GRAD_PSI):
+        say("case fp32 adjoint modes m", m, "pre", pre)
+        dp = DevicePlan(N, n, m, 0, M, precision="float")
+        dp.set_x(x)
+        if pre:
+            dp.precompute(pre)
+        dp.set_f(fv)
+        dp.set_grad(gv)
+        errs = {}
+        for flags, ref in refs.items():
+            dp.adj(flags)
+            errs[flags] = rel(dp.f_hat.cpu().numpy().reshape(-1), ref)
+        err = dp.api.last_error()
+        dp.close()
+        assert err is None, err
+        say("  rel errors", errs)
+        assert max(errs.values()) < 1e-5
