@@ -581,15 +581,23 @@ static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
   return true;
 }
 
-// Scatter path: the column scatter (default; matrix-core kernel where it
-// applies) or, for comparison, the LDS-atomic kernels (PNFFT_AMD_SCATTER=atomic)
-// or the vector column kernel everywhere (PNFFT_AMD_SCATTER=cols).
-static const char* scatter_env() {
-  static const char* e = getenv("PNFFT_AMD_SCATTER");
-  return e ? e : "";
+// Scatter path: the column scatter (default) or, for comparison, the
+// LDS-atomic kernels (PNFFT_AMD_SCATTER=atomic).  PNFFT_AMD_MFMA=0 keeps the
+// vector kernels where the matrix-core gather and scatter would apply.
+static bool cols_enabled() {
+  static const bool v = [] {
+    const char* e = getenv("PNFFT_AMD_SCATTER");
+    return !(e && strcmp(e, "atomic") == 0);
+  }();
+  return v;
 }
-static bool cols_enabled() { return strcmp(scatter_env(), "atomic") != 0; }
-static bool cols_vector_only() { return strcmp(scatter_env(), "cols") == 0; }
+static bool mfma_disabled() {
+  static const bool v = [] {
+    const char* e = getenv("PNFFT_AMD_MFMA");
+    return e && strcmp(e, "0") == 0;
+  }();
+  return v;
+}
 
 // Minimum node count for a tile to take the LDS path (tunable by PNFFT_AMD_DENSE_MIN).
 static int dense_min() {
@@ -609,14 +617,16 @@ static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false)
   A.n_items = for_scatter ? D->n_items_s : D->n_items;
   // the scatter's list holds every nonempty tile; the gather's when dense_min is 1
   A.all_dense = (for_scatter || dense_min() == 1) ? 1 : 0;
+  A.no_mfma = mfma_disabled() ? 1 : 0;
+  if (D->n_torder > 0) {
+    A.torder = D->d_torder;
+    A.n_torder = D->n_torder;
+  }
   if (for_scatter && D->n_torder > 0) {
     A.slots = D->d_slots;
     A.tile_slot = D->d_tile_slot;
-    A.torder = D->d_torder;
-    A.n_torder = D->n_torder;
     A.stale_count = D->d_stale_count;
     A.stale_list = D->d_stale;
-    A.scatter_vec = cols_vector_only() ? 1 : 0;
   }
   for (int t = 0; t < 3; t++) {
     A.ts[t] = D->bin_tiles.ts[t];

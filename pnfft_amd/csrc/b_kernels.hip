@@ -293,7 +293,239 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core gather (FP32 plans, P = 8 + 2m <= 16, i.e. m <= 4; f, grad,
+// f + grad).  For a plane a of the tile's padded subgrid and a group of 16
+// nodes,
+//   T_a[b][j] = sum_c G[a][b][c] Z_j[c]      (Z_j = psi_z of node j, zero-padded)
+// is a 16 x 16 x 16 product: four v_mfma_f32_16x16x4f32 (K = 4 values of c
+// each) per real and imaginary part, four more each with psi'_z for the
+// gradient.  In the accumulator layout a lane holds T_a[b][j] for one node j and
+// four rows b, so f_j = sum_a X_j[a] sum_b Y_j[b] T_a[b][j] (and the gradient
+// sums, one factor replaced by its derivative) run on the vector ALUs; the four
+// lane groups' partial sums over b are added at the end.  One block per
+// nonempty tile (heavy tiles first), four waves, each taking one group of 16
+// sorted nodes per batch; a plane that no node of the group reaches is skipped
+// by a uniform branch (sorted nodes of a tile share their plane offset most of
+// the time).  Nodes found outside their tile (bins older than x) are gathered
+// straight from the grid by the thread that evaluated their windows.
+// ---------------------------------------------------------------------------
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+// position of c in a subgrid row or z-window row: c = 4 kc + k sits at 4 k + kc,
+// so lane group k reads its four K-slices (kc = 0..3) as one float4
+__device__ __forceinline__ int cpos(int c) { return ((c & 3) << 2) | (c >> 2); }
+
+template <int MT, int MODE>
+struct GatherMfmaShape {
+  static constexpr int CM = 2 * MT + 1, TS = 8, P = TS + 2 * MT;
+  static constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
+  static constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
+  static constexpr int NB = 64;  // nodes per batch: one group of 16 per wave
+  static constexpr int GP = 20;  // row pitch (floats): rows b = 0..15 start in distinct bank quads
+  static constexpr int ROWS = WG ? 6 : 3;  // window rows per node: x, y, z (and x', y', z')
+  static constexpr size_t LDS = sizeof(float) * (2 * P * 16 * GP + ROWS * NB * 16) + sizeof(int) * NB;
+};
+
+template <int MT, int MODE>
+__global__ __launch_bounds__(256, 2) void gather_mfma_k(BArgs<float> A) {
+  using S = GatherMfmaShape<MT, MODE>;
+  constexpr int CM = S::CM, TS = S::TS, P = S::P, NB = S::NB, GP = S::GP;
+  constexpr bool WG = S::WG;
+  static_assert(P <= 16, "one 16 x 16 block per plane");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* s_gr = reinterpret_cast<float*>(smem_raw);  // [P][16][GP], G by (a, b, cpos(c))
+  float* s_gi = s_gr + P * 16 * GP;
+  float* s_x = s_gi + P * 16 * GP;  // [NB][16] window rows (z by cpos), zero outside the stencil
+  float* s_y = s_x + NB * 16;
+  float* s_z = s_y + NB * 16;
+  float* s_dx = s_z + NB * 16;  // gradient rows (WG only)
+  float* s_dy = s_dx + NB * 16;
+  float* s_dz = s_dy + NB * 16;
+  int* s_j = reinterpret_cast<int*>(s_x + S::ROWS * NB * 16);
+
+  const int tile = A.torder[blockIdx.x];
+  const int t2 = tile % A.ntile[2];
+  const int t1 = (tile / A.ntile[2]) % A.ntile[1];
+  const int t0 = tile / (A.ntile[2] * A.ntile[1]);
+  const int o0 = t0 * TS, o1 = t1 * TS, o2 = t2 * TS;
+  const unsigned n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
+  const int beg = A.tile_start[tile], end = A.tile_start[tile + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+
+  // stage the subgrid, split into real and imaginary parts; pairs of cells along c
+  if constexpr (P < 16) {
+    for (int q = tid; q < 2 * P * 16 * GP / 4; q += 256)
+      reinterpret_cast<float4*>(s_gr)[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+  }
+  {
+    constexpr int H = P / 2, NPAIR = P * P * H, K = (NPAIR + 255) / 256;
+    const float4* src = reinterpret_cast<const float4*>(A.grid);
+    float4 v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = tid + 256 * k;
+      if (q < NPAIR) {
+        const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
+        const unsigned g0 = (unsigned)(o0 + cx) & (n0 - 1), g1 = (unsigned)(o1 + cy) & (n1 - 1);
+        const unsigned g2 = (unsigned)(o2 + 2 * hz) & (n2 - 1);
+        v[k] = src[((g0 * n1 + g1) * n2 + g2) >> 1];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = tid + 256 * k;
+      if (q < NPAIR) {
+        const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
+        const int r = (cx * 16 + cy) * GP;
+        s_gr[r + cpos(2 * hz)] = v[k].x;
+        s_gi[r + cpos(2 * hz)] = v[k].y;
+        s_gr[r + cpos(2 * hz + 1)] = v[k].z;
+        s_gi[r + cpos(2 * hz + 1)] = v[k].w;
+      }
+    }
+  }
+
+  for (int base = beg; base < end; base += NB) {
+    const int nb = min(NB, end - base);
+    if (tid < NB) {
+      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int r = 0; r < S::ROWS; r++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) reinterpret_cast<float4*>(s_x + (r * NB + tid) * 16)[e] = z4;
+      s_j[tid] = -1;
+      if (tid < nb) {
+        const int p = base + tid;
+        const long j = A.perm[p];
+        NodeWin<float, CM> w;
+        node_windows<float, MT, CM, WG>(A, p, j, MT, w);
+        const int l0 = local_off(w.start[0], o0, TS, (int)n0);
+        const int l1 = local_off(w.start[1], o1, TS, (int)n1);
+        const int l2 = local_off(w.start[2], o2, TS, (int)n2);
+        if (l0 >= 0 && l1 >= 0 && l2 >= 0) {
+#pragma unroll
+          for (int s = 0; s < CM; s++) {
+            s_x[tid * 16 + l0 + s] = w.px[s];
+            s_y[tid * 16 + l1 + s] = w.py[s];
+            s_z[tid * 16 + cpos(l2 + s)] = w.pz[s];
+            if (WG) {
+              s_dx[tid * 16 + l0 + s] = w.dx[s];
+              s_dy[tid * 16 + l1 + s] = w.dy[s];
+              s_dz[tid * 16 + cpos(l2 + s)] = w.dz[s];
+            }
+          }
+          s_j[tid] = (int)j;
+        } else {
+          gather_node_global<float, CM, MODE, true>(A, j, w, CM);
+        }
+      }
+    }
+    __syncthreads();
+    if (16 * wave < nb) {
+      const int node = 16 * wave + li;
+      const float4 zc = reinterpret_cast<const float4*>(s_z + node * 16)[lk];  // Z_j[4 kc + lk], kc = 0..3
+      const float4 yb = reinterpret_cast<const float4*>(s_y + node * 16)[lk];  // Y_j[4 lk + e], e = 0..3
+      float4 dzc = zc, dyb = yb;
+      if (WG) {
+        dzc = reinterpret_cast<const float4*>(s_dz + node * 16)[lk];
+        dyb = reinterpret_cast<const float4*>(s_dy + node * 16)[lk];
+      }
+      float fr = 0.f, fi = 0.f, g0r = 0.f, g0i = 0.f, g1r = 0.f, g1i = 0.f, g2r = 0.f, g2i = 0.f;
+      for (int a = 0; a < P; a++) {
+        const float xa = s_x[node * 16 + a];
+        const float dxa = WG ? s_dx[node * 16 + a] : 0.f;
+        if (!__any(xa != 0.f || dxa != 0.f)) continue;
+        const float4 gr = reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * GP)[lk];
+        const float4 gi = reinterpret_cast<const float4*>(s_gi + (a * 16 + li) * GP)[lk];
+        mfma_f4 tr = {0.f, 0.f, 0.f, 0.f}, ti = tr;
+        tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.x, zc.x, tr, 0, 0, 0);
+        ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.x, zc.x, ti, 0, 0, 0);
+        tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.y, zc.y, tr, 0, 0, 0);
+        ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.y, zc.y, ti, 0, 0, 0);
+        tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.z, zc.z, tr, 0, 0, 0);
+        ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.z, zc.z, ti, 0, 0, 0);
+        tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.w, zc.w, tr, 0, 0, 0);
+        ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.w, zc.w, ti, 0, 0, 0);
+        const float sr = yb.x * tr[0] + yb.y * tr[1] + yb.z * tr[2] + yb.w * tr[3];
+        const float si = yb.x * ti[0] + yb.y * ti[1] + yb.z * ti[2] + yb.w * ti[3];
+        fr += xa * sr;
+        fi += xa * si;
+        if (WG) {
+          mfma_f4 ur = {0.f, 0.f, 0.f, 0.f}, ui = ur;
+          ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.x, dzc.x, ur, 0, 0, 0);
+          ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.x, dzc.x, ui, 0, 0, 0);
+          ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.y, dzc.y, ur, 0, 0, 0);
+          ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.y, dzc.y, ui, 0, 0, 0);
+          ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.z, dzc.z, ur, 0, 0, 0);
+          ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.z, dzc.z, ui, 0, 0, 0);
+          ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.w, dzc.w, ur, 0, 0, 0);
+          ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.w, dzc.w, ui, 0, 0, 0);
+          const float syr = dyb.x * tr[0] + dyb.y * tr[1] + dyb.z * tr[2] + dyb.w * tr[3];
+          const float syi = dyb.x * ti[0] + dyb.y * ti[1] + dyb.z * ti[2] + dyb.w * ti[3];
+          const float szr = yb.x * ur[0] + yb.y * ur[1] + yb.z * ur[2] + yb.w * ur[3];
+          const float szi = yb.x * ui[0] + yb.y * ui[1] + yb.z * ui[2] + yb.w * ui[3];
+          g0r += dxa * sr;
+          g0i += dxa * si;
+          g1r += xa * syr;
+          g1i += xa * syi;
+          g2r += xa * szr;
+          g2i += xa * szi;
+        }
+      }
+      // sum over the four lane groups (rows b = 4 lk + e)
+      auto red = [](float v) {
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        return v;
+      };
+      fr = red(fr);
+      fi = red(fi);
+      if (WG) {
+        g0r = red(g0r);
+        g0i = red(g0i);
+        g1r = red(g1r);
+        g1i = red(g1i);
+        g2r = red(g2r);
+        g2i = red(g2i);
+      }
+      const int j = s_j[node];
+      if (lk == 0 && j >= 0) {
+        Cplx<float> v[4] = {cmk<float>(fr, fi), cmk<float>(g0r, g0i), cmk<float>(g1r, g1i), cmk<float>(g2r, g2i)};
+        gather_store<float, MODE>(A, j, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// the matrix-core gather is used for FP32 f with m <= 4 on the standard tile
+// layout, once the nonempty tiles are listed (the gradient modes measured slower
+// than the vector kernel so far)
+template <class R, int MT, int MODE>
+bool mfma_gather(const BArgs<R>& A) {
+  if constexpr (sizeof(R) != 4 || MT > 4 || MODE != B_F) {
+    return false;
+  } else {
+    return !A.no_mfma && A.torder && A.n_torder > 0 && A.ts[0] == 8 && A.ts[1] == 8 && A.ts[2] == 8;
+  }
+}
+
+template <class R, int MT, int MODE>
+void launch_gather_mfma(const BArgs<R>& A, hipStream_t s) {
+  if constexpr (sizeof(R) == 4 && MT <= 4 && MODE <= B_FGRAD) {
+    using S = GatherMfmaShape<MT, MODE>;
+    (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)S::LDS);
+    hipLaunchKernelGGL((gather_mfma_k<MT, MODE>), dim3((unsigned)A.n_torder), dim3(256), S::LDS, s, A);
+    if (A.mid) (void)hipEventRecord(A.mid, s);
+    if (A.mid2) (void)hipEventRecord(A.mid2, s);
+  }
+}
 
 }  // namespace
 
@@ -316,6 +548,10 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
 #define PNFFT_B_CASE(MV)                                                                                  \
   case MV:                                                                                                \
     if constexpr (KIND == 0) {                                                                            \
+      if (mfma_gather<R, MV, MODE>(A)) {                                                                  \
+        launch_gather_mfma<R, MV, MODE>(A, s);                                                            \
+        break;                                                                                            \
+      }                                                                                                   \
       if (tiled) {                                                                                        \
         (void)hipFuncSetAttribute((const void*)gather_tiled_k<R, MV, MODE>,                               \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                  \

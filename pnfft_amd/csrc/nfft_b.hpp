@@ -33,12 +33,13 @@ struct BArgs {
   int ts[3];               // tile size per axis
   int ntile[3];            // tiles per axis
   int all_dense;           // every nonempty tile is a work item: the direct kernel has no nodes to take
+  // nonempty tiles, most nodes first (column scatter and matrix-core gather)
+  const int* torder;       // [n_torder]
+  int n_torder;
   // column scatter (scatter_cols.hip): when slots is set, scatter() runs it instead of the LDS-atomic path
   Cplx<R>* slots;          // [n_torder][P^3] padded subgrid of every nonempty tile
   const int* tile_slot;    // [ntiles] slot of each tile, -1 if empty
-  const int* torder;       // [n_torder] nonempty tiles, most nodes first
-  int n_torder;
-  int scatter_vec;         // 1: vector column kernel also where the matrix-core kernel applies
+  int no_mfma;             // 1: vector kernels also where the matrix-core gather / scatter apply
   // window tables of pnfft_precompute_psi in the sorted order, [3 CM][M] each (or null)
   const R* pre_psi;
   const R* pre_dpsi;

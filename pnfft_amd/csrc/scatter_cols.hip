@@ -419,7 +419,7 @@ void launch_m(const BArgs<R>& A, hipStream_t s) {
   hipLaunchKernelGGL(zero_count_k, dim3(1), dim3(64), 0, s, A.stale_count);
   bool done = false;
   if constexpr (sizeof(R) == 4 && S::P <= 16) {
-    if (!A.scatter_vec) {
+    if (!A.no_mfma) {
       hipLaunchKernelGGL((scatter_mfma_k<MT, MODE>), dim3((unsigned)A.n_torder), dim3(256), 0, s, A);
       done = true;
     }

@@ -100,9 +100,10 @@ def test_c2r_direct_matches_reference_ndft(gpu, oracle):
     assert rel(g["adj_grad"], oracle.ndft_adj_c2r(list(N), x, grad=gv)) < 1e-12
 
 
-def test_c2r_fp32(gpu, oracle):
-    print("case c2r fp32", flush=True)
-    N, n, m = (16, 16, 16), (32, 32, 32), 6
+@pytest.mark.parametrize("m", [6, 4, 3])
+def test_c2r_fp32(gpu, oracle, m):
+    print("case c2r fp32 m", m, flush=True)
+    N, n = (16, 16, 16), (32, 32, 32)
     rng = SplitMix64(43)
     M = 1000
     x = (rng.uniform(3 * M).reshape(M, 3) - 0.5).astype(np.float32).astype(np.float64)

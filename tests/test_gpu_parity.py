@@ -283,14 +283,15 @@ def test_precompute_then_move_nodes(gpu, oracle):
     dp.close()
 
 
-@pytest.mark.parametrize("precision,tol", [("double", 1e-12), ("float", 1e-5)])
-def test_stale_bins_after_in_place_move(gpu, oracle, precision, tol):
+@pytest.mark.parametrize("precision,tol,m", [("double", 1e-12, 6), ("float", 1e-5, 6), ("float", 1e-5, 4)])
+def test_stale_bins_after_in_place_move(gpu, oracle, precision, tol, m):
     """precompute_psi bins reused after x changed in place (no set_x): nodes that left their
-    tiles go through the direct paths, trafo and adj still match the oracle for the new x."""
+    tiles go through the direct paths, trafo (f and grad) and adj still match the oracle for the
+    new x (m = 4 in FP32: the matrix-core gather and scatter)."""
     w = make(1, M=1500)
-    say("case stale bins", precision)
-    P = oracle.Plan(w["N"], w["n"], w["m"], 0)
-    dp = DevicePlan(w["N"], w["n"], w["m"], 0, w["M"], precision=precision)
+    say("case stale bins", precision, m)
+    P = oracle.Plan(w["N"], w["n"], m, 0)
+    dp = DevicePlan(w["N"], w["n"], m, 0, w["M"], precision=precision)
     x = w["x"]
     if precision == "float":
         x = x.astype(np.float32).astype(np.float64)
@@ -306,15 +307,18 @@ def test_stale_bins_after_in_place_move(gpu, oracle, precision, tol):
     if precision == "float":
         x2 = x2.astype(np.float32).astype(np.float64)
     dp.x.copy_(torch.as_tensor(x2).to(dp.x.dtype))
-    dp.trafo(PA.PNFFT_COMPUTE_F)
+    dp.trafo(PA.PNFFT_COMPUTE_F | PA.PNFFT_COMPUTE_GRAD_F)
     f_gpu = dp.f.cpu().numpy().copy()
+    g_gpu = dp.grad.cpu().numpy().reshape(-1).copy()
     dp.set_f(w["f"])
     dp.adj(PA.PNFFT_COMPUTE_F)
     fh_gpu = dp.f_hat.cpu().numpy().copy()
     err = dp.api.last_error()
     dp.close()
     assert err is None, err
-    assert rel(f_gpu, P.trafo(x2, w["f_hat"])[0]) < tol
+    f_o, g_o = P.trafo(x2, w["f_hat"], compute_grad=True)
+    assert rel(f_gpu, f_o) < tol
+    assert rel(g_gpu, g_o) < tol
     assert rel(fh_gpu, P.adj(x2, f=w["f"])) < tol
 
 
@@ -399,32 +403,42 @@ def test_fp32_adjoint_of_large_gradients(gpu, oracle):
 
 
 @pytest.mark.parametrize("m", [2, 3, 4])
-def test_fp32_adjoint_modes_clustered(gpu, oracle, m):
-    """FP32 adjoint of f, of grad_f and of both (for m <= 4 the matrix-core tile scatter with one,
-    three and four rank-one terms per node) on clustered nodes: tiles from one node to several
-    hundred (several batches per tile), with and without precomputed window tables."""
+def test_fp32_modes_clustered(gpu, oracle, m):
+    """FP32 trafo and adjoint of f, of grad_f and of both (for m <= 4 the matrix-core tile gather
+    and scatter; the scatter takes one, three and four rank-one terms per node) on clustered nodes:
+    tiles from one node to several hundred (several batches per tile), with and without
+    precomputed window tables."""
     from pnfft_amd.workloads import nodes
     rng = SplitMix64(400 + m)
     M = 20000
     N, n = (32, 32, 32), (64, 64, 64)
     x = nodes("gaussmix", M, rng).astype(np.float32).astype(np.float64)
+    fh = rng.complex_normal(int(np.prod(N)))
     fv = rng.complex_normal(M)
     gv = rng.complex_normal(3 * M)
     P = oracle.Plan(N, n, m, 0)
     F, G = PA.PNFFT_COMPUTE_F, PA.PNFFT_COMPUTE_GRAD_F
+    f_o, g_o = P.trafo(x, fh, compute_grad=True)
     refs = {F: P.adj(x, f=fv), G: P.adj(x, grad=gv), F | G: P.adj(x, f=fv, grad=gv)}
     for pre in (0, PA.PNFFT_PRE_PSI | PA.PNFFT_PRE_GRAD_PSI):
-        say("case fp32 adjoint modes m", m, "pre", pre)
+        say("case fp32 modes m", m, "pre", pre)
         dp = DevicePlan(N, n, m, 0, M, precision="float")
         dp.set_x(x)
         if pre:
             dp.precompute(pre)
+        errs = {}
+        for flags in (F, G, F | G):
+            dp.set_f_hat(fh)
+            dp.trafo(flags)
+            if flags & F:
+                errs[("f", flags)] = rel(dp.f.cpu().numpy(), f_o)
+            if flags & G:
+                errs[("grad", flags)] = rel(dp.grad.cpu().numpy().reshape(-1), g_o)
         dp.set_f(fv)
         dp.set_grad(gv)
-        errs = {}
         for flags, ref in refs.items():
             dp.adj(flags)
-            errs[flags] = rel(dp.f_hat.cpu().numpy().reshape(-1), ref)
+            errs[("adj", flags)] = rel(dp.f_hat.cpu().numpy().reshape(-1), ref)
         err = dp.api.last_error()
         dp.close()
         assert err is None, err
