@@ -621,12 +621,12 @@ static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false)
   if (D->n_torder > 0) {
     A.torder = D->d_torder;
     A.n_torder = D->n_torder;
+    A.stale_count = D->d_stale_count;
+    A.stale_list = D->d_stale;
   }
   if (for_scatter && D->n_torder > 0) {
     A.slots = D->d_slots;
     A.tile_slot = D->d_tile_slot;
-    A.stale_count = D->d_stale_count;
-    A.stale_list = D->d_stale;
   }
   for (int t = 0; t < 3; t++) {
     A.ts[t] = D->bin_tiles.ts[t];

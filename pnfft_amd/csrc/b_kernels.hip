@@ -303,46 +303,72 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
 // gradient.  In the accumulator layout a lane holds T_a[b][j] for one node j and
 // four rows b, so f_j = sum_a X_j[a] sum_b Y_j[b] T_a[b][j] (and the gradient
 // sums, one factor replaced by its derivative) run on the vector ALUs; the four
-// lane groups' partial sums over b are added at the end.  One block per
-// nonempty tile (heavy tiles first), four waves, each taking one group of 16
-// sorted nodes per batch; a plane that no node of the group reaches is skipped
-// by a uniform branch (sorted nodes of a tile share their plane offset most of
-// the time).  Nodes found outside their tile (bins older than x) are gathered
-// straight from the grid by the thread that evaluated their windows.
+// lane groups' partial sums over b are added at the end.
+//
+// One block per nonempty tile (heavy tiles first).  After the subgrid is staged
+// in LDS the four waves work independently, each on every fourth group of 16
+// sorted nodes: lane group t < 3 of a wave evaluates (or reads from the
+// precomputed tables) the axis-t windows of the group's nodes and writes them
+// as zero-padded rows into the wave's LDS rows, while the loads for the wave's
+// next group are in flight during the matrix-core phase.  A plane that no node
+// of the group reaches is skipped by a uniform branch (sorted nodes of a tile
+// share their plane offset most of the time).  Nodes found outside their tile
+// (bins older than x) are gathered straight from the grid.
 // ---------------------------------------------------------------------------
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
-// position of c in a subgrid row or z-window row: c = 4 kc + k sits at 4 k + kc,
-// so lane group k reads its four K-slices (kc = 0..3) as one float4
-__device__ __forceinline__ int cpos(int c) { return ((c & 3) << 2) | (c >> 2); }
+// LDS rows of 16 floats, swizzled by the row's index r among 16 so that every
+// float4 / scalar read of the matrix-core phase is free of bank conflicts:
+//   subgrid row b (and z row of node r), element c -> chunk (c & 3) ^ sw(r), entry c >> 2
+//     (lane group k reads the chunk holding the K-slices c = 4 kc + k, kc = 0..3)
+//   y row of node r, element b -> chunk (b >> 2) ^ sw(r), entry b & 3
+//     (lane group k reads the rows b = 4 k + e of its accumulator)
+//   x row of node r, element a -> a ^ 4 sw(r)
+__device__ __forceinline__ int sw4(int r) { return (r >> 2) & 3; }
+__device__ __forceinline__ int kpos(int r, int c) { return (((c & 3) ^ sw4(r)) << 2) | (c >> 2); }
+__device__ __forceinline__ int bpos(int r, int b) { return (((b >> 2) ^ sw4(r)) << 2) | (b & 3); }
+__device__ __forceinline__ int apos(int r, int a) { return a ^ (sw4(r) << 2); }
+
+// LDS writes of one lane become visible to the other lanes of its wave
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int MT, int MODE>
 struct GatherMfmaShape {
   static constexpr int CM = 2 * MT + 1, TS = 8, P = TS + 2 * MT;
   static constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
   static constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
-  static constexpr int NB = 64;  // nodes per batch: one group of 16 per wave
-  static constexpr int GP = 20;  // row pitch (floats): rows b = 0..15 start in distinct bank quads
   static constexpr int ROWS = WG ? 6 : 3;  // window rows per node: x, y, z (and x', y', z')
-  static constexpr size_t LDS = sizeof(float) * (2 * P * 16 * GP + ROWS * NB * 16) + sizeof(int) * NB;
+  static constexpr int WAVES = WG ? 2 : 3;  // waves per SIMD the kernel is compiled for
+  static constexpr size_t LDS = sizeof(float) * (2 * P * 256 + 4 * ROWS * 256);
+};
+
+// one lane's share of a node's window data: axis lk of node li of a group
+template <int CM, bool WG>
+struct AxisLoad {
+  int j;     // node id, -1: no node
+  float x;   // coordinate now ...
+  float px;  // ... and when the tables were made
+  float psi[CM];
+  float dpsi[WG ? CM : 1];
 };
 
 template <int MT, int MODE>
-__global__ __launch_bounds__(256, 2) void gather_mfma_k(BArgs<float> A) {
+__global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gather_mfma_k(BArgs<float> A) {
   using S = GatherMfmaShape<MT, MODE>;
-  constexpr int CM = S::CM, TS = S::TS, P = S::P, NB = S::NB, GP = S::GP;
+  constexpr int CM = S::CM, TS = S::TS, P = S::P;
   constexpr bool WG = S::WG;
   static_assert(P <= 16, "one 16 x 16 block per plane");
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  float* s_gr = reinterpret_cast<float*>(smem_raw);  // [P][16][GP], G by (a, b, cpos(c))
-  float* s_gi = s_gr + P * 16 * GP;
-  float* s_x = s_gi + P * 16 * GP;  // [NB][16] window rows (z by cpos), zero outside the stencil
-  float* s_y = s_x + NB * 16;
-  float* s_z = s_y + NB * 16;
-  float* s_dx = s_z + NB * 16;  // gradient rows (WG only)
-  float* s_dy = s_dx + NB * 16;
-  float* s_dz = s_dy + NB * 16;
-  int* s_j = reinterpret_cast<int*>(s_x + S::ROWS * NB * 16);
+  float* s_gr = reinterpret_cast<float*>(smem_raw);  // [P][16 rows b][16], element c at kpos(b, c)
+  float* s_gi = s_gr + P * 256;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  // this wave's window rows: row r (0..2: x, y, z; 3..5: x', y', z') of node i at (r * 16 + i) * 16
+  float* rows = s_gi + P * 256 + wave * S::ROWS * 256;
 
   const int tile = A.torder[blockIdx.x];
   const int t2 = tile % A.ntile[2];
@@ -351,12 +377,10 @@ __global__ __launch_bounds__(256, 2) void gather_mfma_k(BArgs<float> A) {
   const int o0 = t0 * TS, o1 = t1 * TS, o2 = t2 * TS;
   const unsigned n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
   const int beg = A.tile_start[tile], end = A.tile_start[tile + 1];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int li = lane & 15, lk = lane >> 4;
 
   // stage the subgrid, split into real and imaginary parts; pairs of cells along c
   if constexpr (P < 16) {
-    for (int q = tid; q < 2 * P * 16 * GP / 4; q += 256)
+    for (int q = tid; q < 2 * P * 256 / 4; q += 256)
       reinterpret_cast<float4*>(s_gr)[q] = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
   }
@@ -379,139 +403,222 @@ __global__ __launch_bounds__(256, 2) void gather_mfma_k(BArgs<float> A) {
       const int q = tid + 256 * k;
       if (q < NPAIR) {
         const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
-        const int r = (cx * 16 + cy) * GP;
-        s_gr[r + cpos(2 * hz)] = v[k].x;
-        s_gi[r + cpos(2 * hz)] = v[k].y;
-        s_gr[r + cpos(2 * hz + 1)] = v[k].z;
-        s_gi[r + cpos(2 * hz + 1)] = v[k].w;
+        const int r = (cx * 16 + cy) * 16;
+        s_gr[r + kpos(cy, 2 * hz)] = v[k].x;
+        s_gi[r + kpos(cy, 2 * hz)] = v[k].y;
+        s_gr[r + kpos(cy, 2 * hz + 1)] = v[k].z;
+        s_gi[r + kpos(cy, 2 * hz + 1)] = v[k].w;
       }
     }
   }
+  __syncthreads();
 
-  for (int base = beg; base < end; base += NB) {
-    const int nb = min(NB, end - base);
-    if (tid < NB) {
+  const int ngroups = (end - beg + 15) >> 4;
+  int g = wave;
+  if (g >= ngroups) return;  // no block barrier follows
+
+  // this lane's axis (lane group 3 holds none)
+  const int ax = lk < 3 ? lk : 0;
+  const int nt = ax == 0 ? (int)n0 : (ax == 1 ? (int)n1 : (int)n2);
+  const int ot = ax == 0 ? o0 : (ax == 1 ? o1 : o2);
+  const float bt = (float)(ax == 0 ? A.g.b[0] : (ax == 1 ? A.g.b[1] : A.g.b[2]));
+  const float wst = (float)(ax == 0 ? A.g.ws[0] : (ax == 1 ? A.g.ws[1] : A.g.ws[2]));
+  const long M = A.M;
+  const bool tab = A.pre_psi != nullptr && (!WG || A.pre_dpsi != nullptr);
+
+  auto perm_of = [&](int gg) {
+    const int q = beg + 16 * gg + li;
+    return q < end ? A.perm[q] : -1;
+  };
+  auto load_group = [&](int gg, int j, AxisLoad<CM, WG>& L) {
+    L.j = j;
+    if (j >= 0 && lk < 3) {
+      const long q = beg + 16 * gg + li;
+      L.x = A.x[3 * (long)j + ax];
+      if (tab) {
+        L.px = A.pre_x[(long)ax * M + q];
+#pragma unroll
+        for (int s = 0; s < CM; s++) L.psi[s] = A.pre_psi[(long)(ax * CM + s) * M + q];
+        if (WG)
+#pragma unroll
+          for (int s = 0; s < CM; s++) L.dpsi[s] = A.pre_dpsi[(long)(ax * CM + s) * M + q];
+      }
+    }
+  };
+
+  AxisLoad<CM, WG> L;
+  load_group(g, perm_of(g), L);
+  int jn = g + 4 < ngroups ? perm_of(g + 4) : -1;
+
+  for (; g < ngroups; g += 4) {
+    // windows of this lane's axis for node li of the group
+    const int jg = L.j;
+    float w[CM], dw[CM];
+    int l = 0;
+    bool bad = false;
+    if (lk < 3 && jg >= 0) {
+      const NodeAxis<float> na = node_axis<float>(L.x, nt, MT);
+      l = local_off(na.start, ot, TS, nt);
+      bad = l < 0;
+      if (tab && L.x == L.px) {
+#pragma unroll
+        for (int s = 0; s < CM; s++) {
+          w[s] = L.psi[s];
+          dw[s] = WG ? L.dpsi[s] : 0.f;
+        }
+      } else {
+        window_axis_s<float, CM, WG>(A.g.win, na.frac, (float)nt, bt, w, dw);
+#pragma unroll
+        for (int s = 0; s < CM; s++) {
+          w[s] *= wst;
+          if (WG) dw[s] *= wst;
+        }
+      }
+    }
+    const unsigned long long badm = __ballot(bad);
+    const unsigned stale = (unsigned)(badm | (badm >> 16) | (badm >> 32)) & 0xFFFFu;
+    const bool ok = jg >= 0 && !((stale >> li) & 1u);
+    if (lk < 3) {
+      float* r0 = rows + (lk * 16 + li) * 16;
+      float* r1 = rows + ((3 + lk) * 16 + li) * 16;
       const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-      for (int r = 0; r < S::ROWS; r++)
+      for (int e = 0; e < 4; e++) {
+        reinterpret_cast<float4*>(r0)[e] = z4;
+        if (WG) reinterpret_cast<float4*>(r1)[e] = z4;
+      }
+      if (ok) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) reinterpret_cast<float4*>(s_x + (r * NB + tid) * 16)[e] = z4;
-      s_j[tid] = -1;
-      if (tid < nb) {
-        const int p = base + tid;
-        const long j = A.perm[p];
-        NodeWin<float, CM> w;
-        node_windows<float, MT, CM, WG>(A, p, j, MT, w);
-        const int l0 = local_off(w.start[0], o0, TS, (int)n0);
-        const int l1 = local_off(w.start[1], o1, TS, (int)n1);
-        const int l2 = local_off(w.start[2], o2, TS, (int)n2);
-        if (l0 >= 0 && l1 >= 0 && l2 >= 0) {
-#pragma unroll
-          for (int s = 0; s < CM; s++) {
-            s_x[tid * 16 + l0 + s] = w.px[s];
-            s_y[tid * 16 + l1 + s] = w.py[s];
-            s_z[tid * 16 + cpos(l2 + s)] = w.pz[s];
-            if (WG) {
-              s_dx[tid * 16 + l0 + s] = w.dx[s];
-              s_dy[tid * 16 + l1 + s] = w.dy[s];
-              s_dz[tid * 16 + cpos(l2 + s)] = w.dz[s];
-            }
-          }
-          s_j[tid] = (int)j;
-        } else {
-          gather_node_global<float, CM, MODE, true>(A, j, w, CM);
+        for (int s = 0; s < CM; s++) {
+          const int v = l + s;
+          const int pos = lk == 0 ? apos(li, v) : (lk == 1 ? bpos(li, v) : kpos(li, v));
+          r0[pos] = w[s];
+          if (WG) r1[pos] = dw[s];
         }
       }
     }
-    __syncthreads();
-    if (16 * wave < nb) {
-      const int node = 16 * wave + li;
-      const float4 zc = reinterpret_cast<const float4*>(s_z + node * 16)[lk];  // Z_j[4 kc + lk], kc = 0..3
-      const float4 yb = reinterpret_cast<const float4*>(s_y + node * 16)[lk];  // Y_j[4 lk + e], e = 0..3
-      float4 dzc = zc, dyb = yb;
+    wave_sync();
+
+    // loads for the wave's next group, in flight during the matrix-core phase
+    AxisLoad<CM, WG> Ln;
+    Ln.j = -1;
+    if (g + 4 < ngroups) {
+      load_group(g + 4, jn, Ln);
+      jn = g + 8 < ngroups ? perm_of(g + 8) : -1;
+    }
+
+    const float* xr = rows + li * 16;
+    const float* dxr = rows + (48 + li) * 16;
+    const int ch = 4 * (lk ^ sw4(li));
+    const float4 yb = *reinterpret_cast<const float4*>(rows + (16 + li) * 16 + ch);  // Y_j[4 lk + e]
+    const float4 zc = *reinterpret_cast<const float4*>(rows + (32 + li) * 16 + ch);  // Z_j[4 kc + lk]
+    float4 dyb = yb, dzc = zc;
+    if (WG) {
+      dyb = *reinterpret_cast<const float4*>(rows + (64 + li) * 16 + ch);
+      dzc = *reinterpret_cast<const float4*>(rows + (80 + li) * 16 + ch);
+    }
+    float fr = 0.f, fi = 0.f, g0r = 0.f, g0i = 0.f, g1r = 0.f, g1i = 0.f, g2r = 0.f, g2i = 0.f;
+#pragma unroll 1
+    for (int a = 0; a < P; a++) {
+      const float xa = xr[apos(li, a)];
+      const float dxa = WG ? dxr[apos(li, a)] : 0.f;
+      if (!__any(xa != 0.f || dxa != 0.f)) continue;
+      const float4 gr = *reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * 16 + ch);
+      const float4 gi = *reinterpret_cast<const float4*>(s_gi + (a * 16 + li) * 16 + ch);
+      mfma_f4 tr = {0.f, 0.f, 0.f, 0.f}, ti = tr;
+      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.x, zc.x, tr, 0, 0, 0);
+      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.x, zc.x, ti, 0, 0, 0);
+      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.y, zc.y, tr, 0, 0, 0);
+      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.y, zc.y, ti, 0, 0, 0);
+      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.z, zc.z, tr, 0, 0, 0);
+      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.z, zc.z, ti, 0, 0, 0);
+      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.w, zc.w, tr, 0, 0, 0);
+      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.w, zc.w, ti, 0, 0, 0);
+      const float sr = yb.x * tr[0] + yb.y * tr[1] + yb.z * tr[2] + yb.w * tr[3];
+      const float si = yb.x * ti[0] + yb.y * ti[1] + yb.z * ti[2] + yb.w * ti[3];
+      fr += xa * sr;
+      fi += xa * si;
       if (WG) {
-        dzc = reinterpret_cast<const float4*>(s_dz + node * 16)[lk];
-        dyb = reinterpret_cast<const float4*>(s_dy + node * 16)[lk];
+        mfma_f4 ur = {0.f, 0.f, 0.f, 0.f}, ui = ur;
+        ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.x, dzc.x, ur, 0, 0, 0);
+        ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.x, dzc.x, ui, 0, 0, 0);
+        ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.y, dzc.y, ur, 0, 0, 0);
+        ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.y, dzc.y, ui, 0, 0, 0);
+        ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.z, dzc.z, ur, 0, 0, 0);
+        ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.z, dzc.z, ui, 0, 0, 0);
+        ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.w, dzc.w, ur, 0, 0, 0);
+        ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.w, dzc.w, ui, 0, 0, 0);
+        const float syr = dyb.x * tr[0] + dyb.y * tr[1] + dyb.z * tr[2] + dyb.w * tr[3];
+        const float syi = dyb.x * ti[0] + dyb.y * ti[1] + dyb.z * ti[2] + dyb.w * ti[3];
+        const float szr = yb.x * ur[0] + yb.y * ur[1] + yb.z * ur[2] + yb.w * ur[3];
+        const float szi = yb.x * ui[0] + yb.y * ui[1] + yb.z * ui[2] + yb.w * ui[3];
+        g0r += dxa * sr;
+        g0i += dxa * si;
+        g1r += xa * syr;
+        g1i += xa * syi;
+        g2r += xa * szr;
+        g2i += xa * szi;
       }
-      float fr = 0.f, fi = 0.f, g0r = 0.f, g0i = 0.f, g1r = 0.f, g1i = 0.f, g2r = 0.f, g2i = 0.f;
-      for (int a = 0; a < P; a++) {
-        const float xa = s_x[node * 16 + a];
-        const float dxa = WG ? s_dx[node * 16 + a] : 0.f;
-        if (!__any(xa != 0.f || dxa != 0.f)) continue;
-        const float4 gr = reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * GP)[lk];
-        const float4 gi = reinterpret_cast<const float4*>(s_gi + (a * 16 + li) * GP)[lk];
-        mfma_f4 tr = {0.f, 0.f, 0.f, 0.f}, ti = tr;
-        tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.x, zc.x, tr, 0, 0, 0);
-        ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.x, zc.x, ti, 0, 0, 0);
-        tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.y, zc.y, tr, 0, 0, 0);
-        ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.y, zc.y, ti, 0, 0, 0);
-        tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.z, zc.z, tr, 0, 0, 0);
-        ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.z, zc.z, ti, 0, 0, 0);
-        tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.w, zc.w, tr, 0, 0, 0);
-        ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.w, zc.w, ti, 0, 0, 0);
-        const float sr = yb.x * tr[0] + yb.y * tr[1] + yb.z * tr[2] + yb.w * tr[3];
-        const float si = yb.x * ti[0] + yb.y * ti[1] + yb.z * ti[2] + yb.w * ti[3];
-        fr += xa * sr;
-        fi += xa * si;
-        if (WG) {
-          mfma_f4 ur = {0.f, 0.f, 0.f, 0.f}, ui = ur;
-          ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.x, dzc.x, ur, 0, 0, 0);
-          ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.x, dzc.x, ui, 0, 0, 0);
-          ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.y, dzc.y, ur, 0, 0, 0);
-          ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.y, dzc.y, ui, 0, 0, 0);
-          ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.z, dzc.z, ur, 0, 0, 0);
-          ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.z, dzc.z, ui, 0, 0, 0);
-          ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.w, dzc.w, ur, 0, 0, 0);
-          ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.w, dzc.w, ui, 0, 0, 0);
-          const float syr = dyb.x * tr[0] + dyb.y * tr[1] + dyb.z * tr[2] + dyb.w * tr[3];
-          const float syi = dyb.x * ti[0] + dyb.y * ti[1] + dyb.z * ti[2] + dyb.w * ti[3];
-          const float szr = yb.x * ur[0] + yb.y * ur[1] + yb.z * ur[2] + yb.w * ur[3];
-          const float szi = yb.x * ui[0] + yb.y * ui[1] + yb.z * ui[2] + yb.w * ui[3];
-          g0r += dxa * sr;
-          g0i += dxa * si;
-          g1r += xa * syr;
-          g1i += xa * syi;
-          g2r += xa * szr;
-          g2i += xa * szi;
-        }
-      }
-      // sum over the four lane groups (rows b = 4 lk + e)
-      auto red = [](float v) {
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        return v;
-      };
-      fr = red(fr);
-      fi = red(fi);
-      if (WG) {
-        g0r = red(g0r);
-        g0i = red(g0i);
-        g1r = red(g1r);
-        g1i = red(g1i);
-        g2r = red(g2r);
-        g2i = red(g2i);
-      }
-      const int j = s_j[node];
-      if (lk == 0 && j >= 0) {
+    }
+    // sum over the four lane groups (rows b = 4 lk + e)
+    auto red = [](float v) {
+      v += __shfl_xor(v, 16);
+      v += __shfl_xor(v, 32);
+      return v;
+    };
+    fr = red(fr);
+    fi = red(fi);
+    if (WG) {
+      g0r = red(g0r);
+      g0i = red(g0i);
+      g1r = red(g1r);
+      g1i = red(g1i);
+      g2r = red(g2r);
+      g2i = red(g2i);
+    }
+    if (lk == 0 && jg >= 0) {
+      if (ok) {
         Cplx<float> v[4] = {cmk<float>(fr, fi), cmk<float>(g0r, g0i), cmk<float>(g1r, g1i), cmk<float>(g2r, g2i)};
-        gather_store<float, MODE>(A, j, v);
+        gather_store<float, MODE>(A, jg, v);
+      } else {
+        A.stale_list[atomicAdd(A.stale_count, 1u)] = beg + 16 * g + li;  // gathered by gather_stale_k
       }
     }
-    __syncthreads();
+    L = Ln;
+    wave_sync();
+  }
+}
+
+__global__ void zero_u32_k(unsigned* c) {
+  if (threadIdx.x == 0) *c = 0u;
+}
+
+// nodes the matrix-core gather listed as outside their tile: gathered straight
+// from the grid
+template <int MT, int MODE>
+__global__ __launch_bounds__(256) void gather_stale_k(BArgs<float> A) {
+  constexpr int CM = 2 * MT + 1;
+  constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
+  const unsigned count = *A.stale_count;
+  for (unsigned k = blockIdx.x * 256 + threadIdx.x; k < count; k += gridDim.x * 256) {
+    const long j = A.perm[A.stale_list[k]];
+    NodeWin<float, CM> w;
+    node_windows<float, MT, CM, WG>(A, -1, j, MT, w);
+    gather_node_global<float, CM, MODE, true>(A, j, w, CM);
   }
 }
 
 inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
-// the matrix-core gather is used for FP32 f with m <= 4 on the standard tile
-// layout, once the nonempty tiles are listed (the gradient modes measured slower
-// than the vector kernel so far)
+// the matrix-core gather applies to FP32 f / grad / f + grad with m <= 4 on the
+// standard tile layout, once the nonempty tiles are listed
 template <class R, int MT, int MODE>
 bool mfma_gather(const BArgs<R>& A) {
-  if constexpr (sizeof(R) != 4 || MT > 4 || MODE != B_F) {
+  if constexpr (sizeof(R) != 4 || MT > 4 || MODE > B_FGRAD) {
     return false;
   } else {
-    return !A.no_mfma && A.torder && A.n_torder > 0 && A.ts[0] == 8 && A.ts[1] == 8 && A.ts[2] == 8;
+    return !A.no_mfma && A.torder && A.n_torder > 0 && A.stale_count && A.stale_list && A.ts[0] == 8 &&
+           A.ts[1] == 8 && A.ts[2] == 8;
   }
 }
 
@@ -521,9 +628,11 @@ void launch_gather_mfma(const BArgs<R>& A, hipStream_t s) {
     using S = GatherMfmaShape<MT, MODE>;
     (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)S::LDS);
+    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, A.stale_count);
     hipLaunchKernelGGL((gather_mfma_k<MT, MODE>), dim3((unsigned)A.n_torder), dim3(256), S::LDS, s, A);
     if (A.mid) (void)hipEventRecord(A.mid, s);
     if (A.mid2) (void)hipEventRecord(A.mid2, s);
+    hipLaunchKernelGGL((gather_stale_k<MT, MODE>), dim3(256), dim3(256), 0, s, A);
   }
 }
 
