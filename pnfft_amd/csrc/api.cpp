@@ -591,6 +591,15 @@ static bool cols_enabled() {
   }();
   return v;
 }
+// PNFFT_AMD_TABLES=0: precompute_psi bins the nodes but tabulates no windows
+// (the B kernels evaluate them), for comparison
+static bool tables_enabled() {
+  static const bool v = [] {
+    const char* e = getenv("PNFFT_AMD_TABLES");
+    return !(e && strcmp(e, "0") == 0);
+  }();
+  return v;
+}
 static bool mfma_disabled() {
   static const bool v = [] {
     const char* e = getenv("PNFFT_AMD_MFMA");
@@ -985,7 +994,7 @@ static void precompute_impl(PlanT<R>* P, NodesT<R>* D, unsigned flags) {
   R* dx = stage_in<R>(D->x, D->d_x, 3 * D->M, true, P->stream, &host_x);
   if (!dx) return;
   if (bin(P, D, dx) != kOk) return;
-  if (!host_x && P->m >= 2 && P->m <= 8) {
+  if (!host_x && P->m >= 2 && P->m <= 8 && tables_enabled()) {
     const size_t M = (size_t)D->M, cm = 2 * (size_t)P->m + 1;
     const bool want_grad = (flags & fl::PRE_GRAD_PSI) != 0;
     R* psi = grow(D->d_pre_psi, D->pre_psi_cap, 3 * cm * M);

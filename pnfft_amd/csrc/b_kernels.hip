@@ -426,29 +426,33 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
   const long M = A.M;
   const bool tab = A.pre_psi != nullptr && (!WG || A.pre_dpsi != nullptr);
 
-  auto perm_of = [&](int gg) {
-    const int q = beg + 16 * gg + li;
-    return q < end ? A.perm[q] : -1;
-  };
+  // The loads below are unconditional, from clamped positions (their values
+  // are discarded where no node exists), so that no register copy of a loaded
+  // value is needed before the matrix-core phase; such a copy would make the
+  // wave wait for every load in flight.
+  auto perm_raw = [&](int gg) { return A.perm[min(beg + 16 * gg + li, end - 1)]; };
+  auto perm_ok = [&](int gg) { return beg + 16 * gg + li < end; };
   auto load_group = [&](int gg, int j, AxisLoad<CM, WG>& L) {
+    const long q = min(beg + 16 * gg + li, end - 1);
     L.j = j;
-    if (j >= 0 && lk < 3) {
-      const long q = beg + 16 * gg + li;
-      L.x = A.x[3 * (long)j + ax];
-      if (tab) {
-        L.px = A.pre_x[(long)ax * M + q];
+    L.x = A.x[3 * (long)max(j, 0) + ax];
+    if (tab) {
+      L.px = A.pre_x[(long)ax * M + q];
 #pragma unroll
-        for (int s = 0; s < CM; s++) L.psi[s] = A.pre_psi[(long)(ax * CM + s) * M + q];
-        if (WG)
+      for (int s = 0; s < CM; s++) L.psi[s] = A.pre_psi[(long)(ax * CM + s) * M + q];
+      if (WG)
 #pragma unroll
-          for (int s = 0; s < CM; s++) L.dpsi[s] = A.pre_dpsi[(long)(ax * CM + s) * M + q];
-      }
+        for (int s = 0; s < CM; s++) L.dpsi[s] = A.pre_dpsi[(long)(ax * CM + s) * M + q];
     }
   };
 
   AxisLoad<CM, WG> L;
-  load_group(g, perm_of(g), L);
-  int jn = g + 4 < ngroups ? perm_of(g + 4) : -1;
+  {
+    const int j0 = perm_raw(g);
+    load_group(g, perm_ok(g) ? j0 : -1, L);
+  }
+  int jn = perm_raw(g + 4 < ngroups ? g + 4 : g);
+  bool jn_ok = g + 4 < ngroups && perm_ok(g + 4);
 
   for (; g < ngroups; g += 4) {
     // windows of this lane's axis for node li of the group
@@ -478,6 +482,13 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
     const unsigned long long badm = __ballot(bad);
     const unsigned stale = (unsigned)(badm | (badm >> 16) | (badm >> 32)) & 0xFFFFu;
     const bool ok = jg >= 0 && !((stale >> li) & 1u);
+    // planes the group's stencils reach: OR of the x lanes' plane masks
+    unsigned pm = (lk == 0 && ok) ? (((1u << CM) - 1u) << l) : 0u;
+    pm |= __shfl_xor(pm, 1);
+    pm |= __shfl_xor(pm, 2);
+    pm |= __shfl_xor(pm, 4);
+    pm |= __shfl_xor(pm, 8);
+    pm = __builtin_amdgcn_readfirstlane(pm);
     if (lk < 3) {
       float* r0 = rows + (lk * 16 + li) * 16;
       float* r1 = rows + ((3 + lk) * 16 + li) * 16;
@@ -501,11 +512,10 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
 
     // loads for the wave's next group, in flight during the matrix-core phase
     AxisLoad<CM, WG> Ln;
-    Ln.j = -1;
-    if (g + 4 < ngroups) {
-      load_group(g + 4, jn, Ln);
-      jn = g + 8 < ngroups ? perm_of(g + 8) : -1;
-    }
+    const int gn = g + 4 < ngroups ? g + 4 : g;
+    load_group(gn, jn_ok ? jn : -1, Ln);
+    const int jnn = perm_raw(g + 8 < ngroups ? g + 8 : g);
+    const bool jnn_ok = g + 8 < ngroups && perm_ok(g + 8);
 
     const float* xr = rows + li * 16;
     const float* dxr = rows + (48 + li) * 16;
@@ -518,13 +528,21 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
       dzc = *reinterpret_cast<const float4*>(rows + (80 + li) * 16 + ch);
     }
     float fr = 0.f, fi = 0.f, g0r = 0.f, g0i = 0.f, g1r = 0.f, g1i = 0.f, g2r = 0.f, g2i = 0.f;
-#pragma unroll 1
-    for (int a = 0; a < P; a++) {
-      const float xa = xr[apos(li, a)];
-      const float dxa = WG ? dxr[apos(li, a)] : 0.f;
-      if (!__any(xa != 0.f || dxa != 0.f)) continue;
-      const float4 gr = *reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * 16 + ch);
-      const float4 gi = *reinterpret_cast<const float4*>(s_gi + (a * 16 + li) * 16 + ch);
+    const int amin = pm ? __builtin_ctz(pm) : 0, amax = pm ? 31 - __builtin_clz(pm) : -1;
+    // plane a + 1's operands are read while plane a is on the matrix cores
+    auto ld_plane = [&](int a, float4& gr, float4& gi, float& xa, float& dxa) {
+      gr = *reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * 16 + ch);
+      gi = *reinterpret_cast<const float4*>(s_gi + (a * 16 + li) * 16 + ch);
+      xa = xr[apos(li, a)];
+      dxa = WG ? dxr[apos(li, a)] : 0.f;
+    };
+    float4 gr, gi;
+    float xa, dxa;
+    ld_plane(amax >= amin ? amin : 0, gr, gi, xa, dxa);
+    for (int a = amin; a <= amax; a++) {
+      float4 grn, gin;
+      float xan, dxan;
+      ld_plane(a < amax ? a + 1 : a, grn, gin, xan, dxan);
       mfma_f4 tr = {0.f, 0.f, 0.f, 0.f}, ti = tr;
       tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.x, zc.x, tr, 0, 0, 0);
       ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.x, zc.x, ti, 0, 0, 0);
@@ -559,6 +577,10 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
         g2r += xa * szr;
         g2i += xa * szi;
       }
+      gr = grn;
+      gi = gin;
+      xa = xan;
+      dxa = dxan;
     }
     // sum over the four lane groups (rows b = 4 lk + e)
     auto red = [](float v) {
@@ -585,6 +607,9 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
       }
     }
     L = Ln;
+    if (g + 4 >= ngroups) L.j = -1;
+    jn = jnn;
+    jn_ok = jnn_ok;
     wave_sync();
   }
 }

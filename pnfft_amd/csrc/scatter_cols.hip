@@ -233,6 +233,19 @@ struct MfmaShape {
 // row position of plane a: wave (a & 3) reads the float4 at 4 (a & 3), entry a >> 2
 __device__ __forceinline__ int xpos(int a) { return ((a & 3) << 2) | (a >> 2); }
 
+// one thread's share of a node's data for the rows phase: axis `role` (0..2)
+// or the complex weights (role 3)
+template <int CM, bool WF, bool WG>
+struct ScatterLoad {
+  int j;     // node id, -1: no node
+  float x;   // coordinate now ...
+  float px;  // ... and when the tables were made
+  float psi[CM];
+  float dpsi[WG ? CM : 1];
+  Cplx<float> f;
+  Cplx<float> g[WG ? 3 : 1];
+};
+
 template <int MT, int MODE>
 __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(BArgs<float> A) {
   using S = ColsShape<float, MT, MODE>;
@@ -240,10 +253,12 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
   constexpr int CM = S::CM, P = S::P, TS = S::TS, V = MS::V, NB = MS::NB, NV = MS::NV;
   constexpr bool WF = S::WF, WG = S::WG;
   static_assert(P <= 16, "one 16 x 16 block per plane");
+  static_assert(NB == 64, "one node per lane of a wave in the rows phase");
   __shared__ __align__(16) float s_x[NV][16];  // X_v by xpos
   __shared__ __align__(16) float s_y[NV][16];  // Y_v[b], zero outside the stencil
   __shared__ __align__(16) float s_z[NV][16];  // Z_v[c]
   __shared__ float2 s_w[NV];                   // W_v
+  __shared__ unsigned char s_bad[3][NB];       // node outside the tile on axis t
 
   const int tile = A.torder[blockIdx.x];
   const int t2 = tile % A.ntile[2];
@@ -254,63 +269,122 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
   const int beg = A.tile_start[tile], end = A.tile_start[tile + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
+  // rows phase: thread (role, q) handles node q of the batch; roles 0..2 the
+  // window of axis `role`, role 3 the weights
+  const int q = lane, role = wave;
+  const int ax = role < 3 ? role : 0;
+  const int nt = ax == 0 ? n0 : (ax == 1 ? n1 : n2);
+  const int ot = ax == 0 ? o0 : (ax == 1 ? o1 : o2);
+  const float bt = (float)(ax == 0 ? A.g.b[0] : (ax == 1 ? A.g.b[1] : A.g.b[2]));
+  const float wst = (float)(ax == 0 ? A.g.ws[0] : (ax == 1 ? A.g.ws[1] : A.g.ws[2]));
+  float(*srow)[16] = ax == 0 ? s_x : (ax == 1 ? s_y : s_z);
+  const long M = A.M;
+  const bool tab = A.pre_psi != nullptr && (!WG || A.pre_dpsi != nullptr);
+
+  // The loads below are unconditional, from clamped positions (their values
+  // are discarded where no node exists), so that no register copy of a loaded
+  // value is needed before the matrix-core phase; such a copy would make the
+  // wave wait for every load in flight.
+  auto perm_raw = [&](int base) { return A.perm[min(base + q, end - 1)]; };
+  auto perm_ok = [&](int base) { return base + q < end; };
+  auto load = [&](int base, int j, ScatterLoad<CM, WF, WG>& L) {
+    L.j = j;
+    const int jj = max(j, 0);
+    if (role < 3) {
+      L.x = A.x[3 * (long)jj + ax];
+      if (tab) {
+        const long p = min(base + q, end - 1);
+        L.px = A.pre_x[(long)ax * M + p];
+#pragma unroll
+        for (int s = 0; s < CM; s++) L.psi[s] = A.pre_psi[(long)(ax * CM + s) * M + p];
+        if (WG)
+#pragma unroll
+          for (int s = 0; s < CM; s++) L.dpsi[s] = A.pre_dpsi[(long)(ax * CM + s) * M + p];
+      }
+    } else {
+      if (WF) L.f = load_f(A, jj);
+      if (WG)
+#pragma unroll
+        for (int t = 0; t < 3; t++) L.g[t] = load_g(A, jj, t);
+    }
+  };
 
   mfma_f4 cre[4], cim[4];  // plane wave + 4q: D[b = 4 lk + e][c = li]
   double tre[4][4], tim[4][4];
 #pragma unroll
-  for (int q = 0; q < 4; q++)
+  for (int qq = 0; qq < 4; qq++)
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      cre[q][e] = cim[q][e] = 0.f;
-      tre[q][e] = tim[q][e] = 0.0;
+      cre[qq][e] = cim[qq][e] = 0.f;
+      tre[qq][e] = tim[qq][e] = 0.0;
     }
+
+  ScatterLoad<CM, WF, WG> L;
+  {
+    const int j0 = perm_raw(beg);
+    load(beg, perm_ok(beg) ? j0 : -1, L);
+  }
+  int jn = perm_raw(beg + NB < end ? beg + NB : beg);
+  bool jn_ok = perm_ok(beg + NB);
 
   for (int base = beg; base < end; base += NB) {
     const int nb = min(NB, end - base);
-    if (tid < NB) {
+    // rows phase
+    if (role < 3) {
+      float w[CM], dw[CM];
+      int l = 0;
+      bool bad = false;
+      if (L.j >= 0) {
+        const NodeAxis<float> na = node_axis<float>(L.x, nt, MT);
+        l = local_off(na.start, ot, TS, nt);
+        bad = l < 0;
+        if (tab && L.x == L.px) {
+#pragma unroll
+          for (int s = 0; s < CM; s++) {
+            w[s] = L.psi[s];
+            dw[s] = WG ? L.dpsi[s] : 0.f;
+          }
+        } else {
+          window_axis_s<float, CM, WG>(A.g.win, na.frac, (float)nt, bt, w, dw);
+#pragma unroll
+          for (int s = 0; s < CM; s++) {
+            w[s] *= wst;
+            if (WG) dw[s] *= wst;
+          }
+        }
+      }
       const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
       for (int u = 0; u < V; u++) {
-        const int v = tid * V + u;
+        float* row = srow[q * V + u];
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          reinterpret_cast<float4*>(s_x[v])[e] = z4;
-          reinterpret_cast<float4*>(s_y[v])[e] = z4;
-          reinterpret_cast<float4*>(s_z[v])[e] = z4;
-        }
-        s_w[v] = make_float2(0.f, 0.f);
-      }
-      if (tid < nb) {
-        const long j = A.perm[base + tid];
-        NodeWin<float, CM> w;
-        node_windows<float, MT, CM, WG>(A, base + tid, j, MT, w);
-        const int l0 = local_off(w.start[0], o0, TS, n0);
-        const int l1 = local_off(w.start[1], o1, TS, n1);
-        const int l2 = local_off(w.start[2], o2, TS, n2);
-        if (l0 >= 0 && l1 >= 0 && l2 >= 0) {
-          // term u: (x, y, z) factors ψ or ψ' and its weight
-          auto put = [&](int v, const float(&X)[CM], const float(&Y)[CM], const float(&Z)[CM], Cplx<float> wv) {
+        for (int e = 0; e < 4; e++) reinterpret_cast<float4*>(row)[e] = z4;
+        if (L.j >= 0 && !bad) {
+          // term u differentiates axis ax when it is the gradient term of that axis
+          const bool d = WG && u == (WF ? 1 : 0) + ax;
 #pragma unroll
-            for (int s = 0; s < CM; s++) {
-              s_x[v][xpos(l0 + s)] = X[s];
-              s_y[v][l1 + s] = Y[s];
-              s_z[v][l2 + s] = Z[s];
-            }
-            s_w[v] = make_float2(wv.x, wv.y);
-          };
-          int v = tid * V;
-          if (WF) put(v++, w.px, w.py, w.pz, load_f(A, j));
-          if (WG) {
-            put(v++, w.dx, w.py, w.pz, load_g(A, j, 0));
-            put(v++, w.px, w.dy, w.pz, load_g(A, j, 1));
-            put(v++, w.px, w.py, w.dz, load_g(A, j, 2));
-          }
-        } else {
-          A.stale_list[atomicAdd(A.stale_count, 1u)] = base + tid;
+          for (int s = 0; s < CM; s++) row[ax == 0 ? xpos(l + s) : l + s] = d ? dw[s] : w[s];
         }
       }
+      s_bad[ax][q] = bad ? 1 : 0;
+    } else {
+      int v = q * V;
+      if (WF) s_w[v++] = L.j >= 0 ? make_float2(L.f.x, L.f.y) : make_float2(0.f, 0.f);
+      if (WG)
+#pragma unroll
+        for (int t = 0; t < 3; t++) s_w[v++] = L.j >= 0 ? make_float2(L.g[t].x, L.g[t].y) : make_float2(0.f, 0.f);
     }
     __syncthreads();
+    // a node outside its tile has a zero row on the axis where it left, so it
+    // adds nothing here; it is listed for the direct kernel
+    if (role == 3 && L.j >= 0 && (s_bad[0][q] | s_bad[1][q] | s_bad[2][q]))
+      A.stale_list[atomicAdd(A.stale_count, 1u)] = base + q;
+    // loads of the next batch, in flight during the matrix-core phase
+    ScatterLoad<CM, WF, WG> Ln;
+    const int bn = base + NB < end ? base + NB : base;
+    load(bn, jn_ok ? jn : -1, Ln);
+    const int jnn = perm_raw(base + 2 * NB < end ? base + 2 * NB : base);
+    const bool jnn_ok = perm_ok(base + 2 * NB);
     const int nq = (nb * V + 3) >> 2;
     for (int q4 = 0; q4 < nq; q4++) {
       const int v = 4 * q4 + lk;
@@ -320,33 +394,36 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
       const float bre = wv.x * z, bim = wv.y * z;
       const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        if (__any(xs[q] != 0.f)) {
-          const float a = xs[q] * y;
-          cre[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bre, cre[q], 0, 0, 0);
-          cim[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bim, cim[q], 0, 0, 0);
+      for (int qq = 0; qq < 4; qq++) {
+        if (__any(xs[qq] != 0.f)) {
+          const float a = xs[qq] * y;
+          cre[qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bre, cre[qq], 0, 0, 0);
+          cim[qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bim, cim[qq], 0, 0, 0);
         }
       }
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int qq = 0; qq < 4; qq++)
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        tre[q][e] += (double)cre[q][e];
-        tim[q][e] += (double)cim[q][e];
-        cre[q][e] = cim[q][e] = 0.f;
+        tre[qq][e] += (double)cre[qq][e];
+        tim[qq][e] += (double)cim[qq][e];
+        cre[qq][e] = cim[qq][e] = 0.f;
       }
     __syncthreads();
+    L = Ln;
+    jn = jnn;
+    jn_ok = jnn_ok;
   }
 
   Cplx<float>* slot = A.slots + (size_t)blockIdx.x * (size_t)(P * P * P);
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int a = wave + 4 * q;
+  for (int qq = 0; qq < 4; qq++) {
+    const int a = wave + 4 * qq;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       const int b = 4 * lk + e;
-      if (a < P && b < P && li < P) slot[(a * P + b) * P + li] = cmk<float>((float)tre[q][e], (float)tim[q][e]);
+      if (a < P && b < P && li < P) slot[(a * P + b) * P + li] = cmk<float>((float)tre[qq][e], (float)tim[qq][e]);
     }
   }
 }
