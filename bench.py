@@ -60,32 +60,113 @@ def algorithmic_bytes(w, precision, c2r=False):
     }
 
 
-def roofline_from_kernels(kt, ab, steps):
-    """Dominant kernel class of one step and its achieved HBM rate.
+def algorithmic_flops(w):
+    """Algorithmic flops of one launch of the B kernels (tensor-product evaluation, FMA = 2 flops):
+    (2m+1)^3 stencil points per node, 4 flops each for f (complex value times real weight) and 8
+    for the three gradient sums."""
+    c3 = (2 * w["m"] + 1) ** 3
+    M = w["M"]
+    return {"gather_f": 4 * c3 * M, "gather_grad": 8 * c3 * M, "scatter_f": 4 * c3 * M}
+
+
+# MI355X peaks: HBM3E 8 TB/s; dense FP32 matrix-core rate 157.3 TFLOP/s (FP64 78.6)
+PEAK_HBM_GBPS = 8000.0
+PEAK_MFMA_TFLOPS = {"float": 157.3, "double": 78.6}
+
+
+def roofline_from_kernels(kt, ab, af, steps, precision):
+    """Dominant kernel class of one step, with its binding roofline.
 
     kt: {class: (ms, launches)} over the timed steps (HIP events on the plan's stream).  One step
-    launches: FFT rows 2 forward + 1 backward, cols1 3, cols0 3; gather stage (tiled + direct
-    kernels) 2 (f, grad); scatter stage (tiled kernel, fold of the tile subgrids into the grid,
-    direct kernel for stale bins) 1.
+    launches: FFT rows 2 forward + 1 backward, cols1 3, cols0 3 (HBM-bound: GB/s of algorithmic
+    bytes); gather stage 2 (f, grad) and scatter stage 1 (subgrid kernel, fold, stale nodes), whose
+    FP32 kernels run on the matrix cores for m <= 4 (TFLOP/s of algorithmic flops).
     """
     def ms(k):
         return kt[k][0] / steps
 
+    mfma = precision == "float"
     classes = {
-        "fft_rows": (ms("fft_rows"), 2 * ab["fft_rows_fwd"] + ab["fft_rows_bwd"], 3),
-        "fft_cols1": (ms("fft_cols1"), 3 * ab["fft_cols1"], 3),
-        "fft_cols0": (ms("fft_cols0"), 3 * ab["fft_cols0"], 3),
-        "gather_tiled+direct": (ms("gather_tiled") + ms("gather_direct"), ab["gather_f"] + ab["gather_grad"], 2),
-        "scatter_tiled+fold+direct": (ms("scatter_tiled") + ms("scatter_fold") + ms("scatter_direct"),
-                                      ab["scatter_f"], 1),
+        "fft_rows": (ms("fft_rows"), 2 * ab["fft_rows_fwd"] + ab["fft_rows_bwd"], 0, 3),
+        "fft_cols1": (ms("fft_cols1"), 3 * ab["fft_cols1"], 0, 3),
+        "fft_cols0": (ms("fft_cols0"), 3 * ab["fft_cols0"], 0, 3),
+        "gather": (ms("gather_tiled") + ms("gather_direct"), ab["gather_f"] + ab["gather_grad"],
+                   af["gather_f"] + af["gather_grad"], 2),
+        "scatter": (ms("scatter_tiled") + ms("scatter_fold") + ms("scatter_direct"), ab["scatter_f"],
+                    af["scatter_f"], 1),
     }
     dom = max(classes, key=lambda k: classes[k][0])
-    t_ms, nbytes, nl = classes[dom]
-    achieved = (nbytes / nl) / (t_ms / nl * 1e-3) / 1e9 if t_ms > 0 else 0.0
+    t_ms, nbytes, nflops, nl = classes[dom]
+    lsec = t_ms / nl * 1e-3
+    gbps = (nbytes / nl) / lsec / 1e9 if t_ms > 0 else 0.0
+    out = {"kernel": dom, "launch_ms": round(t_ms / nl, 4), "algorithmic_bytes_per_launch": int(nbytes / nl),
+           "hbm_GBps": round(gbps, 1)}
+    if nflops and mfma:
+        tf = (nflops / nl) / lsec / 1e12 if t_ms > 0 else 0.0
+        peak = PEAK_MFMA_TFLOPS[precision]
+        out.update(bound="mfma", achieved=round(tf, 2), peak=peak, unit="TFLOP/s", frac=round(tf / peak, 4),
+                   algorithmic_flops_per_launch=int(nflops / nl))
+    else:
+        out.update(bound="hbm", achieved=round(gbps, 1), peak=PEAK_HBM_GBPS, unit="GB/s",
+                   frac=round(gbps / PEAK_HBM_GBPS, 4))
     per_kernel = {k: round(v[0], 3) for k, v in classes.items()}
     per_kernel.update({k: round(ms(k), 3) for k in ("gather_tiled", "gather_direct", "scatter_tiled", "scatter_fold",
                                                     "scatter_direct")})
-    return dom, achieved, nbytes / nl, t_ms / nl, per_kernel
+    out["kernel_ms_per_step"] = per_kernel
+    return out
+
+
+# kernels of each class in a rocprofv3 counter collection
+TRAFFIC_CLASSES = {
+    "gather": ("gather_mfma_k", "gather_tiled_k", "gather_direct_k", "gather_stale_k"),
+    "scatter": ("scatter_mfma_k", "scatter_cols_k", "fold_k", "scatter_stale_k", "scatter_tiled_k",
+                "scatter_direct_k"),
+    "fft_rows": ("fft_rows",),
+    "fft_cols": ("fft_cols",),
+}
+# launches per step of each class (each class's kernels are summed per launch)
+CLASS_LAUNCHES = {"gather": 2, "scatter": 1, "fft_rows": 3, "fft_cols": 6}
+
+
+def _counter_sums(path):
+    """{kernel short name: (sum of the counter in bytes, dispatches)} from a rocprofv3 counter CSV
+    (the size counters are in KiB)."""
+    import csv
+    import glob
+    if os.path.isdir(path):
+        path = glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True)[0]
+    tot = {}
+    for r in csv.DictReader(open(path)):
+        name = r["Kernel_Name"].replace("(anonymous namespace)::", "").split("(")[0].split("<")[0]
+        name = name.split("::")[-1].split()[-1]
+        v, n = tot.get(name, (0.0, 0))
+        tot[name] = (v + float(r["Counter_Value"]) * 1024.0, n + 1)
+    return tot
+
+
+def pmc_traffic(fetch, write, copy_fetch, copy_write, workload, steps_seen):
+    """HBM bytes per launch of each kernel class from FETCH_SIZE / WRITE_SIZE runs, each scaled by
+    what the same counter reads for the plain 1 GiB copy (copy_probe)."""
+    cf, cw = _counter_sums(copy_fetch), _counter_sums(copy_write)
+    big_f = max(cf.values(), key=lambda v: v[0] / v[1])
+    big_w = max(cw.values(), key=lambda v: v[0] / v[1])
+    f_scale = (1 << 30) / (big_f[0] / big_f[1])
+    w_scale = (1 << 30) / (big_w[0] / big_w[1])
+    fs, ws = _counter_sums(fetch), _counter_sums(write)
+    out = {"workload": workload, "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+           "classes": {}}
+    for cls, names in TRAFFIC_CLASSES.items():
+        fb = sum(v[0] for k, v in fs.items() if k in names)
+        wb = sum(v[0] for k, v in ws.items() if k in names)
+        launches = CLASS_LAUNCHES[cls] * steps_seen
+        out["classes"][cls] = {"fetch_bytes_per_launch_raw": int(fb / launches),
+                               "write_bytes_per_launch_raw": int(wb / launches),
+                               "bytes_per_launch": int((fb * f_scale + wb * w_scale) / launches)}
+    return out
+
+
+def traffic_file(workload):
+    return os.path.join(ROOT, "profiles", "traffic_%s.json" % workload.replace("^", "e"))
 
 
 def run_plan(w, precision, steps, warmup, x_np):
@@ -232,9 +313,20 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (calibrates the HBM byte counters)")
+    ap.add_argument("--pmc-traffic", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="rocprofv3 FETCH_SIZE / WRITE_SIZE outputs of a bench run (--steps 1 --warmup 0: "
+                         "two calls of each stage) and of --copy-probe; writes profiles/traffic_<workload>.json")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.pmc_traffic:
+        from pnfft_amd.workloads import CONFIGS
+        name = CONFIGS[args.config]["name"]
+        t = pmc_traffic(*args.pmc_traffic, workload=name, steps_seen=2)
+        with open(traffic_file(name), "w") as fo:
+            json.dump(t, fo, indent=1)
+        print(json.dumps(t, indent=1))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
@@ -266,12 +358,19 @@ def main():
 
     steps = args.steps
     ab = algorithmic_bytes(w, precision)
-    dom, achieved, lbytes, lms, per_kernel = roofline_from_kernels(res["kernel_times"], ab, steps)
-    log(f"  dominant kernel {dom}: {lms:.3f} ms/launch, {lbytes / 1e9:.3f} GB/launch, {achieved:.0f} GB/s")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": dom,
-                "launch_ms": round(lms, 4), "algorithmic_bytes_per_launch": int(lbytes),
-                "kernel_ms_per_step": per_kernel}
+    af = algorithmic_flops(w)
+    rl = roofline_from_kernels(res["kernel_times"], ab, af, steps, precision)
+    log(f"  dominant kernel {rl['kernel']}: {rl['launch_ms']:.3f} ms/launch, {rl['achieved']} {rl['unit']}")
+    traffic = None
+    tf = traffic_file(w["name"])
+    if os.path.exists(tf):
+        tj = json.load(open(tf))
+        cls = "fft_cols" if rl["kernel"].startswith("fft_cols") else rl["kernel"]
+        if tj.get("workload") == w["name"] and cls in tj["classes"]:
+            traffic = tj["classes"][cls]["bytes_per_launch"]
+    roofline = {"bound": rl["bound"], "achieved": rl["achieved"], "peak": rl["peak"], "unit": rl["unit"],
+                "frac": rl["frac"], "traffic": traffic}
+    roofline.update({k: v for k, v in rl.items() if k not in roofline})
     cpu = None
     if not args.no_cpu:
         log("  cpu baseline ...")

@@ -482,13 +482,15 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
     const unsigned long long badm = __ballot(bad);
     const unsigned stale = (unsigned)(badm | (badm >> 16) | (badm >> 32)) & 0xFFFFu;
     const bool ok = jg >= 0 && !((stale >> li) & 1u);
-    // planes the group's stencils reach: OR of the x lanes' plane masks
-    unsigned pm = (lk == 0 && ok) ? (((1u << CM) - 1u) << l) : 0u;
-    pm |= __shfl_xor(pm, 1);
-    pm |= __shfl_xor(pm, 2);
-    pm |= __shfl_xor(pm, 4);
-    pm |= __shfl_xor(pm, 8);
-    pm = __builtin_amdgcn_readfirstlane(pm);
+    // planes the group's stencils reach, [amin, amax]: from the x lanes' offsets (uniform ballots)
+    int amin = P, amax = -1;
+#pragma unroll
+    for (int v = 0; v < TS; v++) {
+      if (__ballot(lk == 0 && ok && l == v)) {
+        amin = min(amin, v);
+        amax = max(amax, v + CM - 1);
+      }
+    }
     if (lk < 3) {
       float* r0 = rows + (lk * 16 + li) * 16;
       float* r1 = rows + ((3 + lk) * 16 + li) * 16;
@@ -528,7 +530,6 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
       dzc = *reinterpret_cast<const float4*>(rows + (80 + li) * 16 + ch);
     }
     float fr = 0.f, fi = 0.f, g0r = 0.f, g0i = 0.f, g1r = 0.f, g1i = 0.f, g2r = 0.f, g2i = 0.f;
-    const int amin = pm ? __builtin_ctz(pm) : 0, amax = pm ? 31 - __builtin_clz(pm) : -1;
     // plane a + 1's operands are read while plane a is on the matrix cores
     auto ld_plane = [&](int a, float4& gr, float4& gi, float& xa, float& dxa) {
       gr = *reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * 16 + ch);
