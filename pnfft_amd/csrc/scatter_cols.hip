@@ -467,6 +467,76 @@ __global__ __launch_bounds__(TS* TS* TS) void fold_k(BArgs<R> A) {
   A.grid[(x0 * n1 + x1) * n2 + x2] = cmk<R>((R)sr, (R)si);
 }
 
+// fold_k for tiles of 8^3 cells: 128 threads per grid tile, four consecutive
+// cells (along axis 2) per thread, loaded from each covering slot as two
+// 16-byte loads where all four lie inside the subgrid; same sums, same order.
+template <class R, int P>
+__global__ __launch_bounds__(128) void fold8_k(BArgs<R> A) {
+  constexpr int TS = 8;
+  constexpr int D = (P + TS - 1) / TS;
+  constexpr size_t VOL = (size_t)P * P * P;
+  const int nt0 = A.ntile[0], nt1 = A.ntile[1], nt2 = A.ntile[2];
+  const int T = blockIdx.x;
+  const int T2 = T % nt2, T1 = (T / nt2) % nt1, T0 = T / (nt2 * nt1);
+  const int h = threadIdx.x & 1, q1 = (threadIdx.x >> 1) & 7, q0 = threadIdx.x >> 4;
+  const int q2 = 4 * h;  // first of the thread's four cells
+  double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int d0 = 0; d0 < D; d0++) {
+    const int s0 = (T0 - d0 % nt0 + nt0) % nt0;
+#pragma unroll
+    for (int d1 = 0; d1 < D; d1++) {
+      const int s1 = (T1 - d1 % nt1 + nt1) % nt1;
+#pragma unroll
+      for (int d2 = 0; d2 < D; d2++) {
+        const int s2 = (T2 - d2 % nt2 + nt2) % nt2;
+        const int slot = A.tile_slot[((long)s0 * nt1 + s1) * nt2 + s2];
+        if (slot < 0) continue;
+        const int p0 = q0 + d0 * TS, p1 = q1 + d1 * TS, p2 = q2 + d2 * TS;
+        if (p0 >= P || p1 >= P) continue;
+        const Cplx<R>* src = A.slots + (size_t)slot * VOL + ((size_t)p0 * P + p1) * P + p2;
+        if (p2 + 3 < P) {
+          Cplx<R> v[4];
+          if constexpr (sizeof(R) == 4) {  // two aligned 16-byte loads (p2 is a multiple of 4)
+            const float4 a = reinterpret_cast<const float4*>(src)[0], b = reinterpret_cast<const float4*>(src)[1];
+            v[0] = cmk<R>(a.x, a.y);
+            v[1] = cmk<R>(a.z, a.w);
+            v[2] = cmk<R>(b.x, b.y);
+            v[3] = cmk<R>(b.z, b.w);
+          } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) v[k] = src[k];
+          }
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            sr[k] += (double)v[k].x;
+            si[k] += (double)v[k].y;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            if (p2 + k < P) {
+              const Cplx<R> v = src[k];
+              sr[k] += (double)v.x;
+              si[k] += (double)v.y;
+            }
+          }
+        }
+      }
+    }
+  }
+  const long n1 = A.g.n[1], n2 = A.g.n[2];
+  const long x0 = (long)T0 * TS + q0, x1 = (long)T1 * TS + q1, x2 = (long)T2 * TS + q2;
+  Cplx<R>* dst = A.grid + (x0 * n1 + x1) * n2 + x2;
+  if constexpr (sizeof(R) == 4) {
+    reinterpret_cast<float4*>(dst)[0] = make_float4((float)sr[0], (float)si[0], (float)sr[1], (float)si[1]);
+    reinterpret_cast<float4*>(dst)[1] = make_float4((float)sr[2], (float)si[2], (float)sr[3], (float)si[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[k] = cmk<R>((R)sr[k], (R)si[k]);
+  }
+}
+
 // nodes whose stencil start left the tile they were binned to (bins older
 // than the coordinates), listed by the column kernel: added to the folded grid
 // through global atomics
@@ -504,7 +574,10 @@ void launch_m(const BArgs<R>& A, hipStream_t s) {
   if (!done) hipLaunchKernelGGL((scatter_cols_k<R, MT, MODE>), dim3((unsigned)A.n_torder), dim3(S::NT), 0, s, A);
   if (A.mid) (void)hipEventRecord(A.mid, s);
   const long ntiles = (long)A.ntile[0] * A.ntile[1] * A.ntile[2];
-  hipLaunchKernelGGL((fold_k<R, S::TS, S::P>), dim3((unsigned)ntiles), dim3(S::TS * S::TS * S::TS), 0, s, A);
+  if constexpr (S::TS == 8)
+    hipLaunchKernelGGL((fold8_k<R, S::P>), dim3((unsigned)ntiles), dim3(128), 0, s, A);
+  else
+    hipLaunchKernelGGL((fold_k<R, S::TS, S::P>), dim3((unsigned)ntiles), dim3(S::TS * S::TS * S::TS), 0, s, A);
   if (A.mid2) (void)hipEventRecord(A.mid2, s);
   hipLaunchKernelGGL((scatter_stale_k<R, MT, MODE>), dim3(256), dim3(256), 0, s, A);
 }
