@@ -317,17 +317,14 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
 // ---------------------------------------------------------------------------
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
-// LDS rows of 16 floats, swizzled by the row's index r among 16 so that every
-// float4 / scalar read of the matrix-core phase is free of bank conflicts:
-//   subgrid row b (and z row of node r), element c -> chunk (c & 3) ^ sw(r), entry c >> 2
-//     (lane group k reads the chunk holding the K-slices c = 4 kc + k, kc = 0..3)
-//   y row of node r, element b -> chunk (b >> 2) ^ sw(r), entry b & 3
-//     (lane group k reads the rows b = 4 k + e of its accumulator)
-//   x row of node r, element a -> a ^ 4 sw(r)
+// LDS rows of 16 floats (subgrid rows b and the nodes' window rows), element v
+// of row r at v ^ 4 sw(r), sw(r) = (r >> 2) & 3: a float4 chunk k of rows r =
+// 0..15 (read by one lane group) and a scalar of rows 0..15 fall in distinct
+// banks.  Lane group k supplies the MFMA K-slices c = 4k + e, e = 0..3, of the
+// subgrid row and the z-window row as one float4 (chunk k), and reads the rows
+// b = 4k + e of its accumulator with the y-window chunk k.
 __device__ __forceinline__ int sw4(int r) { return (r >> 2) & 3; }
-__device__ __forceinline__ int kpos(int r, int c) { return (((c & 3) ^ sw4(r)) << 2) | (c >> 2); }
-__device__ __forceinline__ int bpos(int r, int b) { return (((b >> 2) ^ sw4(r)) << 2) | (b & 3); }
-__device__ __forceinline__ int apos(int r, int a) { return a ^ (sw4(r) << 2); }
+__device__ __forceinline__ int rpos(int r, int v) { return v ^ (sw4(r) << 2); }
 
 // LDS writes of one lane become visible to the other lanes of its wave
 __device__ __forceinline__ void wave_sync() {
@@ -363,7 +360,7 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
   constexpr bool WG = S::WG;
   static_assert(P <= 16, "one 16 x 16 block per plane");
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  float* s_gr = reinterpret_cast<float*>(smem_raw);  // [P][16 rows b][16], element c at kpos(b, c)
+  float* s_gr = reinterpret_cast<float*>(smem_raw);  // [P][16 rows b][16], element c at rpos(b, c)
   float* s_gi = s_gr + P * 256;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -403,11 +400,10 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
       const int q = tid + 256 * k;
       if (q < NPAIR) {
         const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
-        const int r = (cx * 16 + cy) * 16;
-        s_gr[r + kpos(cy, 2 * hz)] = v[k].x;
-        s_gi[r + kpos(cy, 2 * hz)] = v[k].y;
-        s_gr[r + kpos(cy, 2 * hz + 1)] = v[k].z;
-        s_gi[r + kpos(cy, 2 * hz + 1)] = v[k].w;
+        // cells c, c + 1 (c even) stay adjacent under the swizzle
+        const int r = (cx * 16 + cy) * 16 + rpos(cy, 2 * hz);
+        *reinterpret_cast<float2*>(s_gr + r) = make_float2(v[k].x, v[k].z);
+        *reinterpret_cast<float2*>(s_gi + r) = make_float2(v[k].y, v[k].w);
       }
     }
   }
@@ -501,10 +497,10 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
         if (WG) reinterpret_cast<float4*>(r1)[e] = z4;
       }
       if (ok) {
+        const int sx = sw4(li) << 2;
 #pragma unroll
         for (int s = 0; s < CM; s++) {
-          const int v = l + s;
-          const int pos = lk == 0 ? apos(li, v) : (lk == 1 ? bpos(li, v) : kpos(li, v));
+          const int pos = (l + s) ^ sx;
           r0[pos] = w[s];
           if (WG) r1[pos] = dw[s];
         }
@@ -523,7 +519,7 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
     const float* dxr = rows + (48 + li) * 16;
     const int ch = 4 * (lk ^ sw4(li));
     const float4 yb = *reinterpret_cast<const float4*>(rows + (16 + li) * 16 + ch);  // Y_j[4 lk + e]
-    const float4 zc = *reinterpret_cast<const float4*>(rows + (32 + li) * 16 + ch);  // Z_j[4 kc + lk]
+    const float4 zc = *reinterpret_cast<const float4*>(rows + (32 + li) * 16 + ch);  // Z_j[4 lk + e]
     float4 dyb = yb, dzc = zc;
     if (WG) {
       dyb = *reinterpret_cast<const float4*>(rows + (64 + li) * 16 + ch);
@@ -534,8 +530,8 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
     auto ld_plane = [&](int a, float4& gr, float4& gi, float& xa, float& dxa) {
       gr = *reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * 16 + ch);
       gi = *reinterpret_cast<const float4*>(s_gi + (a * 16 + li) * 16 + ch);
-      xa = xr[apos(li, a)];
-      dxa = WG ? dxr[apos(li, a)] : 0.f;
+      xa = xr[rpos(li, a)];
+      dxa = WG ? dxr[rpos(li, a)] : 0.f;
     };
     float4 gr, gi;
     float xa, dxa;
