@@ -289,6 +289,75 @@ __global__ __launch_bounds__(kThreads) void fft_rows(PassArgs<R> P) {
   }
 }
 
+// LDS writes of one lane become visible to the other lanes of its wave
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rows pass for L = 512 = 8^3: one wave per pencil (four per block), no block
+// barrier.  Lane j holds butterfly j of every stage: the first stage runs on
+// the loaded registers (positions j + 64 q, consecutive lanes on consecutive
+// elements), the middle stage goes through the wave's LDS pencil (element p at
+// p + p / 8, which keeps the stage's reads and writes within two-way bank
+// conflicts), and the last stage's outputs j + 64 q go straight to HBM.  Same
+// Stockham stages, twiddles and per-element hooks as fft_rows.
+constexpr int kRows512Pencils = 4;
+
+template <class R>
+__global__ __launch_bounds__(256) void fft_rows512(PassArgs<R> P) {
+  constexpr int L = 512, NP = L + L / 8;
+  __shared__ Cplx<R> sh[kRows512Pencils][NP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = blockIdx.x * kRows512Pencils + wave;
+  if (col >= P.n_inner) return;  // whole wave; no barrier follows
+  Cplx<R>* pen = sh[wave];
+  auto pad = [](int p) { return p + (p >> 3); };
+  Cplx<R> v[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int e = load_index(P, lane + 64 * q, L);
+    v[q] = cmk<R>(R(0), R(0));
+    if (e >= 0) v[q] = P.in[(long)col * P.in_outer + e];
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int e = load_index(P, lane + 64 * q, L);
+    if (e >= 0) v[q] = on_load(P, v[q], col, e);
+  }
+  // stage span 1: outputs at 8 j + q
+  dft8(v);
+#pragma unroll
+  for (int q = 0; q < 8; q++) pen[pad(8 * lane + q)] = v[q];
+  wave_sync();
+  // stage span 8: inputs j + 64 q, twiddles tw[8 k q] with k = j mod 8, outputs (j - k) 8 + k + 8 q
+  {
+    const int k = lane & 7;
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = pen[pad(lane + 64 * q)];
+#pragma unroll
+    for (int q = 1; q < 8; q++) v[q] = cmul(v[q], P.tw[8 * k * q]);
+    dft8(v);
+    wave_sync();  // every read of the pencil precedes the writes
+    const int base = (lane - k) * 8 + k;
+#pragma unroll
+    for (int q = 0; q < 8; q++) pen[pad(base + 8 * q)] = v[q];
+    wave_sync();
+  }
+  // stage span 64: inputs j + 64 q, twiddles tw[j q], outputs j + 64 q
+#pragma unroll
+  for (int q = 0; q < 8; q++) v[q] = pen[pad(lane + 64 * q)];
+#pragma unroll
+  for (int q = 1; q < 8; q++) v[q] = cmul(v[q], P.tw[lane * q]);
+  dft8(v);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int e = store_index(P, lane + 64 * q, L);
+    if (e >= 0) on_store(P, v[q], col, e, (long)col * P.out_outer + e);
+  }
+}
+
 // Columns pass (axes 0 and 1): pencil c of the block is column c0 + c of
 // plane blockIdx.y, consecutive lanes take consecutive columns.  The first
 // stage runs on the loaded registers and the last stage stores straight to
@@ -418,8 +487,12 @@ static int pass_attr() {
 template <class R, int L>
 static void pass_launch(bool rows, dim3 grid, const PassArgs<R>& P, hipStream_t s) {
   if (rows) {
-    constexpr size_t lds = PassShape<R, L>::LDS;
-    hipLaunchKernelGGL((fft_rows<R, L>), grid, dim3(kThreads), lds, s, P);
+    if constexpr (L == 512) {
+      hipLaunchKernelGGL((fft_rows512<R>), grid, dim3(256), 0, s, P);
+    } else {
+      constexpr size_t lds = PassShape<R, L>::LDS;
+      hipLaunchKernelGGL((fft_rows<R, L>), grid, dim3(kThreads), lds, s, P);
+    }
   } else {
     constexpr int nt = cols_threads<R, L>();
     constexpr size_t lds = PassShape<R, L, nt>::LDS;
@@ -459,7 +532,7 @@ template <class R>
 int FftAxis<R>::init(int L_) {
   L = L_;
   if (!fft_length_supported(L, sizeof(R) == 8)) return kUnsupported;
-  C = FftEpt<R>::value * kThreads / L;
+  C = L == 512 ? kRows512Pencils : FftEpt<R>::value * kThreads / L;
   Ccols = 0;
   (void)with_length<R>(L, [&](auto Lc) {
     constexpr int LL = decltype(Lc)::value;
