@@ -444,3 +444,27 @@ def test_fp32_modes_clustered(gpu, oracle, m):
         assert err is None, err
         say("  rel errors", errs)
         assert max(errs.values()) < 1e-5
+
+
+def _golden():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nfft_small.npz")
+    z = np.load(path)  # allow_pickle stays False
+    cases = sorted({k.split("__")[0] for k in z.files})
+    return {c: {k.split("__")[1]: z[k] for k in z.files if k.startswith(c + "__")} for c in cases}
+
+
+@pytest.mark.parametrize("case", ["kb_m3", "gauss_m3", "bspline_m3", "sincpow_m3", "kb_m6_aniso"])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-12), ("float", 1e-5)])
+def test_matches_golden_fixture(gpu, case, precision, tol):
+    """trafo (f, grad) and adj (of f, of grad) against the committed FP64 fast-transform outputs of
+    tests/golden/make_golden.py (every window; an anisotropic grid at m = 6)."""
+    g = _golden()[case]
+    say("case golden", case, precision)
+    N, n, m, fl = [int(v) for v in g["N"]], [int(v) for v in g["n"]], int(g["m"]), int(g["flags"])
+    x = g["x"] if precision == "double" else g["x"].astype(np.float32).astype(np.float64)
+    out = run_gpu(N, n, m, fl, x, g["fhat"], g["f"], g["g"], precision=precision)
+    assert rel(out["f"], g["nfft_f"]) < tol
+    assert rel(out["grad"], g["nfft_grad"]) < tol
+    assert rel(out["adj"], g["nfft_adj_f"]) < tol
+    assert rel(out["adj_grad"], g["nfft_adj_grad"]) < tol

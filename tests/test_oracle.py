@@ -5,6 +5,8 @@ formulas: a library routine (scipy, mpmath, numpy.fft, numpy's stable sort),
 a closed form, a numerical integral, finite differences, or an invariant
 (adjointness, convergence to the NDFT at the window's accuracy).
 """
+import os
+
 import mpmath
 import numpy as np
 import pytest
@@ -318,3 +320,42 @@ def test_block_parallel_scatter_matches_serial(oracle):
            dp(grid.view(np.float64)), 3)
         out.append(grid)
     assert oracle.rel_l2(out[1], out[0]) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# committed fixtures (tests/golden/make_golden.py)
+# ---------------------------------------------------------------------------
+def _golden():
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nfft_small.npz")
+    z = np.load(path)  # allow_pickle stays False
+    cases = sorted({k.split("__")[0] for k in z.files})
+    return {c: {k.split("__")[1]: z[k] for k in z.files if k.startswith(c + "__")} for c in cases}
+
+
+# fast transform vs direct sums, per case: the window's accuracy at these m
+GOLDEN_NDFT_TOL = {"kb_m3": 2e-4, "gauss_m3": 1e-2, "bspline_m3": 1e-2, "sincpow_m3": 5e-3, "kb_m6_aniso": 1e-9}
+
+
+@pytest.mark.parametrize("case", sorted(GOLDEN_NDFT_TOL))
+def test_oracle_matches_golden_fixture(oracle, case):
+    g = _golden()[case]
+    N, n, m, fl = list(g["N"]), list(g["n"]), int(g["m"]), int(g["flags"])
+    P = oracle.Plan(N, n, m, fl)
+    f, gr = P.trafo(g["x"], g["fhat"], compute_grad=True)
+    pairs = [(f, g["nfft_f"]), (gr, g["nfft_grad"]), (P.adj(g["x"], f=g["f"]), g["nfft_adj_f"]),
+             (P.adj(g["x"], grad=g["g"]), g["nfft_adj_grad"])]
+    for a, b in pairs:
+        assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(b)
+    # the oracle's direct sums reproduce the fixture's numpy sums of the definitions
+    of, og = oracle.ndft_trafo(N, g["x"], g["fhat"], compute_grad=True)
+    assert np.linalg.norm(of - g["ndft_f"]) <= 1e-12 * np.linalg.norm(g["ndft_f"])
+    assert np.linalg.norm(og - g["ndft_grad"]) <= 1e-12 * np.linalg.norm(g["ndft_grad"])
+    oa = oracle.ndft_adj(N, g["x"], f=g["f"])
+    assert np.linalg.norm(oa - g["ndft_adj_f"]) <= 1e-12 * np.linalg.norm(g["ndft_adj_f"])
+    oag = oracle.ndft_adj(N, g["x"], grad=g["g"])
+    assert np.linalg.norm(oag - g["ndft_adj_grad"]) <= 1e-12 * np.linalg.norm(g["ndft_adj_grad"])
+    # and the fast transform approximates them to the window's accuracy
+    tol = GOLDEN_NDFT_TOL[case]
+    for fast, direct in (("nfft_f", "ndft_f"), ("nfft_grad", "ndft_grad"), ("nfft_adj_f", "ndft_adj_f"),
+                         ("nfft_adj_grad", "ndft_adj_grad")):
+        assert np.linalg.norm(g[fast] - g[direct]) <= tol * np.linalg.norm(g[direct])
