@@ -228,6 +228,9 @@ struct MfmaShape {
   static constexpr int NV = NB * V;                                      // terms per batch
   // waves per SIMD the kernel is compiled for: the gradient windows need the larger register budget
   static constexpr int WAVES = V == 1 ? 3 : 2;
+  // row buffers: two for f (rows of the next batch written during this batch's
+  // matrix-core phase); the gradient modes' three or four terms per node keep one
+  static constexpr int NBUF = V == 1 ? 2 : 1;
 };
 
 // row position of plane a: wave (a & 3) reads the float4 at 4 (a & 3), entry a >> 2
@@ -254,11 +257,14 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
   constexpr bool WF = S::WF, WG = S::WG;
   static_assert(P <= 16, "one 16 x 16 block per plane");
   static_assert(NB == 64, "one node per lane of a wave in the rows phase");
-  __shared__ __align__(16) float s_x[NV][16];  // X_v by xpos
-  __shared__ __align__(16) float s_y[NV][16];  // Y_v[b], zero outside the stencil
-  __shared__ __align__(16) float s_z[NV][16];  // Z_v[c]
-  __shared__ float2 s_w[NV];                   // W_v
-  __shared__ unsigned char s_bad[3][NB];       // node outside the tile on axis t
+  // window rows of a batch; with two buffers the rows of batch b + 1 are written
+  // while batch b is on the matrix cores (one barrier per batch)
+  constexpr int NBUF = MS::NBUF;
+  __shared__ __align__(16) float s_x[NBUF][NV][16];  // X_v by xpos
+  __shared__ __align__(16) float s_y[NBUF][NV][16];  // Y_v[b], zero outside the stencil
+  __shared__ __align__(16) float s_z[NBUF][NV][16];  // Z_v[c]
+  __shared__ float2 s_w[NBUF][NV];                   // W_v
+  __shared__ unsigned char s_bad[NBUF][3][NB];       // node outside the tile on axis t
 
   const int tile = A.torder[blockIdx.x];
   const int t2 = tile % A.ntile[2];
@@ -277,7 +283,6 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
   const int ot = ax == 0 ? o0 : (ax == 1 ? o1 : o2);
   const float bt = (float)(ax == 0 ? A.g.b[0] : (ax == 1 ? A.g.b[1] : A.g.b[2]));
   const float wst = (float)(ax == 0 ? A.g.ws[0] : (ax == 1 ? A.g.ws[1] : A.g.ws[2]));
-  float(*srow)[16] = ax == 0 ? s_x : (ax == 1 ? s_y : s_z);
   const long M = A.M;
   const bool tab = A.pre_psi != nullptr && (!WG || A.pre_dpsi != nullptr);
 
@@ -327,9 +332,9 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
   int jn = perm_raw(beg + NB < end ? beg + NB : beg);
   bool jn_ok = perm_ok(beg + NB);
 
-  for (int base = beg; base < end; base += NB) {
-    const int nb = min(NB, end - base);
-    // rows phase
+  // rows phase: window rows (roles 0..2) and weights (role 3) of the batch at
+  // base, from the loads in L, into row buffer bf
+  auto rows_phase = [&](const ScatterLoad<CM, WF, WG>& L, int bf) __attribute__((always_inline)) {
     if (role < 3) {
       float w[CM], dw[CM];
       int l = 0;
@@ -353,6 +358,7 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
           }
         }
       }
+      float(*srow)[16] = ax == 0 ? s_x[bf] : (ax == 1 ? s_y[bf] : s_z[bf]);
       const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
       for (int u = 0; u < V; u++) {
@@ -366,31 +372,50 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
           for (int s = 0; s < CM; s++) row[ax == 0 ? xpos(l + s) : l + s] = d ? dw[s] : w[s];
         }
       }
-      s_bad[ax][q] = bad ? 1 : 0;
+      s_bad[bf][ax][q] = bad ? 1 : 0;
     } else {
       int v = q * V;
-      if (WF) s_w[v++] = L.j >= 0 ? make_float2(L.f.x, L.f.y) : make_float2(0.f, 0.f);
+      if (WF) s_w[bf][v++] = L.j >= 0 ? make_float2(L.f.x, L.f.y) : make_float2(0.f, 0.f);
       if (WG)
 #pragma unroll
-        for (int t = 0; t < 3; t++) s_w[v++] = L.j >= 0 ? make_float2(L.g[t].x, L.g[t].y) : make_float2(0.f, 0.f);
+        for (int t = 0; t < 3; t++) s_w[bf][v++] = L.j >= 0 ? make_float2(L.g[t].x, L.g[t].y) : make_float2(0.f, 0.f);
     }
+  };
+
+  if (NBUF == 2) {  // rows of the first batch
+    rows_phase(L, 0);
+    ScatterLoad<CM, WF, WG> Ln;
+    const int bn = beg + NB < end ? beg + NB : beg;
+    load(bn, jn_ok ? jn : -1, Ln);
+    jn = perm_raw(beg + 2 * NB < end ? beg + 2 * NB : beg);
+    jn_ok = perm_ok(beg + 2 * NB);
+    L = Ln;
     __syncthreads();
+  }
+  for (int base = beg, it = 0; base < end; base += NB, it++) {
+    const int nb = min(NB, end - base);
+    const int bf = NBUF == 2 ? (it & 1) : 0;
+    if (NBUF == 1) {
+      rows_phase(L, 0);
+      __syncthreads();
+    }
     // a node outside its tile has a zero row on the axis where it left, so it
     // adds nothing here; it is listed for the direct kernel
-    if (role == 3 && L.j >= 0 && (s_bad[0][q] | s_bad[1][q] | s_bad[2][q]))
+    if (role == 3 && base + q < end && (s_bad[bf][0][q] | s_bad[bf][1][q] | s_bad[bf][2][q]))
       A.stale_list[atomicAdd(A.stale_count, 1u)] = base + q;
-    // loads of the next batch, in flight during the matrix-core phase
+    if (NBUF == 2 && base + NB < end) rows_phase(L, bf ^ 1);  // the next batch, into the other buffer
+    // loads of the batch after the one in L, in flight during the matrix-core phase
     ScatterLoad<CM, WF, WG> Ln;
-    const int bn = base + NB < end ? base + NB : base;
-    load(bn, jn_ok ? jn : -1, Ln);
-    const int jnn = perm_raw(base + 2 * NB < end ? base + 2 * NB : base);
-    const bool jnn_ok = perm_ok(base + 2 * NB);
+    const int bnext = base + NBUF * NB;
+    load(bnext < end ? bnext : base, jn_ok ? jn : -1, Ln);
+    const int jnn = perm_raw(bnext + NB < end ? bnext + NB : base);
+    const bool jnn_ok = perm_ok(bnext + NB);
     const int nq = (nb * V + 3) >> 2;
     for (int q4 = 0; q4 < nq; q4++) {
       const int v = 4 * q4 + lk;
-      const float4 x = reinterpret_cast<const float4*>(s_x[v])[wave];
-      const float y = s_y[v][li], z = s_z[v][li];
-      const float2 wv = s_w[v];
+      const float4 x = reinterpret_cast<const float4*>(s_x[bf][v])[wave];
+      const float y = s_y[bf][v][li], z = s_z[bf][v][li];
+      const float2 wv = s_w[bf][v];
       const float bre = wv.x * z, bim = wv.y * z;
       const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
