@@ -103,9 +103,9 @@ __device__ inline void dft8(Cplx<R>* v) {
 
 // Block shape of a pass over pencils of length L with NT threads: C pencils per
 // block with C * L = EPT * NT, LDS layout [position][pencil] with pitch LD.
-template <class R, int L, int NT = kThreads>
+template <class R, int L, int NT = kThreads, int EPT_ = FftEpt<R>::value>
 struct PassShape {
-  static constexpr int EPT = FftEpt<R>::value;
+  static constexpr int EPT = EPT_;
   static constexpr int C = EPT * NT / L;
   static constexpr int LD = C == 1 ? 1 : C + 1;
   static constexpr size_t LDS = sizeof(Cplx<R>) * (size_t)L * (size_t)LD;
@@ -118,6 +118,19 @@ template <class R, int L>
 constexpr int cols_threads() {
   return PassShape<R, L, 2 * kThreads>::LDS <= 160 * 1024 ? 2 * kThreads : kThreads;
 }
+
+// Elements per thread of a columns pass.  The narrow variant, for the axis-1
+// pass (pencil elements n2 apart), takes 16 for 512-point FP32 pencils: 16
+// pencils per 512-thread block, 64 KB of LDS, two blocks per CU.  The axis-0
+// pass (elements n1 n2 apart) keeps 32: its 256-byte row segments matter more
+// there than the second block.
+template <class R, int L, bool NARROW>
+constexpr int cols_ept() {
+  return (NARROW && sizeof(R) == 4 && L == 512) ? 16 : FftEpt<R>::value;
+}
+
+template <class R, int L, bool NARROW>
+using ColsShape = PassShape<R, L, cols_threads<R, L>(), cols_ept<R, L, NARROW>()>;
 
 // radix of the stage that starts at span Ns (8 while possible, then 4 or 2)
 template <int L, int Ns>
@@ -135,9 +148,9 @@ __device__ inline void dft(Cplx<R>* w) {
 // One Stockham stage LDS -> LDS of radix RR at span Ns.  Butterfly bf of the
 // block covers pencil c = bf % C, index j = bf / C; it reads positions
 // j + q L/RR and writes (j - k) RR + k + q Ns with k = j mod Ns.
-template <class R, int L, int Ns, int NT>
+template <class R, int L, int Ns, int NT, int EP>
 __device__ inline void stage_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
-  using S = PassShape<R, L, NT>;
+  using S = PassShape<R, L, NT, EP>;
   constexpr int RR = StageRadix<L, Ns>::value;
   constexpr int NB = S::EPT / RR;
   constexpr int nbf = L / RR;
@@ -171,11 +184,11 @@ __device__ inline void stage_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
 }
 
 // stages LDS -> LDS from span Ns up to, but not including, span END
-template <class R, int L, int Ns, int END, int NT>
+template <class R, int L, int Ns, int END, int NT, int EP>
 __device__ inline void stages_lds(Cplx<R>* sh, const Cplx<R>* __restrict__ tw) {
   if constexpr (Ns < END) {
-    stage_lds<R, L, Ns, NT>(sh, tw);
-    stages_lds<R, L, Ns * StageRadix<L, Ns>::value, END, NT>(sh, tw);
+    stage_lds<R, L, Ns, NT, EP>(sh, tw);
+    stages_lds<R, L, Ns * StageRadix<L, Ns>::value, END, NT, EP>(sh, tw);
   }
 }
 
@@ -272,7 +285,7 @@ __global__ __launch_bounds__(kThreads) void fft_rows(PassArgs<R> P) {
     sh[p * S::LD + c] = w;
   }
   __syncthreads();
-  stages_lds<R, L, 1, L, kThreads>(sh, P.tw);
+  stages_lds<R, L, 1, L, kThreads, S::EPT>(sh, P.tw);
 #pragma unroll
   for (int i = 0; i < S::EPT; i++) {
     const int idx = tid + i * kThreads;
@@ -362,10 +375,10 @@ __global__ __launch_bounds__(256) void fft_rows512(PassArgs<R> P) {
 // plane blockIdx.y, consecutive lanes take consecutive columns.  The first
 // stage runs on the loaded registers and the last stage stores straight to
 // HBM, so only the middle stages touch LDS.
-template <class R, int L>
+template <class R, int L, bool NARROW>
 __global__ __launch_bounds__((cols_threads<R, L>())) void fft_cols(PassArgs<R> P) {
   constexpr int NT = cols_threads<R, L>();
-  using S = PassShape<R, L, NT>;
+  using S = ColsShape<R, L, NARROW>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<R>* sh = reinterpret_cast<Cplx<R>*>(smem_raw);
   constexpr int R1 = StageRadix<L, 1>::value;
@@ -406,7 +419,7 @@ __global__ __launch_bounds__((cols_threads<R, L>())) void fft_cols(PassArgs<R> P
       for (int q = 0; q < R1; q++) sh[(j * R1 + q) * S::LD + c] = v[i * R1 + q];
     }
     __syncthreads();
-    stages_lds<R, L, R1, NsL, NT>(sh, P.tw);
+    stages_lds<R, L, R1, NsL, NT, S::EPT>(sh, P.tw);
     // ---- last stage (span NsL): outputs at positions j + q NsL
 #pragma unroll
     for (int i = 0; i < NBL; i++) {
@@ -472,14 +485,19 @@ static PassArgs<R> make_args(const FftAxis<R>& ax) {
 template <class R, int L>
 static int pass_attr() {
   constexpr size_t lr = PassShape<R, L>::LDS;
-  constexpr size_t lc = PassShape<R, L, cols_threads<R, L>()>::LDS;
+  constexpr size_t lc = ColsShape<R, L, false>::LDS;
+  constexpr size_t ln = ColsShape<R, L, true>::LDS;
   if (lr > 64 * 1024 &&
       hipFuncSetAttribute((const void*)fft_rows<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr) !=
           hipSuccess)
     return kHipError;
   if (lc > 64 * 1024 &&
-      hipFuncSetAttribute((const void*)fft_cols<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lc) !=
-          hipSuccess)
+      hipFuncSetAttribute((const void*)fft_cols<R, L, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lc) != hipSuccess)
+    return kHipError;
+  if (ln > 64 * 1024 &&
+      hipFuncSetAttribute((const void*)fft_cols<R, L, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)ln) != hipSuccess)
     return kHipError;
   return kOk;
 }
@@ -495,8 +513,14 @@ static void pass_launch(bool rows, dim3 grid, const PassArgs<R>& P, hipStream_t 
     }
   } else {
     constexpr int nt = cols_threads<R, L>();
-    constexpr size_t lds = PassShape<R, L, nt>::LDS;
-    hipLaunchKernelGGL((fft_cols<R, L>), grid, dim3(nt), lds, s, P);
+    // the narrow variant is the one whose pencil count the pass was planned with
+    if (ColsShape<R, L, true>::C != ColsShape<R, L, false>::C && P.C == ColsShape<R, L, true>::C) {
+      constexpr size_t lds = ColsShape<R, L, true>::LDS;
+      hipLaunchKernelGGL((fft_cols<R, L, true>), grid, dim3(nt), lds, s, P);
+    } else {
+      constexpr size_t lds = ColsShape<R, L, false>::LDS;
+      hipLaunchKernelGGL((fft_cols<R, L, false>), grid, dim3(nt), lds, s, P);
+    }
   }
 }
 
@@ -536,7 +560,8 @@ int FftAxis<R>::init(int L_) {
   Ccols = 0;
   (void)with_length<R>(L, [&](auto Lc) {
     constexpr int LL = decltype(Lc)::value;
-    Ccols = PassShape<R, LL, cols_threads<R, LL>()>::C;
+    Ccols = ColsShape<R, LL, false>::C;
+    Ccols_narrow = ColsShape<R, LL, true>::C;
     return kOk;
   });
   if (Ccols <= 0) return kUnsupported;
@@ -589,7 +614,7 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.in_outer = (long)N1 * n2; P.in_elem = n2;
     P.out_outer = (long)n1 * n2; P.out_elem = n2;
     P.n_inner = n2;
-    P.C = F.ax[1].Ccols;
+    P.C = F.ax[1].Ccols_narrow;
     P.Lin = N1; P.Lout = n1;
     P.h = N1 / 2; P.pos = N1 - N1 / 2;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);
@@ -640,7 +665,7 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.in_outer = (long)n1 * n2; P.in_elem = n2;
     P.out_outer = (long)N1 * n2; P.out_elem = n2;
     P.n_inner = n2;
-    P.C = F.ax[1].Ccols;
+    P.C = F.ax[1].Ccols_narrow;
     P.Lin = n1; P.Lout = N1; P.h = N1 / 2;
     P.backward = 1;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);

@@ -17,7 +17,8 @@ template <class R>
 struct FftAxis {
   int L = 0;
   int C = 0;          // pencils per block, rows pass
-  int Ccols = 0;      // pencils per block, columns passes
+  int Ccols = 0;         // pencils per block, axis-0 columns pass
+  int Ccols_narrow = 0;  // pencils per block, axis-1 columns pass
   Cplx<R>* tw = nullptr;  // device twiddles e^{-2 pi i j / L}
   int init(int L);
   void release();
