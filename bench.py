@@ -121,7 +121,7 @@ TRAFFIC_CLASSES = {
     "gather": ("gather_mfma_k", "gather_tiled_k", "gather_direct_k", "gather_stale_k"),
     "scatter": ("scatter_mfma_k", "scatter_cols_k", "fold_k", "scatter_stale_k", "scatter_tiled_k",
                 "scatter_direct_k"),
-    "fft_rows": ("fft_rows",),
+    "fft_rows": ("fft_rows", "fft_rows512"),
     "fft_cols": ("fft_cols",),
 }
 # launches per step of each class (each class's kernels are summed per launch)
