@@ -74,7 +74,7 @@ PEAK_HBM_GBPS = 8000.0
 PEAK_MFMA_TFLOPS = {"float": 157.3, "double": 78.6}
 
 
-def roofline_from_kernels(kt, ab, af, steps, precision):
+def roofline_from_kernels(kt, ab, af, steps, precision, m=4):
     """Dominant kernel class of one step, with its binding roofline.
 
     kt: {class: (ms, launches)} over the timed steps (HIP events on the plan's stream).  One step
@@ -85,7 +85,7 @@ def roofline_from_kernels(kt, ab, af, steps, precision):
     def ms(k):
         return kt[k][0] / steps
 
-    mfma = precision == "float"
+    mfma = precision == "float" and m <= 4  # the matrix-core B kernels (FP32, m <= 4)
     classes = {
         "fft_rows": (ms("fft_rows"), 2 * ab["fft_rows_fwd"] + ab["fft_rows_bwd"], 0, 3),
         "fft_cols1": (ms("fft_cols1"), 3 * ab["fft_cols1"], 0, 3),
@@ -359,7 +359,7 @@ def main():
     steps = args.steps
     ab = algorithmic_bytes(w, precision)
     af = algorithmic_flops(w)
-    rl = roofline_from_kernels(res["kernel_times"], ab, af, steps, precision)
+    rl = roofline_from_kernels(res["kernel_times"], ab, af, steps, precision, w["m"])
     log(f"  dominant kernel {rl['kernel']}: {rl['launch_ms']:.3f} ms/launch, {rl['achieved']} {rl['unit']}")
     traffic = None
     tf = traffic_file(w["name"])
