@@ -380,7 +380,9 @@ def main():
         cpu.pop("seconds", None)
     value = w["M"] / (res["ms_per_step"] * 1e-3)
     line = {
-        "metric": METRIC, "value": value, "unit": "pts/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "metric": METRIC if args.config == 3 else
+        f"nonuniform pts/s (trafo+adj+grad), {w['name']}, {'FP32' if precision == 'float' else 'FP64'}",
+        "value": value, "unit": "pts/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if precision == "float" else "f64", "data": "synthetic",
         "config": {"workload": w["name"], "N": list(w["N"]), "n": list(w["n"]), "m": w["m"], "M": w["M"],
