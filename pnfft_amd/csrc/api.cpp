@@ -307,12 +307,11 @@ static void tables_compute(PlanT<R>* P) {
     (void)hipMemcpy(P->d_tab, P->h_tab.data(), sizeof(double) * P->h_tab.size(), hipMemcpyHostToDevice);
 }
 
-// Tile layout for binning: tiles of 8 cells per axis for m <= 6 and 4 beyond, so
-// that the padded subgrid (TS + 2m)^3 of FP64 accumulators fits in 160 KB of LDS.
-static TileLayout choose_tiles(const long* n, int m) {
+// Tile layout for binning: std_tile cells per axis (8 or 4, fewer on shorter axes).
+static TileLayout choose_tiles(const long* n, int m, bool is_double) {
   TileLayout T{};
   for (int t = 0; t < 3; t++) {
-    int ts = m <= 6 ? 8 : 4;
+    int ts = std_tile(m, is_double);
     while (ts > 1 && n[t] % ts) ts >>= 1;
     T.ts[t] = ts;
     T.ntile[t] = (int)((n[t] + ts - 1) / ts);
@@ -403,7 +402,7 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
     P->geom.n[t] = (int)n[t];
     P->geom.b[t] = P->b[t];
   }
-  P->tiles = choose_tiles(P->n, m);
+  P->tiles = choose_tiles(P->n, m, sizeof(R) == 8);
 
   bool ok = hipStreamCreate(&P->stream) == hipSuccess;
   int Ni[3] = {(int)N[0], (int)N[1], (int)N[2]}, ni[3] = {(int)n[0], (int)n[1], (int)n[2]};

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kTileThreads, (gather_waves<R, MODE>())) void gathe
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
   const int P1 = it.P[1], P2 = it.P[2];
   const int vol = it.P[0] * P1 * P2;
-  constexpr int TSL = MT <= 6 ? 8 : 4;  // tile size of the standard layout
+  constexpr int TSL = std_tile(MT, sizeof(R) == 8);  // tile size of the standard layout
   if (ST && A.ts[0] == TSL && A.ts[1] == TSL && A.ts[2] == TSL) {
     stage_subgrid_static<R, TSL + 2 * MT>(A, it, tile);
   } else {
@@ -294,8 +294,8 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-core gather (FP32 plans, P = 8 + 2m <= 16, i.e. m <= 4; f, grad,
-// f + grad).  For a plane a of the tile's padded subgrid and a group of 16
+// Matrix-core gather (FP32 plans, P = TS + 2m <= 16: tiles of 8 cells for
+// m <= 4, of 4 cells for m = 5, 6; f, grad, f + grad).  For a plane a of the tile's padded subgrid and a group of 16
 // nodes,
 //   T_a[b][j] = sum_c G[a][b][c] Z_j[c]      (Z_j = psi_z of node j, zero-padded)
 // is a 16 x 16 x 16 product: four v_mfma_f32_16x16x4f32 (K = 4 values of c
@@ -335,7 +335,7 @@ __device__ __forceinline__ void wave_sync() {
 
 template <int MT, int MODE>
 struct GatherMfmaShape {
-  static constexpr int CM = 2 * MT + 1, TS = 8, P = TS + 2 * MT;
+  static constexpr int CM = 2 * MT + 1, TS = std_tile(MT, false), P = TS + 2 * MT;
   static constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
   static constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
   static constexpr int ROWS = WG ? 6 : 3;  // window rows per node: x, y, z (and x', y', z')
@@ -467,7 +467,8 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
           dw[s] = WG ? L.dpsi[s] : 0.f;
         }
       } else {
-        window_axis_s<float, CM, WG>(A.g.win, na.frac, (float)nt, bt, w, dw);
+        // inlined: an out-of-line call would keep w and dw in scratch memory
+        [[clang::always_inline]] window_axis_s<float, CM, WG>(A.g.win, na.frac, (float)nt, bt, w, dw);
 #pragma unroll
         for (int s = 0; s < CM; s++) {
           w[s] *= wst;
@@ -632,21 +633,23 @@ __global__ __launch_bounds__(256) void gather_stale_k(BArgs<float> A) {
 
 inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
-// the matrix-core gather applies to FP32 f / grad / f + grad with m <= 4 on the
-// standard tile layout, once the nonempty tiles are listed
+// the matrix-core gather applies to FP32 f / grad / f + grad with m <= 6 on the
+// standard tile layout (padded subgrids of at most 16 cells per axis), once the
+// nonempty tiles are listed
 template <class R, int MT, int MODE>
 bool mfma_gather(const BArgs<R>& A) {
-  if constexpr (sizeof(R) != 4 || MT > 4 || MODE > B_FGRAD) {
+  if constexpr (sizeof(R) != 4 || MT > 6 || MODE > B_FGRAD) {
     return false;
   } else {
-    return !A.no_mfma && A.torder && A.n_torder > 0 && A.stale_count && A.stale_list && A.ts[0] == 8 &&
-           A.ts[1] == 8 && A.ts[2] == 8;
+    constexpr int TS = GatherMfmaShape<MT, MODE>::TS;
+    return !A.no_mfma && A.torder && A.n_torder > 0 && A.stale_count && A.stale_list && A.ts[0] == TS &&
+           A.ts[1] == TS && A.ts[2] == TS;
   }
 }
 
 template <class R, int MT, int MODE>
 void launch_gather_mfma(const BArgs<R>& A, hipStream_t s) {
-  if constexpr (sizeof(R) == 4 && MT <= 4 && MODE <= B_FGRAD) {
+  if constexpr (sizeof(R) == 4 && MT <= 6 && MODE <= B_FGRAD) {
     using S = GatherMfmaShape<MT, MODE>;
     (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)S::LDS);

@@ -332,7 +332,7 @@ int build_items(const int* tile_start, long ntiles, int dense_min, unsigned* chu
 
 int cols_pad(int m, bool is_double, const int* ts) {
   if (m < 2 || m > 8 || (is_double && m > 6)) return 0;
-  const int TS = m <= 6 ? 8 : 4;
+  const int TS = std_tile(m, is_double);
   if (ts[0] != TS || ts[1] != TS || ts[2] != TS) return 0;
   return TS + 2 * m;
 }

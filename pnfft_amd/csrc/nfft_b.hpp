@@ -50,11 +50,17 @@ struct BArgs {
   hipEvent_t mid2;         // host only: recorded between the second and the third kernel of the stage
 };
 
+// Tile size (cells per axis) of the standard binning layout: 8 for m <= 4 and
+// FP64 m = 5, 6; 4 for FP32 m = 5, 6 (padded subgrids of 4 + 2m <= 16 cells,
+// the matrix-core kernels' shape) and for m >= 7 (padded subgrids of FP64
+// accumulators within the LDS).
+constexpr int std_tile(int m, bool is_double) { return (m <= 4 || (m <= 6 && is_double)) ? 8 : 4; }
+
 constexpr int kTileThreads = 512;  // threads per block of the tiled kernels
 constexpr int kTileNodes = 512;    // nodes per work item
 
 // Column scatter: applies for m = 2..8 (FP64: m <= 6) when all three tile
-// sizes equal the layout's TS (8 for m <= 6, 4 above); P = TS + 2m cells per
+// sizes equal the layout's TS (std_tile); P = TS + 2m cells per
 // axis of a tile's padded subgrid.  Returns P, or 0 if the path does not apply.
 int cols_pad(int m, bool is_double, const int* ts);
 

@@ -35,7 +35,7 @@ using namespace bdev;
 template <class R, int MT, int MODE>
 struct ColsShape {
   static constexpr int CM = 2 * MT + 1;
-  static constexpr int TS = MT <= 6 ? 8 : 4;
+  static constexpr int TS = std_tile(MT, sizeof(R) == 8);
   static constexpr int P = TS + 2 * MT;  // even
   static constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
   static constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
@@ -206,7 +206,7 @@ __global__ __launch_bounds__((ColsShape<R, MT, MODE>::NT)) void scatter_cols_k(B
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-core form (FP32 plans, P <= 16, i.e. m <= 4): the tile update
+// Matrix-core form (FP32 plans, P = TS + 2m <= 16, i.e. m <= 6): the tile update
 //   G[a][b][c] += sum_v X_v[a] Y_v[b] W_v Z_v[c]
 // over rank-one terms v (one per node for f; three for grad, (psi'x, psi y,
 // psi z; g0), (psi x, psi'y, psi z; g1), (psi x, psi y, psi'z; g2); four for
@@ -350,7 +350,8 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
             dw[s] = WG ? L.dpsi[s] : 0.f;
           }
         } else {
-          window_axis_s<float, CM, WG>(A.g.win, na.frac, (float)nt, bt, w, dw);
+          // inlined: an out-of-line call would keep w and dw in scratch memory
+          [[clang::always_inline]] window_axis_s<float, CM, WG>(A.g.win, na.frac, (float)nt, bt, w, dw);
 #pragma unroll
           for (int s = 0; s < CM; s++) {
             w[s] *= wst;

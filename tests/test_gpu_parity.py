@@ -216,10 +216,12 @@ def test_direct_ndft_matches_oracle(gpu, oracle):
     dp.close()
 
 
-def _tile_keys(x, n, m):
+def _tile_keys(x, n, m, precision):
+    # tiles of 8 cells per axis for m <= 4 (and FP64 m <= 6), of 4 cells otherwise; halved
+    # on axes the tile size does not divide
     ts = []
     for nt in n:
-        t = 8
+        t = 8 if m <= 4 or (m <= 6 and precision == "double") else 4
         while t > 1 and nt % t:
             t //= 2
         ts.append(t)
@@ -243,7 +245,7 @@ def test_binning_bit_exact(gpu, cfg, M, precision):
     order, keys = dp.node_bins()
     order = order.cpu().numpy()
     keys = keys.cpu().numpy().view(np.uint32).astype(np.int64)
-    k = _tile_keys(x, w["n"], w["m"])
+    k = _tile_keys(x, w["n"], w["m"], precision)
     ref = np.argsort(k, kind="stable")
     assert np.array_equal(order, ref)
     assert np.array_equal(keys, k[ref])
@@ -287,7 +289,8 @@ def test_precompute_then_move_nodes(gpu, oracle):
 def test_stale_bins_after_in_place_move(gpu, oracle, precision, tol, m):
     """precompute_psi bins reused after x changed in place (no set_x): nodes that left their
     tiles go through the direct paths, trafo (f and grad) and adj still match the oracle for the
-    new x (m = 4 in FP32: the matrix-core gather and scatter)."""
+    new x (FP32: the matrix-core gather and scatter, on tiles of 4 cells at m = 6 and of 8 at
+    m = 4)."""
     w = make(1, M=1500)
     say("case stale bins", precision, m)
     P = oracle.Plan(w["N"], w["n"], m, 0)
@@ -402,27 +405,27 @@ def test_fp32_adjoint_of_large_gradients(gpu, oracle):
     assert rel(fh, P.adj(x, f=w["f"], grad=g)) < 1e-5
 
 
-@pytest.mark.parametrize("m", [2, 3, 4])
-def test_fp32_modes_clustered(gpu, oracle, m):
-    """FP32 trafo and adjoint of f, of grad_f and of both (for m <= 4 the matrix-core tile gather
-    and scatter; the scatter takes one, three and four rank-one terms per node) on clustered nodes:
-    tiles from one node to several hundred (several batches per tile), with and without
-    precomputed window tables."""
+@pytest.mark.parametrize("m,wname", [(2, "kb"), (3, "kb"), (4, "kb"), (5, "gauss"), (6, "kb"), (6, "bspline")])
+def test_fp32_modes_clustered(gpu, oracle, m, wname):
+    """FP32 trafo and adjoint of f, of grad_f and of both (the matrix-core tile gather and
+    scatter: tiles of 8 cells for m <= 4, of 4 cells for m = 5, 6; the scatter takes one, three
+    and four rank-one terms per node) on clustered nodes: tiles from one node to several hundred
+    (several batches per tile), with and without precomputed window tables."""
     from pnfft_amd.workloads import nodes
-    rng = SplitMix64(400 + m)
+    rng = SplitMix64(400 + m + WINDOWS[wname])
     M = 20000
     N, n = (32, 32, 32), (64, 64, 64)
     x = nodes("gaussmix", M, rng).astype(np.float32).astype(np.float64)
     fh = rng.complex_normal(int(np.prod(N)))
     fv = rng.complex_normal(M)
     gv = rng.complex_normal(3 * M)
-    P = oracle.Plan(N, n, m, 0)
+    P = oracle.Plan(N, n, m, WINDOWS[wname])
     F, G = PA.PNFFT_COMPUTE_F, PA.PNFFT_COMPUTE_GRAD_F
     f_o, g_o = P.trafo(x, fh, compute_grad=True)
     refs = {F: P.adj(x, f=fv), G: P.adj(x, grad=gv), F | G: P.adj(x, f=fv, grad=gv)}
     for pre in (0, PA.PNFFT_PRE_PSI | PA.PNFFT_PRE_GRAD_PSI):
-        say("case fp32 modes m", m, "pre", pre)
-        dp = DevicePlan(N, n, m, 0, M, precision="float")
+        say("case fp32 modes m", m, wname, "pre", pre)
+        dp = DevicePlan(N, n, m, WINDOWS[wname], M, precision="float")
         dp.set_x(x)
         if pre:
             dp.precompute(pre)
