@@ -563,6 +563,73 @@ __global__ __launch_bounds__(128) void fold8_k(BArgs<R> A) {
   }
 }
 
+// fold_k for tiles of 4^3 cells with P <= 16: one block of 128 threads per
+// brick of four grid tiles along axis 2 (4 x 4 x 16 cells), two consecutive
+// cells along axis 2 per thread.  For each neighbour offset (d0, d1) the seven
+// slots j = 0..6 along axis 2 whose subgrids reach the brick (tile T2 - 3 + j
+// for the brick's first tile T2) are read as whole 16-byte pairs: cell
+// c = 0..15 of the brick sits at p2 = c + 12 - 4j of slot j, and a wave reads
+// up to 128 contiguous bytes of a slot row.  Every cell sums the same
+// (slot, p2) pairs as fold_k (four slots along axis 2, as nt2 >= 4), in the
+// fixed order (d0, d1, j).
+template <class R, int P>
+__global__ __launch_bounds__(128) void fold4_k(BArgs<R> A) {
+  constexpr int TS = 4;
+  constexpr int D = (P + TS - 1) / TS;
+  static_assert(P % 2 == 0 && P <= 16, "pairs of cells inside a 16-cell subgrid");
+  constexpr size_t VOL = (size_t)P * P * P;
+  const int nt0 = A.ntile[0], nt1 = A.ntile[1], nt2 = A.ntile[2];
+  const int nb2 = nt2 >> 2;  // bricks along axis 2
+  const int B = blockIdx.x;
+  const int T2 = (B % nb2) * 4, T1 = (B / nb2) % nt1, T0 = B / (nb2 * nt1);
+  const int h = threadIdx.x & 7, q1 = (threadIdx.x >> 3) & 3, q0 = threadIdx.x >> 5;
+  const int c = 2 * h;  // first of the thread's two cells in the brick
+  double sr[2] = {0.0, 0.0}, si[2] = {0.0, 0.0};
+#pragma unroll
+  for (int d0 = 0; d0 < D; d0++) {
+    const int s0 = (T0 - d0 % nt0 + nt0) % nt0;
+    const int p0 = q0 + d0 * TS;
+#pragma unroll
+    for (int d1 = 0; d1 < D; d1++) {
+      const int s1 = (T1 - d1 % nt1 + nt1) % nt1;
+      const int p1 = q1 + d1 * TS;
+      const bool row_ok = p0 < P && p1 < P;
+#pragma unroll
+      for (int j = 0; j < 7; j++) {
+        const int s2 = ((T2 - 3 + j) % nt2 + nt2) % nt2;
+        const int slot = A.tile_slot[((long)s0 * nt1 + s1) * nt2 + s2];
+        if (slot < 0) continue;
+        const int p2 = c + 12 - 4 * j;
+        if (row_ok && p2 >= 0 && p2 < P) {
+          const Cplx<R>* src = A.slots + (size_t)slot * VOL + ((size_t)p0 * P + p1) * P + p2;
+          if constexpr (sizeof(R) == 4) {  // one aligned 16-byte load (P and p2 even)
+            const float4 v = *reinterpret_cast<const float4*>(src);
+            sr[0] += (double)v.x;
+            si[0] += (double)v.y;
+            sr[1] += (double)v.z;
+            si[1] += (double)v.w;
+          } else {
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+              sr[k] += (double)src[k].x;
+              si[k] += (double)src[k].y;
+            }
+          }
+        }
+      }
+    }
+  }
+  const long n1 = A.g.n[1], n2 = A.g.n[2];
+  const long x0 = (long)T0 * TS + q0, x1 = (long)T1 * TS + q1, x2 = (long)T2 * TS + c;
+  Cplx<R>* dst = A.grid + (x0 * n1 + x1) * n2 + x2;
+  if constexpr (sizeof(R) == 4) {
+    *reinterpret_cast<float4*>(dst) = make_float4((float)sr[0], (float)si[0], (float)sr[1], (float)si[1]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 2; k++) dst[k] = cmk<R>((R)sr[k], (R)si[k]);
+  }
+}
+
 // nodes whose stencil start left the tile they were binned to (bins older
 // than the coordinates), listed by the column kernel: added to the folded grid
 // through global atomics
@@ -600,10 +667,16 @@ void launch_m(const BArgs<R>& A, hipStream_t s) {
   if (!done) hipLaunchKernelGGL((scatter_cols_k<R, MT, MODE>), dim3((unsigned)A.n_torder), dim3(S::NT), 0, s, A);
   if (A.mid) (void)hipEventRecord(A.mid, s);
   const long ntiles = (long)A.ntile[0] * A.ntile[1] * A.ntile[2];
-  if constexpr (S::TS == 8)
+  if constexpr (S::TS == 8) {
     hipLaunchKernelGGL((fold8_k<R, S::P>), dim3((unsigned)ntiles), dim3(128), 0, s, A);
-  else
+  } else if constexpr (S::P <= 16) {
+    if (A.ts[2] == 4 && A.ntile[2] % 4 == 0)
+      hipLaunchKernelGGL((fold4_k<R, S::P>), dim3((unsigned)(ntiles / 4)), dim3(128), 0, s, A);
+    else
+      hipLaunchKernelGGL((fold_k<R, S::TS, S::P>), dim3((unsigned)ntiles), dim3(S::TS * S::TS * S::TS), 0, s, A);
+  } else {
     hipLaunchKernelGGL((fold_k<R, S::TS, S::P>), dim3((unsigned)ntiles), dim3(S::TS * S::TS * S::TS), 0, s, A);
+  }
   if (A.mid2) (void)hipEventRecord(A.mid2, s);
   hipLaunchKernelGGL((scatter_stale_k<R, MT, MODE>), dim3(256), dim3(256), 0, s, A);
 }
