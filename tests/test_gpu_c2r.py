@@ -117,6 +117,25 @@ def test_c2r_fp32(gpu, oracle, m):
     assert rel(g["adj"], P.adj_c2r(x, f=fv)) < 1e-5
 
 
+def test_c2r_particle_mesh_workload_fp32(gpu, oracle):
+    """The particle-mesh geometry in FP32 (N=64^3, n=128^3, Gaussian m=8 on 4-cell tiles,
+    charges +-1, 1/|k|^2 field): trafo (potential and field) and adj (charge spreading)."""
+    print("case c2r particle mesh fp32", flush=True)
+    w = make(4, M=1 << 14)
+    rng = SplitMix64(20261214)
+    N, n, m = w["N"], w["n"], w["m"]
+    M = w["M"]
+    x = w["x"].astype(np.float32).astype(np.float64)
+    q = np.where(np.arange(M) % 2 == 0, 1.0, -1.0)[rng.permutation(M)]
+    fh = coulomb_half(N, rng)
+    g = _run(N, n, m, w["flags"], x, fh, q, precision="float")
+    P = oracle.Plan(N, n, m, w["flags"], c2r=True)
+    f_o, g_o = P.trafo_c2r(x, fh, compute_grad=True)
+    assert rel(g["f"], f_o) < 1e-5
+    assert rel(g["grad"], g_o) < 1e-5
+    assert rel(g["adj"], P.adj_c2r(x, f=q)) < 1e-5
+
+
 @pytest.mark.skip(reason=FP64_M8_STALL)
 def test_c2r_particle_mesh_workload(gpu, oracle):
     """The particle-mesh geometry: N=64^3, n=128^3, Gaussian m=8, charges +-1, 1/|k|^2 field."""

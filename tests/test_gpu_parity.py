@@ -405,12 +405,14 @@ def test_fp32_adjoint_of_large_gradients(gpu, oracle):
     assert rel(fh, P.adj(x, f=w["f"], grad=g)) < 1e-5
 
 
-@pytest.mark.parametrize("m,wname", [(2, "kb"), (3, "kb"), (4, "kb"), (5, "gauss"), (6, "kb"), (6, "bspline")])
+@pytest.mark.parametrize("m,wname", [(2, "kb"), (3, "kb"), (4, "kb"), (5, "gauss"), (6, "kb"), (6, "bspline"),
+                                     (8, "gauss")])
 def test_fp32_modes_clustered(gpu, oracle, m, wname):
     """FP32 trafo and adjoint of f, of grad_f and of both (the matrix-core tile gather and
     scatter: tiles of 8 cells for m <= 4, of 4 cells for m = 5, 6; the scatter takes one, three
-    and four rank-one terms per node) on clustered nodes: tiles from one node to several hundred
-    (several batches per tile), with and without precomputed window tables."""
+    and four rank-one terms per node; at m = 8 the vector gather and column scatter on 4-cell
+    tiles) on clustered nodes: tiles from one node to several hundred (several batches per tile),
+    with and without precomputed window tables."""
     from pnfft_amd.workloads import nodes
     rng = SplitMix64(400 + m + WINDOWS[wname])
     M = 20000
