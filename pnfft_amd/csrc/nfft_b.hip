@@ -215,15 +215,36 @@ __global__ __launch_bounds__(kB) void tile_items_k(const unsigned* __restrict__ 
   }
 }
 
+// For each class of the wave's active lanes, one lane adds the class's lane
+// count to ctr[class] and every lane of the class gets the old value plus its
+// rank among them (lanes in order).  Neighbouring tiles mostly share a class,
+// so a wave issues a few atomics instead of one per tile.
+__device__ inline unsigned wave_class_add(bool act, int cls, unsigned* ctr) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  unsigned pos = 0;
+  unsigned long long todo = __ballot(act);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int c = __shfl(cls, leader);
+    const unsigned long long same = __ballot(act && cls == c);
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[c], (unsigned)__popcll(same));
+    base = __shfl(base, leader);
+    if (act && cls == c) pos = base + (unsigned)__popcll(same & lt_mask);
+    todo &= ~same;
+  }
+  return pos;
+}
+
 // nonempty tiles by node-count class (leading zeros of the count: more nodes,
 // smaller class), then placed class by class: most nodes first
 __global__ __launch_bounds__(kB) void tile_class_k(const int* __restrict__ tile_start, long ntiles, unsigned* work,
                                                    int* tile_slot) {
   const long t = (long)blockIdx.x * kB + threadIdx.x;
-  if (t >= ntiles) return;
-  const int cnt = tile_start[t + 1] - tile_start[t];
-  tile_slot[t] = -1;
-  if (cnt > 0) atomicAdd(&work[__clz(cnt)], 1u);
+  const int cnt = t < ntiles ? tile_start[t + 1] - tile_start[t] : 0;
+  if (t < ntiles) tile_slot[t] = -1;
+  (void)wave_class_add(cnt > 0, cnt > 0 ? __clz(cnt) : 0, work);
 }
 
 // work[0..31] class sizes -> work[32..63] class offsets, work[64] total
@@ -240,12 +261,12 @@ __global__ void tile_class_scan_k(unsigned* work) {
 __global__ __launch_bounds__(kB) void tile_place_k(const int* __restrict__ tile_start, long ntiles, unsigned* work,
                                                    int* order, int* tile_slot) {
   const long t = (long)blockIdx.x * kB + threadIdx.x;
-  if (t >= ntiles) return;
-  const int cnt = tile_start[t + 1] - tile_start[t];
-  if (cnt <= 0) return;
-  const unsigned pos = atomicAdd(&work[32 + __clz(cnt)], 1u);
-  order[pos] = (int)t;
-  tile_slot[t] = (int)pos;
+  const int cnt = t < ntiles ? tile_start[t + 1] - tile_start[t] : 0;
+  const unsigned pos = wave_class_add(cnt > 0, cnt > 0 ? __clz(cnt) : 0, work + 32);
+  if (cnt > 0) {
+    order[pos] = (int)t;
+    tile_slot[t] = (int)pos;
+  }
 }
 
 __global__ void fill_uint_k(unsigned* p, long n, unsigned v) {
