@@ -388,6 +388,8 @@ def main():
         "config": {"workload": w["name"], "N": list(w["N"]), "n": list(w["n"]), "m": w["m"], "M": w["M"],
                    "window": "kaiser_bessel" if w["flags"] == 0 else w["flags"], "setup_ms": round(res["setup_ms"], 2)},
         "accuracy_rel_l2_vs_fp64": acc,
+        # the same rate with the node binning and window tables redone for every step
+        "value_incl_setup": w["M"] / ((res["ms_per_step"] + res["setup_ms"]) * 1e-3),
         "fp64_ms_per_step": fp64["ms_per_step"] if fp64 else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
