@@ -53,6 +53,8 @@ def lib():
         L.orc_window_param_b.argtypes = [lp, lp, ctypes.c_int, ctypes.c_uint, dp]
         L.orc_bessel_i0.argtypes = [ctypes.c_double]
         L.orc_bessel_i0.restype = ctypes.c_double
+        L.orc_bessel_i1.argtypes = [ctypes.c_double]
+        L.orc_bessel_i1.restype = ctypes.c_double
         L.orc_sinc.argtypes = [ctypes.c_double]
         L.orc_sinc.restype = ctypes.c_double
         L.orc_bspline.argtypes = [ctypes.c_int, ctypes.c_double]
@@ -120,6 +122,10 @@ def window_b(N, n, m, flags):
 
 def bessel_i0(x):
     return lib().orc_bessel_i0(float(x))
+
+
+def bessel_i1(x):
+    return lib().orc_bessel_i1(float(x))
 
 
 def sinc(x):

@@ -95,6 +95,37 @@ double orc_bessel_i0(double x)
   return exp(ax) / sqrt(2.0 * ORC_PI * ax) * sum;
 }
 
+/* Modified Bessel function I1 in FP64 (the reference calls GSL,
+ * kernel/bessel_i1.c:567).  Restated from the mathematics: the power series
+ * (x/2) sum_k (x^2/4)^k/(k!(k+1)!) for |x| <= 40, and the Hankel expansion
+ * e^x/sqrt(2 pi x) sum_k t_k, t_k = t_{k-1} ((2k-1)^2 - 4)/(8 k x) beyond
+ * (Abramowitz & Stegun 9.7.1), truncated at its smallest term.  Odd in x. */
+double orc_bessel_i1(double x)
+{
+  double ax = fabs(x), r;
+  if (ax <= 40.0) {
+    double q = 0.25 * ax * ax, term = 1.0, sum = 1.0;
+    for (int k = 1; k < 500; k++) {
+      term *= q / ((double)k * (double)(k + 1));
+      sum += term;
+      if (term < 1e-18 * sum) break;
+    }
+    r = 0.5 * ax * sum;
+  } else {
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 60; k++) {
+      double c = (2.0 * k - 1.0);
+      double next = term * (c * c - 4.0) / (8.0 * k * ax);
+      if (fabs(next) >= fabs(term)) break;
+      term = next;
+      sum += term;
+      if (fabs(term) < 1e-18 * fabs(sum)) break;
+    }
+    r = exp(ax) / sqrt(2.0 * ORC_PI * ax) * sum;
+  }
+  return x < 0 ? -r : r;
+}
+
 /* sin(x)/x with the Taylor branches of kernel/sinc.c:26-52 (thresholds
  * eps, eps^(1/2), eps^(1/4) with eps = DBL_EPSILON). */
 double orc_sinc(double x)
@@ -201,6 +232,25 @@ static double kb_d1d(double xn, long n, double b, int m, double psi) /* :2256-22
                  : -(double)n * m * b * b * b / (3.0 * ORC_PI);
 }
 
+/* Bessel-I0 window, window_bessel_i0_1d / _derivative_1d of
+ * kernel/ndft-parallel.c:2195-2218 with xn = x*n: I0(b sqrt(m^2 - xn^2))/2 on
+ * the support, -(b/2) n xn I1(b r)/r for the derivative (-(b n/2)^2 xn/n at
+ * r = 0). */
+static double bi0_1d(double xn, double b, int m)
+{
+  double d = (double)m * m - xn * xn;
+  return (d < 0) ? 0.0 : 0.5 * orc_bessel_i0(b * sqrt(d));
+}
+
+static double bi0_d1d(double xn, long n, double b, int m)
+{
+  double d = (double)m * m - xn * xn;
+  if (d < 0) return 0.0;
+  double r = sqrt(d);
+  return (d > 0) ? -0.5 * b * (double)n * xn * orc_bessel_i1(b * r) / r
+                 : -(0.5 * b * n) * (0.5 * b * n) * xn / (double)n;
+}
+
 void orc_psi_axis(int window, long n, double b, int m, double nx,
                   double *psi, double *dpsi)
 {
@@ -254,6 +304,13 @@ void orc_psi_axis(int window, long n, double b, int m, double nx,
       }
       break;
     }
+    case W_BESSEL_I0: { /* pre_psi_tensor_bessel_i0 :1765-1781, pre_dpsi :1919-1935 */
+      double uj = nx - fl + m;
+      for (int s = 0; s < c; s++) psi[s] = bi0_1d(uj - s, b, m);
+      if (dpsi)
+        for (int s = 0; s < c; s++) dpsi[s] = bi0_d1d(uj - s, n, b, m);
+      break;
+    }
     default: { /* Kaiser-Bessel, :1791-1796 and :1946-1951 */
       for (int s = 0; s < c; s++) {
         double u = (fl - nx - m) + s;
@@ -276,6 +333,7 @@ double orc_psi(int window, long n, double b, int m, double x)
     case W_GAUSS: return exp(-(xn * xn) / b) / sqrt(ORC_PI * b);
     case W_BSPLINE: return orc_bspline(2 * m, xn + m);
     case W_SINCPOW: return pow(orc_sinc(ORC_PI * xn / b), 2.0 * m) / b;
+    case W_BESSEL_I0: return bi0_1d(xn, b, m);
     default: return kb_1d(xn, b, m);
   }
 }
@@ -294,6 +352,7 @@ double orc_dpsi(int window, long n, double b, int m, double x)
                orc_psi(window, n, b, m, x);
       return 0.0;
     }
+    case W_BESSEL_I0: return bi0_d1d(xn, n, b, m);
     default: return kb_d1d(xn, n, b, m, kb_1d(xn, b, m));
   }
 }

@@ -17,7 +17,7 @@ import scipy.special
 
 from pnfft_amd.workloads import SplitMix64
 
-WINDOWS = {"kb": 0, "gauss": 1 << 13, "bspline": 1 << 14, "sincpow": 1 << 15}
+WINDOWS = {"kb": 0, "gauss": 1 << 13, "bspline": 1 << 14, "sincpow": 1 << 15, "besseli0": 1 << 16}
 
 
 # ---------------------------------------------------------------- special functions
@@ -28,6 +28,17 @@ def test_bessel_i0_vs_mpmath_and_scipy(oracle):
         got = oracle.bessel_i0(x)
         assert abs(got - ref) <= 4e-15 * ref, (x, got, ref)
         assert abs(got - scipy.special.i0(x)) <= 1e-14 * ref
+
+
+def test_bessel_i1_vs_mpmath_and_scipy(oracle):
+    """I1 of the Bessel-I0 window's derivative: series below 40, Hankel expansion above, odd."""
+    xs = np.concatenate([np.linspace(0, 45, 181), [0.1, 1e-3, 17.77, 28.27, 39.99, 40.01, 60.0, 120.0, 300.0]])
+    for x in xs:
+        ref = float(mpmath.besseli(1, mpmath.mpf(float(x))))
+        got = oracle.bessel_i1(x)
+        assert abs(got - ref) <= 4e-15 * max(ref, 1e-300), (x, got, ref)
+        assert abs(got - scipy.special.i1(x)) <= 1e-14 * max(ref, 1e-300)
+        assert oracle.bessel_i1(-x) == -got
 
 
 def test_sinc_branches(oracle):
@@ -176,7 +187,7 @@ def test_ndft_adjointness(oracle):
 
 # ---------------------------------------------------------------- fast NFFT vs NDFT
 # thresholds: each window's accuracy at sigma = 2, m = 6 (relative l2), with margin
-ACCURACY = {"kb": 1e-10, "gauss": 2e-5, "bspline": 8e-6, "sincpow": 3e-6}
+ACCURACY = {"kb": 1e-10, "gauss": 2e-5, "bspline": 8e-6, "sincpow": 3e-6, "besseli0": 2e-10}
 
 
 @pytest.mark.parametrize("wname", list(WINDOWS))
