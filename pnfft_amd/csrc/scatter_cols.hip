@@ -563,6 +563,76 @@ __global__ __launch_bounds__(128) void fold8_k(BArgs<R> A) {
   }
 }
 
+// fold8_k for P <= 16 as bricks of two grid tiles along axis 2 (8 x 8 x 16
+// cells, 256 threads, four consecutive cells along axis 2 per thread): a wave
+// writes 128 contiguous bytes of a grid row (FP32).  Slot j = 0..2 along axis 2
+// is tile T2 - 1 + j for the brick's first tile T2; cell c = 0..15 of the
+// brick sits at p2 = c + 8 - 8j of it.  Every cell sums the same (slot, p2)
+// pairs as fold8_k (two slots along axis 2, as nt2 >= 2), in the fixed order
+// (d0, d1, j).
+template <class R, int P>
+__global__ __launch_bounds__(256) void fold8b_k(BArgs<R> A) {
+  constexpr int TS = 8;
+  constexpr int D = (P + TS - 1) / TS;
+  static_assert(P % 4 == 0 && P <= 16, "aligned quads of cells inside a 16-cell subgrid");
+  constexpr size_t VOL = (size_t)P * P * P;
+  const int nt0 = A.ntile[0], nt1 = A.ntile[1], nt2 = A.ntile[2];
+  const int nb2 = nt2 >> 1;  // bricks along axis 2
+  const int B = blockIdx.x;
+  const int T2 = (B % nb2) * 2, T1 = (B / nb2) % nt1, T0 = B / (nb2 * nt1);
+  const int h = threadIdx.x & 3, q1 = (threadIdx.x >> 2) & 7, q0 = threadIdx.x >> 5;
+  const int c = 4 * h;  // first of the thread's four cells in the brick
+  double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int d0 = 0; d0 < D; d0++) {
+    const int s0 = (T0 - d0 % nt0 + nt0) % nt0;
+    const int p0 = q0 + d0 * TS;
+#pragma unroll
+    for (int d1 = 0; d1 < D; d1++) {
+      const int s1 = (T1 - d1 % nt1 + nt1) % nt1;
+      const int p1 = q1 + d1 * TS;
+      const bool row_ok = p0 < P && p1 < P;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const int s2 = ((T2 - 1 + j) % nt2 + nt2) % nt2;
+        const int slot = A.tile_slot[((long)s0 * nt1 + s1) * nt2 + s2];
+        if (slot < 0) continue;
+        const int p2 = c + 8 - 8 * j;
+        if (row_ok && p2 >= 0 && p2 < P) {
+          const Cplx<R>* src = A.slots + (size_t)slot * VOL + ((size_t)p0 * P + p1) * P + p2;
+          if constexpr (sizeof(R) == 4) {  // two aligned 16-byte loads (P and p2 multiples of 4)
+            const float4 a = reinterpret_cast<const float4*>(src)[0], b = reinterpret_cast<const float4*>(src)[1];
+            sr[0] += (double)a.x;
+            si[0] += (double)a.y;
+            sr[1] += (double)a.z;
+            si[1] += (double)a.w;
+            sr[2] += (double)b.x;
+            si[2] += (double)b.y;
+            sr[3] += (double)b.z;
+            si[3] += (double)b.w;
+          } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              sr[k] += (double)src[k].x;
+              si[k] += (double)src[k].y;
+            }
+          }
+        }
+      }
+    }
+  }
+  const long n1 = A.g.n[1], n2 = A.g.n[2];
+  const long x0 = (long)T0 * TS + q0, x1 = (long)T1 * TS + q1, x2 = (long)T2 * TS + c;
+  Cplx<R>* dst = A.grid + (x0 * n1 + x1) * n2 + x2;
+  if constexpr (sizeof(R) == 4) {
+    reinterpret_cast<float4*>(dst)[0] = make_float4((float)sr[0], (float)si[0], (float)sr[1], (float)si[1]);
+    reinterpret_cast<float4*>(dst)[1] = make_float4((float)sr[2], (float)si[2], (float)sr[3], (float)si[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[k] = cmk<R>((R)sr[k], (R)si[k]);
+  }
+}
+
 // fold_k for tiles of 4^3 cells with P <= 16: one block of 128 threads per
 // brick of four grid tiles along axis 2 (4 x 4 x 16 cells), two consecutive
 // cells along axis 2 per thread.  For each neighbour offset (d0, d1) the seven
@@ -668,7 +738,14 @@ void launch_m(const BArgs<R>& A, hipStream_t s) {
   if (A.mid) (void)hipEventRecord(A.mid, s);
   const long ntiles = (long)A.ntile[0] * A.ntile[1] * A.ntile[2];
   if constexpr (S::TS == 8) {
-    hipLaunchKernelGGL((fold8_k<R, S::P>), dim3((unsigned)ntiles), dim3(128), 0, s, A);
+    bool brick = false;
+    if constexpr (S::P <= 16 && S::P % 4 == 0) {
+      if (A.ts[2] == 8 && A.ntile[2] % 2 == 0) {
+        hipLaunchKernelGGL((fold8b_k<R, S::P>), dim3((unsigned)(ntiles / 2)), dim3(256), 0, s, A);
+        brick = true;
+      }
+    }
+    if (!brick) hipLaunchKernelGGL((fold8_k<R, S::P>), dim3((unsigned)ntiles), dim3(128), 0, s, A);
   } else if constexpr (S::P <= 16) {
     if (A.ts[2] == 4 && A.ntile[2] % 4 == 0)
       hipLaunchKernelGGL((fold4_k<R, S::P>), dim3((unsigned)(ntiles / 4)), dim3(128), 0, s, A);
