@@ -273,6 +273,7 @@ struct NodesT {
   Cplx<R>* d_slots = nullptr;
   size_t slots_cap = 0;  // complex elements
   int n_torder = 0;
+  int n_heavy = 0;       // torder[0, n_heavy): tiles with 32 nodes or more
   int* d_stale = nullptr;            // [M] sorted positions of nodes outside their tile
   unsigned* d_stale_count = nullptr;
   // window tables of precompute_psi (sorted order, see precompute_tables)
@@ -629,6 +630,7 @@ static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false)
   if (D->n_torder > 0) {
     A.torder = D->d_torder;
     A.n_torder = D->n_torder;
+    A.n_heavy = D->n_heavy;
     A.stale_count = D->d_stale_count;
     A.stale_list = D->d_stale;
   }
@@ -675,6 +677,7 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx) {
   const int pad = cols_enabled() ? cols_pad(P->m, sizeof(R) == 8, P->tiles.ts) : 0;
   if (pad > 0 && D->M > 0) {
     const int nt = build_tile_order(D->d_tile_start, P->tiles.ntiles(), D->d_twork, D->d_torder, D->d_tile_slot,
+                                    &D->n_heavy,
                                     P->stream);
     if (nt < 0) {
       set_error("building the tile order failed");

@@ -333,14 +333,17 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int MT, int MODE>
+// NW waves per block: 4, or 2 for tiles of at most 31 nodes (two groups), where
+// two more waves would have no group to take
+template <int MT, int MODE, int NW = 4>
 struct GatherMfmaShape {
   static constexpr int CM = 2 * MT + 1, TS = std_tile(MT, false), P = TS + 2 * MT;
   static constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
   static constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
   static constexpr int ROWS = WG ? 6 : 3;  // window rows per node: x, y, z (and x', y', z')
   static constexpr int WAVES = WG ? 2 : 3;  // waves per SIMD the kernel is compiled for
-  static constexpr size_t LDS = sizeof(float) * (2 * P * 256 + 4 * ROWS * 256);
+  static constexpr int NT = 64 * NW;        // threads per block
+  static constexpr size_t LDS = sizeof(float) * (2 * P * 256 + NW * ROWS * 256);
 };
 
 // one lane's share of a node's window data: axis lk of node li of a group
@@ -353,10 +356,12 @@ struct AxisLoad {
   float dpsi[WG ? CM : 1];
 };
 
-template <int MT, int MODE>
-__global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gather_mfma_k(BArgs<float> A) {
-  using S = GatherMfmaShape<MT, MODE>;
-  constexpr int CM = S::CM, TS = S::TS, P = S::P;
+// blocks take the tiles torder[first + blockIdx.x]
+template <int MT, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) void gather_mfma_k(BArgs<float> A,
+                                                                                                   int first) {
+  using S = GatherMfmaShape<MT, MODE, NW>;
+  constexpr int CM = S::CM, TS = S::TS, P = S::P, NT = S::NT;
   constexpr bool WG = S::WG;
   static_assert(P <= 16, "one 16 x 16 block per plane");
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
   // this wave's window rows: row r (0..2: x, y, z; 3..5: x', y', z') of node i at (r * 16 + i) * 16
   float* rows = s_gi + P * 256 + wave * S::ROWS * 256;
 
-  const int tile = A.torder[blockIdx.x];
+  const int tile = A.torder[first + blockIdx.x];
   const int t2 = tile % A.ntile[2];
   const int t1 = (tile / A.ntile[2]) % A.ntile[1];
   const int t0 = tile / (A.ntile[2] * A.ntile[1]);
@@ -377,17 +382,17 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
 
   // stage the subgrid, split into real and imaginary parts; pairs of cells along c
   if constexpr (P < 16) {
-    for (int q = tid; q < 2 * P * 256 / 4; q += 256)
+    for (int q = tid; q < 2 * P * 256 / 4; q += NT)
       reinterpret_cast<float4*>(s_gr)[q] = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
   }
   {
-    constexpr int H = P / 2, NPAIR = P * P * H, K = (NPAIR + 255) / 256;
+    constexpr int H = P / 2, NPAIR = P * P * H, K = (NPAIR + NT - 1) / NT;
     const float4* src = reinterpret_cast<const float4*>(A.grid);
     float4 v[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const int q = tid + 256 * k;
+      const int q = tid + NT * k;
       if (q < NPAIR) {
         const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
         const unsigned g0 = (unsigned)(o0 + cx) & (n0 - 1), g1 = (unsigned)(o1 + cy) & (n1 - 1);
@@ -397,7 +402,7 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
     }
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const int q = tid + 256 * k;
+      const int q = tid + NT * k;
       if (q < NPAIR) {
         const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
         // cells c, c + 1 (c even) stay adjacent under the swizzle
@@ -447,10 +452,10 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
     const int j0 = perm_raw(g);
     load_group(g, perm_ok(g) ? j0 : -1, L);
   }
-  int jn = perm_raw(g + 4 < ngroups ? g + 4 : g);
-  bool jn_ok = g + 4 < ngroups && perm_ok(g + 4);
+  int jn = perm_raw(g + NW < ngroups ? g + NW : g);
+  bool jn_ok = g + NW < ngroups && perm_ok(g + NW);
 
-  for (; g < ngroups; g += 4) {
+  for (; g < ngroups; g += NW) {
     // windows of this lane's axis for node li of the group
     const int jg = L.j;
     float w[CM], dw[CM];
@@ -511,10 +516,10 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
 
     // loads for the wave's next group, in flight during the matrix-core phase
     AxisLoad<CM, WG> Ln;
-    const int gn = g + 4 < ngroups ? g + 4 : g;
+    const int gn = g + NW < ngroups ? g + NW : g;
     load_group(gn, jn_ok ? jn : -1, Ln);
-    const int jnn = perm_raw(g + 8 < ngroups ? g + 8 : g);
-    const bool jnn_ok = g + 8 < ngroups && perm_ok(g + 8);
+    const int jnn = perm_raw(g + 2 * NW < ngroups ? g + 2 * NW : g);
+    const bool jnn_ok = g + 2 * NW < ngroups && perm_ok(g + 2 * NW);
 
     const float* xr = rows + li * 16;
     const float* dxr = rows + (48 + li) * 16;
@@ -605,7 +610,7 @@ __global__ __launch_bounds__(256, (GatherMfmaShape<MT, MODE>::WAVES)) void gathe
       }
     }
     L = Ln;
-    if (g + 4 >= ngroups) L.j = -1;
+    if (g + NW >= ngroups) L.j = -1;
     jn = jnn;
     jn_ok = jnn_ok;
     wave_sync();
@@ -641,7 +646,7 @@ bool mfma_gather(const BArgs<R>& A) {
   if constexpr (sizeof(R) != 4 || MT > 6 || MODE > B_FGRAD) {
     return false;
   } else {
-    constexpr int TS = GatherMfmaShape<MT, MODE>::TS;
+    constexpr int TS = GatherMfmaShape<MT, MODE, 4>::TS;
     return !A.no_mfma && A.torder && A.n_torder > 0 && A.stale_count && A.stale_list && A.ts[0] == TS &&
            A.ts[1] == TS && A.ts[2] == TS;
   }
@@ -650,11 +655,21 @@ bool mfma_gather(const BArgs<R>& A) {
 template <class R, int MT, int MODE>
 void launch_gather_mfma(const BArgs<R>& A, hipStream_t s) {
   if constexpr (sizeof(R) == 4 && MT <= 6 && MODE <= B_FGRAD) {
-    using S = GatherMfmaShape<MT, MODE>;
-    (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)S::LDS);
+    using S4 = GatherMfmaShape<MT, MODE, 4>;
+    using S2 = GatherMfmaShape<MT, MODE, 2>;
+    const int nh = min(max(A.n_heavy, 0), A.n_torder);
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, A.stale_count);
-    hipLaunchKernelGGL((gather_mfma_k<MT, MODE>), dim3((unsigned)A.n_torder), dim3(256), S::LDS, s, A);
+    if (nh > 0) {
+      (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)S4::LDS);
+      hipLaunchKernelGGL((gather_mfma_k<MT, MODE, 4>), dim3((unsigned)nh), dim3(S4::NT), S4::LDS, s, A, 0);
+    }
+    if (A.n_torder > nh) {  // tiles of at most 31 nodes: two groups, two waves
+      (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)S2::LDS);
+      hipLaunchKernelGGL((gather_mfma_k<MT, MODE, 2>), dim3((unsigned)(A.n_torder - nh)), dim3(S2::NT), S2::LDS, s,
+                         A, nh);
+    }
     if (A.mid) (void)hipEventRecord(A.mid, s);
     if (A.mid2) (void)hipEventRecord(A.mid2, s);
     hipLaunchKernelGGL((gather_stale_k<MT, MODE>), dim3(256), dim3(256), 0, s, A);

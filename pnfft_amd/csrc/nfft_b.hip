@@ -358,16 +358,19 @@ int cols_pad(int m, bool is_double, const int* ts) {
   return TS + 2 * m;
 }
 
-int build_tile_order(const int* tile_start, long ntiles, unsigned* work, int* order, int* tile_slot, hipStream_t s) {
+int build_tile_order(const int* tile_start, long ntiles, unsigned* work, int* order, int* tile_slot, int* n_heavy,
+                     hipStream_t s) {
   hipLaunchKernelGGL(fill_uint_k, dim3(1), dim3(128), 0, s, work, 65L, 0u);
   hipLaunchKernelGGL(tile_class_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, tile_slot);
   hipLaunchKernelGGL(tile_class_scan_k, dim3(1), dim3(64), 0, s, work);
   hipLaunchKernelGGL(tile_place_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, order,
                      tile_slot);
-  unsigned total = 0;
-  if (hipMemcpyAsync(&total, work + 64, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  // after the placement work[32 + k] is the end of class k; classes k <= 26 hold 32 nodes or more
+  unsigned ends[33] = {0};
+  if (hipMemcpyAsync(ends, work + 32, 33 * sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  return (int)total;
+  if (n_heavy) *n_heavy = (int)ends[26];
+  return (int)ends[32];
 }
 
 template <class R>

@@ -36,6 +36,7 @@ struct BArgs {
   // nonempty tiles, most nodes first (column scatter and matrix-core gather)
   const int* torder;       // [n_torder]
   int n_torder;
+  int n_heavy;             // tiles torder[0, n_heavy) hold 32 nodes or more, the rest at most 31
   // column scatter (scatter_cols.hip): when slots is set, scatter() runs it instead of the LDS-atomic path
   Cplx<R>* slots;          // [n_torder][P^3] padded subgrid of every nonempty tile
   const int* tile_slot;    // [ntiles] slot of each tile, -1 if empty
@@ -66,9 +67,11 @@ int cols_pad(int m, bool is_double, const int* ts);
 
 // Nonempty tiles ordered by decreasing node count (power-of-two classes, most
 // nodes first) and tile_slot[t] = position of tile t in that order (-1 if
-// empty).  work holds 64 unsigned.  Returns the number of nonempty tiles (host
-// value; synchronizes s), or -1.
-int build_tile_order(const int* tile_start, long ntiles, unsigned* work, int* order, int* tile_slot, hipStream_t s);
+// empty).  work holds 65 unsigned.  Returns the number of nonempty tiles (host
+// value; synchronizes s), or -1; *n_heavy = the number of them with 32 nodes
+// or more (they come first).
+int build_tile_order(const int* tile_start, long ntiles, unsigned* work, int* order, int* tile_slot, int* n_heavy,
+                     hipStream_t s);
 
 // Launch of the column scatter for MODE (B_F, B_GRAD, B_FGRAD): column kernel
 // into the slots, fold of the slots into the grid (which it overwrites), then
