@@ -676,8 +676,8 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx) {
   D->n_torder = 0;
   const int pad = cols_enabled() ? cols_pad(P->m, sizeof(R) == 8, P->tiles.ts) : 0;
   if (pad > 0 && D->M > 0) {
-    const int nt = build_tile_order(D->d_tile_start, P->tiles.ntiles(), D->d_twork, D->d_torder, D->d_tile_slot,
-                                    &D->n_heavy,
+    const int nt = build_tile_order(D->d_tile_start, P->tiles.ntiles(), D->d_twork, D->d_chunks, D->d_tpartial,
+                                    D->d_torder, D->d_tile_slot, &D->n_heavy,
                                     P->stream);
     if (nt < 0) {
       set_error("building the tile order failed");

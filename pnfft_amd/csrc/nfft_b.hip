@@ -258,14 +258,28 @@ __global__ void tile_class_scan_k(unsigned* work) {
   work[64] = acc;
 }
 
+// tiles of 1..31 nodes (classes 27..31): flag 1, else 0
+__global__ __launch_bounds__(kB) void tile_light_k(const int* __restrict__ tile_start, long ntiles, unsigned* flag) {
+  const long t = (long)blockIdx.x * kB + threadIdx.x;
+  if (t >= ntiles) return;
+  const int cnt = tile_start[t + 1] - tile_start[t];
+  flag[t] = (cnt > 0 && cnt < 32) ? 1u : 0u;
+}
+
+// Tiles of 32 nodes or more are placed class by class (most nodes first); the
+// tiles of 1..31 nodes follow in tile order (light_off = exclusive scan of
+// their flags), so that neighbouring light tiles, whose padded subgrids
+// overlap, are taken by neighbouring blocks.
 __global__ __launch_bounds__(kB) void tile_place_k(const int* __restrict__ tile_start, long ntiles, unsigned* work,
-                                                   int* order, int* tile_slot) {
+                                                   const unsigned* __restrict__ light_off, int* order,
+                                                   int* tile_slot) {
   const long t = (long)blockIdx.x * kB + threadIdx.x;
   const int cnt = t < ntiles ? tile_start[t + 1] - tile_start[t] : 0;
-  const unsigned pos = wave_class_add(cnt > 0, cnt > 0 ? __clz(cnt) : 0, work + 32);
+  const unsigned pos = wave_class_add(cnt >= 32, cnt > 0 ? __clz(cnt) : 0, work + 32);
   if (cnt > 0) {
-    order[pos] = (int)t;
-    tile_slot[t] = (int)pos;
+    const unsigned p = cnt >= 32 ? pos : work[32 + 27] + light_off[t];  // class 27 starts after the heavy tiles
+    order[p] = (int)t;
+    tile_slot[t] = (int)p;
   }
 }
 
@@ -358,12 +372,14 @@ int cols_pad(int m, bool is_double, const int* ts) {
   return TS + 2 * m;
 }
 
-int build_tile_order(const int* tile_start, long ntiles, unsigned* work, int* order, int* tile_slot, int* n_heavy,
-                     hipStream_t s) {
+int build_tile_order(const int* tile_start, long ntiles, unsigned* work, unsigned* light, unsigned* partial,
+                     int* order, int* tile_slot, int* n_heavy, hipStream_t s) {
   hipLaunchKernelGGL(fill_uint_k, dim3(1), dim3(128), 0, s, work, 65L, 0u);
   hipLaunchKernelGGL(tile_class_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, tile_slot);
   hipLaunchKernelGGL(tile_class_scan_k, dim3(1), dim3(64), 0, s, work);
-  hipLaunchKernelGGL(tile_place_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, order,
+  hipLaunchKernelGGL(tile_light_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, light);
+  scan_exclusive(light, ntiles, partial, s);
+  hipLaunchKernelGGL(tile_place_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, light, order,
                      tile_slot);
   // after the placement work[32 + k] is the end of class k; classes k <= 26 hold 32 nodes or more
   unsigned ends[33] = {0};

@@ -65,13 +65,14 @@ constexpr int kTileNodes = 512;    // nodes per work item
 // axis of a tile's padded subgrid.  Returns P, or 0 if the path does not apply.
 int cols_pad(int m, bool is_double, const int* ts);
 
-// Nonempty tiles ordered by decreasing node count (power-of-two classes, most
-// nodes first) and tile_slot[t] = position of tile t in that order (-1 if
-// empty).  work holds 65 unsigned.  Returns the number of nonempty tiles (host
-// value; synchronizes s), or -1; *n_heavy = the number of them with 32 nodes
-// or more (they come first).
-int build_tile_order(const int* tile_start, long ntiles, unsigned* work, int* order, int* tile_slot, int* n_heavy,
-                     hipStream_t s);
+// Nonempty tiles: those of 32 nodes or more by decreasing node count
+// (power-of-two classes, most nodes first), then those of 1..31 nodes in tile
+// order; tile_slot[t] = position of tile t (-1 if empty).  work holds 65
+// unsigned, light ntiles unsigned, partial the scan partials of ntiles values.
+// Returns the number of nonempty tiles (host value; synchronizes s), or -1;
+// *n_heavy = the number of them with 32 nodes or more.
+int build_tile_order(const int* tile_start, long ntiles, unsigned* work, unsigned* light, unsigned* partial,
+                     int* order, int* tile_slot, int* n_heavy, hipStream_t s);
 
 // Launch of the column scatter for MODE (B_F, B_GRAD, B_FGRAD): column kernel
 // into the slots, fold of the slots into the grid (which it overwrites), then
