@@ -80,12 +80,13 @@ def roofline_from_kernels(kt, ab, af, steps, precision, m=4):
     kt: {class: (ms, launches)} over the timed steps (HIP events on the plan's stream).  One step
     launches: FFT rows 2 forward + 1 backward, cols1 3, cols0 3 (HBM-bound: GB/s of algorithmic
     bytes); gather stage 2 (f, grad) and scatter stage 1 (subgrid kernel, fold, stale nodes), whose
-    FP32 kernels run on the matrix cores for m <= 4 (TFLOP/s of algorithmic flops).
+    FP32 kernels run on the matrix cores for m <= 6 (TFLOP/s of algorithmic flops).
     """
     def ms(k):
         return kt[k][0] / steps
 
-    mfma = precision == "float" and m <= 4  # the matrix-core B kernels (FP32, m <= 4)
+    # the matrix-core B kernels: FP32 plans with m <= 6 (padded tile subgrids of at most 16 cells)
+    mfma = precision == "float" and m <= 6
     classes = {
         "fft_rows": (ms("fft_rows"), 2 * ab["fft_rows_fwd"] + ab["fft_rows_bwd"], 0, 3),
         "fft_cols1": (ms("fft_cols1"), 3 * ab["fft_cols1"], 0, 3),
@@ -232,9 +233,11 @@ def run_plan(w, precision, steps, warmup, x_np):
 
 
 def cpu_baseline(w, sample_M, threads):
-    """FP64 oracle (C restatement with OpenMP + scipy FFT) on the host cores."""
+    """FP64 oracle (C restatement with OpenMP + scipy FFT) on the host cores, built with the
+    reference's release flags for this host (-O3 -march=native -ffast-math)."""
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     from oracle import oracle as O
+    L = O.lib_fast()
 
     P = O.Plan(w["N"], w["n"], w["m"], w["flags"], workers=threads)
     xs = w["x"][:sample_M]
@@ -248,7 +251,6 @@ def cpu_baseline(w, sample_M, threads):
     import ctypes
     f = np.zeros(sample_M, np.complex128)
     gr = np.zeros(3 * sample_M, np.complex128)
-    L = O.lib()
     dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     lp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
     n3 = np.asarray(P.n, np.int64)
@@ -277,12 +279,13 @@ def cpu_baseline(w, sample_M, threads):
     scale = w["M"] / sample_M
     # D+F and F^H+D^T are independent of M (the step has two trafos and one adj); the B stages scale with M
     total = 2 * t["DF"] + t["FDT"] + (t["B"] + t["Bgrad"] + t["BT"]) * scale
-    return dict(value=w["M"] / total, unit="pts/s", cores=threads, kind="port",
+    return dict(value=w["M"] / total, unit="pts/s", cores=threads, nproc=os.cpu_count(), kind="port",
                 sample=(f"FP64 oracle (reference-algorithm restatement) on N={w['N'][0]}^3 n={w['n'][0]}^3 "
                         f"m={w['m']}: D+F once ({t['DF']:.2f}s), gather f and grad and adj on {sample_M} of "
                         f"{w['M']} nodes ({t['B']:.2f}+{t['Bgrad']:.2f}+{t['BT']:.2f}s) scaled by {scale:.0f} to "
-                        f"the full node count, F^H+D^T once ({t['FDT']:.2f}s); {threads} threads"),
-                seconds=t)
+                        f"the full node count, F^H+D^T once ({t['FDT']:.2f}s); {threads} threads, "
+                        f"oracle built -O3 -march=native -ffast-math"),
+                seconds=t, f=f, grad=gr)
 
 
 def copy_probe():
@@ -310,7 +313,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload's node count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22,
+                    help="nodes of the CPU baseline's B stages (a quarter of the headline workload)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (calibrates the HBM byte counters)")
     ap.add_argument("--pmc-traffic", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
@@ -372,12 +376,20 @@ def main():
                 "frac": rl["frac"], "traffic": traffic}
     roofline.update({k: v for k, v in rl.items() if k not in roofline})
     cpu = None
+    acc_oracle = None
     if not args.no_cpu:
         log("  cpu baseline ...")
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         threads = max(1, min(threads, os.cpu_count() or 1))
-        cpu = cpu_baseline(w, min(args.cpu_sample, w["M"]), threads)
+        sM = min(args.cpu_sample, w["M"])
+        cpu = cpu_baseline(w, sM, threads)
         cpu.pop("seconds", None)
+        # the GPU outputs of the same nodes against the FP64 oracle's (fast-math build)
+        f_o, g_o = cpu.pop("f"), cpu.pop("grad")
+        acc_oracle = {"f": float(np.linalg.norm(res["out"][0][:sM] - f_o) / np.linalg.norm(f_o)),
+                      "grad_f": float(np.linalg.norm(res["out"][2][:3 * sM] - g_o) / np.linalg.norm(g_o)),
+                      "nodes": sM}
+        log(f"  rel-l2 vs FP64 oracle on {sM} nodes: {acc_oracle}")
     value = w["M"] / (res["ms_per_step"] * 1e-3)
     line = {
         "metric": METRIC if args.config == 3 else
@@ -388,6 +400,7 @@ def main():
         "config": {"workload": w["name"], "N": list(w["N"]), "n": list(w["n"]), "m": w["m"], "M": w["M"],
                    "window": "kaiser_bessel" if w["flags"] == 0 else w["flags"], "setup_ms": round(res["setup_ms"], 2)},
         "accuracy_rel_l2_vs_fp64": acc,
+        "accuracy_rel_l2_vs_oracle": acc_oracle,
         # the same rate with the node binning and window tables redone for every step
         "value_incl_setup": w["M"] / ((res["ms_per_step"] + res["setup_ms"]) * 1e-3),
         "fp64_ms_per_step": fp64["ms_per_step"] if fp64 else None,

@@ -177,12 +177,19 @@ extern "C" {
    * stream) and launch counts accumulated since the plan was made or                         \
    * reset_timer was called, for the kernel classes PNFFT_KT_* below; copies                  \
    * min(n, PNFFT_KT_COUNT) entries and returns PNFFT_KT_COUNT.                               \
+   * oversampled_grid: device pointer to the plan's oversampled grid, n0*n1*n2                \
+   * complex values in natural DFT order (index l_t mod n_t, row-major); after                \
+   * trafo it holds F D f_hat (F f_hat with PNFFT_OMIT_DECONV), after adj B^T f.              \
+   * The plan's D factors and window values carry reciprocal powers of two per               \
+   * axis (2^e and 2^-e, e = exponent of psi(0)), so the grid is 2^e times the               \
+   * reference's g2 when D was applied (exact scaling; B F D is unchanged).                   \
    * last_error: message of the last rejected call, or NULL. */                               \
   void *PNX(malloc_device)(size_t n);                                                         \
   PNX(nodes) PNX(init_nodes_device)(INT local_M, unsigned malloc_flags);                      \
   void PNX(trafo_grad_x_vjp)(PNX(plan) ths, PNX(nodes) nodes, const C *w, R *grad_x);         \
   int PNX(node_bins)(PNX(plan) ths, PNX(nodes) nodes, int *order, unsigned *keys);            \
   int PNX(kernel_times)(PNX(plan) ths, double *ms, long *launches, int n);                     \
+  C *PNX(oversampled_grid)(PNX(plan) ths);                                                    \
   const char *PNX(last_error)(void);
 
 /* kernel classes of PNX(kernel_times) (extension) */

@@ -32,6 +32,8 @@ PNFFT_COMPUTE_ACCUMULATED = 1 << 4
 W_KB, W_GAUSS, W_BSPLINE, W_SINCPOW, W_BESSEL_I0 = range(5)
 
 _lib = None
+_lib_fast = None
+_FAST_PATH = os.path.join(_HERE, "_fast", "liboracle_fast.so")
 
 
 def build():
@@ -40,46 +42,61 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def lib_fast():
+    """The same C restatement built on this host with the reference's release flags
+    (-O3 -march=native -ffast-math, scripts/install_pnfft_gcc.sh:42): CPU-baseline timing only."""
+    global _lib_fast
+    if _lib_fast is None:
+        import subprocess
+        subprocess.run(["make", "-s", "-C", _HERE, "fast"], check=True)
+        _lib_fast = _load(_FAST_PATH)
+    return _lib_fast
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        dp = ctypes.POINTER(ctypes.c_double)
-        lp = ctypes.POINTER(ctypes.c_long)
-        L.orc_window_id.argtypes = [ctypes.c_uint]
-        L.orc_window_id.restype = ctypes.c_int
-        L.orc_window_param_b.argtypes = [lp, lp, ctypes.c_int, ctypes.c_uint, dp]
-        L.orc_bessel_i0.argtypes = [ctypes.c_double]
-        L.orc_bessel_i0.restype = ctypes.c_double
-        L.orc_bessel_i1.argtypes = [ctypes.c_double]
-        L.orc_bessel_i1.restype = ctypes.c_double
-        L.orc_sinc.argtypes = [ctypes.c_double]
-        L.orc_sinc.restype = ctypes.c_double
-        L.orc_bspline.argtypes = [ctypes.c_int, ctypes.c_double]
-        L.orc_bspline.restype = ctypes.c_double
-        L.orc_inv_phi_hat.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.c_double, ctypes.c_int]
-        L.orc_inv_phi_hat.restype = ctypes.c_double
-        L.orc_psi_axis.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int,
-                                   ctypes.c_double, dp, dp]
-        L.orc_psi.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int, ctypes.c_double]
-        L.orc_psi.restype = ctypes.c_double
-        L.orc_dpsi.argtypes = L.orc_psi.argtypes
-        L.orc_dpsi.restype = ctypes.c_double
-        L.orc_ndft_trafo_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
-        L.orc_ndft_adj_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
-        L.orc_ndft_trafo_c2r.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
-        L.orc_inv_phi_hat_tables.argtypes = [lp, lp, ctypes.c_int, ctypes.c_uint, dp]
-        L.orc_D.argtypes = [lp, dp, dp, dp, ctypes.c_int]
-        L.orc_B_trafo.argtypes = [lp, lp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_long, dp, dp,
-                                  ctypes.c_int, dp, dp, ctypes.c_uint]
-        L.orc_B_adj.argtypes = [lp, lp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_long, dp, dp, dp,
-                                ctypes.c_int, dp, ctypes.c_uint]
-        L.orc_B_adj_blocks.argtypes = L.orc_B_adj.argtypes
-        L.orc_sort_nodes.argtypes = [lp, ctypes.c_int, ctypes.c_long, dp, lp]
-        _lib = L
+        _lib = _load(_LIB_PATH)
     return _lib
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lp = ctypes.POINTER(ctypes.c_long)
+    L.orc_window_id.argtypes = [ctypes.c_uint]
+    L.orc_window_id.restype = ctypes.c_int
+    L.orc_window_param_b.argtypes = [lp, lp, ctypes.c_int, ctypes.c_uint, dp]
+    L.orc_bessel_i0.argtypes = [ctypes.c_double]
+    L.orc_bessel_i0.restype = ctypes.c_double
+    L.orc_bessel_i1.argtypes = [ctypes.c_double]
+    L.orc_bessel_i1.restype = ctypes.c_double
+    L.orc_sinc.argtypes = [ctypes.c_double]
+    L.orc_sinc.restype = ctypes.c_double
+    L.orc_bspline.argtypes = [ctypes.c_int, ctypes.c_double]
+    L.orc_bspline.restype = ctypes.c_double
+    L.orc_inv_phi_hat.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.c_double, ctypes.c_int]
+    L.orc_inv_phi_hat.restype = ctypes.c_double
+    L.orc_psi_axis.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int,
+                               ctypes.c_double, dp, dp]
+    L.orc_psi.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.orc_psi.restype = ctypes.c_double
+    L.orc_dpsi.argtypes = L.orc_psi.argtypes
+    L.orc_dpsi.restype = ctypes.c_double
+    L.orc_ndft_trafo_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
+    L.orc_ndft_adj_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
+    L.orc_ndft_trafo_c2r.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
+    L.orc_inv_phi_hat_tables.argtypes = [lp, lp, ctypes.c_int, ctypes.c_uint, dp]
+    L.orc_D.argtypes = [lp, dp, dp, dp, ctypes.c_int]
+    L.orc_B_trafo.argtypes = [lp, lp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_long, dp, dp,
+                              ctypes.c_int, dp, dp, ctypes.c_uint]
+    L.orc_B_adj.argtypes = [lp, lp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_long, dp, dp, dp,
+                            ctypes.c_int, dp, ctypes.c_uint]
+    L.orc_B_adj_blocks.argtypes = L.orc_B_adj.argtypes
+    L.orc_sort_nodes.argtypes = [lp, ctypes.c_int, ctypes.c_long, dp, lp]
+    return L
 
 
 def _dp(a):
@@ -368,7 +385,7 @@ class Plan:
                           _dp(x), _dp(gpad), 1, _dp(f), _dp(grad), fl)
         return f, grad
 
-    def adj_c2r(self, x, f=None, grad=None):
+    def adj_c2r(self, x, f=None, grad=None, parallel=False):
         x = _f64(x).reshape(-1)
         M = x.size // 3
         ng = [v + 2 * self.m for v in self.no]
@@ -381,8 +398,9 @@ class Plan:
         if grad is not None:
             ga = _f64(grad).reshape(-1)
             fl |= PNFFT_COMPUTE_GRAD_F
-        lib().orc_B_adj(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
-                        _dp(x), _dp(fa), _dp(ga), 1, _dp(gpad), fl)
+        scatter = lib().orc_B_adj_blocks if parallel else lib().orc_B_adj
+        scatter(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
+                _dp(x), _dp(fa), _dp(ga), 1, _dp(gpad), fl)
         g2 = self.ghost_reduce(gpad)
         return self._D_half(self._FH_r2c(g2)).reshape(-1)
 
@@ -448,8 +466,9 @@ class Plan:
                           _dp(grad.view(np.float64)) if grad is not None else None, fl)
         return f, grad
 
-    def adj(self, x, f=None, grad=None, fhat_in=None):
-        """pnfft_adj (api/api-basic.c:344-378); adds onto fhat_in if given (ACCUMULATED)."""
+    def adj(self, x, f=None, grad=None, fhat_in=None, parallel=False):
+        """pnfft_adj (api/api-basic.c:344-378); adds onto fhat_in if given (ACCUMULATED).
+        parallel: scatter on all OpenMP threads (orc_B_adj_blocks; same sums, grouped order)."""
         x = _f64(x).reshape(-1)
         M = x.size // 3
         ng = [v + 2 * self.m for v in self.no]
@@ -462,7 +481,8 @@ class Plan:
         if grad is not None:
             ga = _c128(grad).reshape(-1)
             fl |= PNFFT_COMPUTE_GRAD_F
-        lib().orc_B_adj(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
+        scatter = lib().orc_B_adj_blocks if parallel else lib().orc_B_adj
+        scatter(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
                         _dp(x), _dp(fa.view(np.float64)) if fa is not None else None,
                         _dp(ga.view(np.float64)) if ga is not None else None, 0,
                         _dp(gpad.view(np.float64)), fl)

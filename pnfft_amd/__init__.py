@@ -135,6 +135,7 @@ class _Api:
         fn("reset_timer", None, _P)
         fn("node_bins", _I, _P, _P, _P, _P)
         fn("kernel_times", _I, _P, _P, _P, _I)
+        fn("oversampled_grid", _P, _P)
         fn("last_error", ctypes.c_char_p)
 
     # convenience wrappers -------------------------------------------------

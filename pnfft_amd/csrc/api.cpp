@@ -27,20 +27,9 @@
 #include "ndft.hpp"
 #include "nfft_b.hpp"
 #include "window_math.hpp"
+#include "../../include/pnfft.h"
 
-// kernel classes of pnfft_kernel_times (the PNFFT_KT_* values of include/pnfft.h)
-enum : int {
-  PNFFT_KT_FFT_ROWS = 0,
-  PNFFT_KT_FFT_COLS1 = 1,
-  PNFFT_KT_FFT_COLS0 = 2,
-  PNFFT_KT_GATHER_TILED = 3,
-  PNFFT_KT_GATHER_DIRECT = 4,
-  PNFFT_KT_SCATTER_TILED = 5,
-  PNFFT_KT_SCATTER_DIRECT = 6,
-  PNFFT_KT_BINNING = 7,
-  PNFFT_KT_SCATTER_FOLD = 8,
-  PNFFT_KT_COUNT = 9
-};
+// kernel classes of pnfft_kernel_times: the PNFFT_KT_* values of include/pnfft.h
 
 // flag values (api/pnfft.h:302-390)
 namespace fl {
@@ -202,6 +191,10 @@ struct EventTimer {
   }
 };
 
+// the opaque handle types of include/pnfft.h for each precision (defined with the C interface)
+template <class R>
+struct Handles;
+
 template <class R>
 struct PlanT {
   int d = 3;
@@ -332,7 +325,7 @@ static void plan_destroy(PlanT<R>* P) {
   dfree(P->d_tab);
   P->evt.release();
   if (P->stream) (void)hipStreamDestroy(P->stream);
-  delete P;
+  delete static_cast<typename Handles<R>::plan*>(P);
 }
 
 // api/api-guru.c:110-164 + kernel/ndft-parallel.c:869-1069
@@ -355,8 +348,16 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
               pnfft_flags & unsupported);
     return nullptr;
   }
-  if (m < 1 || m > kMaxM) {
-    set_error("window cutoff m=%d outside [1, %d]", m, kMaxM);
+  // the B kernels are compiled for stencils of 2m+1 = 5..17 points
+  if (m < kMinPlanM || m > kMaxPlanM) {
+    set_error("window cutoff m=%d outside [%d, %d] (the cutoffs this build's kernels are compiled for)", m,
+              kMinPlanM, kMaxPlanM);
+    return nullptr;
+  }
+  // FP64 c2r plans with m >= 7 stalled a GPU run of an earlier kernel build and
+  // have not been run since; they are refused until that is understood
+  if (c2r && sizeof(R) == 8 && m >= 7) {
+    set_error("double-precision c2r plans with m=%d >= 7 are not supported by this build", m);
     return nullptr;
   }
   for (int t = 0; t < 3; t++) {
@@ -377,7 +378,7 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
   if (pnfft_flags & fl::PRE_HESSIAN_PSI) pnfft_flags |= fl::PRE_GRAD_PSI;
   if (pnfft_flags & fl::PRE_GRAD_PSI) pnfft_flags |= fl::PRE_PSI;
 
-  auto* P = new PlanT<R>();
+  PlanT<R>* P = new typename Handles<R>::plan();
   P->m = m;
   P->pnfft_flags = pnfft_flags;
   P->pfft_flags = pfft_flags;
@@ -473,8 +474,14 @@ static void nodes_release_device(NodesT<R>* D) {
 }
 
 template <class R>
+static void nodes_destroy(NodesT<R>* D) {
+  nodes_release_device(D);
+  delete static_cast<typename Handles<R>::nodes*>(D);
+}
+
+template <class R>
 static NodesT<R>* nodes_create(long M, unsigned malloc_flags, bool device) {
-  auto* D = new NodesT<R>();
+  NodesT<R>* D = new typename Handles<R>::nodes();
   D->M = M;
   D->device_arrays = device;
   auto alloc = [&](size_t bytes) -> R* {
@@ -1060,14 +1067,34 @@ static void init_x_3d_adv(const R* lo, const R* up, const R* x_max, long M, R* x
 
 }  // namespace pnfft_amd
 
+// ---------------------------------------------------------------------------
+// C interface (include/pnfft.h), both precisions.  The opaque handles of the
+// header are the plan and node objects; the header is included so that every
+// definition below is checked against its declaration.
+// ---------------------------------------------------------------------------
+struct pnfft_plan_s : pnfft_amd::PlanT<double> {};
+struct pnfftf_plan_s : pnfft_amd::PlanT<float> {};
+struct pnfft_nodes_s : pnfft_amd::NodesT<double> {};
+struct pnfftf_nodes_s : pnfft_amd::NodesT<float> {};
+
+namespace pnfft_amd {
+template <>
+struct Handles<double> {
+  using plan = pnfft_plan_s;
+  using nodes = pnfft_nodes_s;
+};
+template <>
+struct Handles<float> {
+  using plan = pnfftf_plan_s;
+  using nodes = pnfftf_nodes_s;
+};
+}  // namespace pnfft_amd
+
 using namespace pnfft_amd;
 
-// ---------------------------------------------------------------------------
-// C interface, both precisions
-// ---------------------------------------------------------------------------
-#define PNFFT_DEFINE_IMPL(PREFIX, R)                                                                       \
-  typedef PlanT<R>* PREFIX##plan_t;                                                                         \
-  typedef NodesT<R>* PREFIX##nodes_t;                                                                       \
+#define PNFFT_DEFINE_IMPL(PREFIX, R, CT)                                                                   \
+  static inline PlanT<R>* PREFIX##P(PREFIX##plan p) { return p; }                                          \
+  static inline NodesT<R>* PREFIX##D(PREFIX##nodes h) { return h; }                                        \
   extern "C" {                                                                                             \
   int PREFIX##create_procmesh_2d(MPI_Comm comm, int np0, int np1, MPI_Comm* cc) {                           \
     (void)np0; (void)np1; if (cc) *cc = comm; return 0;                                                     \
@@ -1101,11 +1128,13 @@ using namespace pnfft_amd;
   }                                                                                                        \
   void PREFIX##local_size_adv(int d, const ptrdiff_t* N, MPI_Comm cc, unsigned pf, ptrdiff_t* lN,          \
                               ptrdiff_t* lNs, R* lo, R* up) {                                               \
+    if (d != 3) { set_error("d != 3 not yet implemented"); return; }                                        \
     ptrdiff_t n[3] = {2 * N[0], 2 * N[1], 2 * N[2]}; R xm[3] = {R(0.5), R(0.5), R(0.5)};                    \
     PREFIX##local_size_guru(d, N, n, xm, 6, cc, pf, lN, lNs, lo, up);                                       \
   }                                                                                                        \
   void PREFIX##local_size_adv_c2r(int d, const ptrdiff_t* N, MPI_Comm cc, unsigned pf, ptrdiff_t* lN,      \
                                   ptrdiff_t* lNs, R* lo, R* up) {                                           \
+    if (d != 3) { set_error("d != 3 not yet implemented"); return; }                                        \
     ptrdiff_t n[3] = {2 * N[0], 2 * N[1], 2 * N[2]}; R xm[3] = {R(0.5), R(0.5), R(0.5)};                    \
     PREFIX##local_size_guru_c2r(d, N, n, xm, 6, cc, pf, lN, lNs, lo, up);                                   \
   }                                                                                                        \
@@ -1115,127 +1144,121 @@ using namespace pnfft_amd;
                                  ptrdiff_t* lNs, R* lo, R* up) {                                            \
     PREFIX##local_size_adv_c2r(3, N, cc, pf, lN, lNs, lo, up);                                              \
   }                                                                                                        \
-  void* PREFIX##init_guru(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,            \
-                          unsigned pnfft_flags, unsigned pfft_flags, MPI_Comm cc) {                         \
-    (void)cc; return plan_create<R>(d, N, n, x_max, m, pnfft_flags, pfft_flags, false);                      \
+  PREFIX##plan PREFIX##init_guru(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,     \
+                                 unsigned pnfft_flags, unsigned pfft_flags, MPI_Comm cc) {                  \
+    (void)cc;                                                                                               \
+    return static_cast<PREFIX##plan>(plan_create<R>(d, N, n, x_max, m, pnfft_flags, pfft_flags, false));    \
   }                                                                                                        \
-  void* PREFIX##init_guru_c2r(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,        \
-                              unsigned pnfft_flags, unsigned pfft_flags, MPI_Comm cc) {                     \
-    (void)cc; return plan_create<R>(d, N, n, x_max, m, pnfft_flags, pfft_flags, true);                       \
+  PREFIX##plan PREFIX##init_guru_c2r(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m, \
+                                     unsigned pnfft_flags, unsigned pfft_flags, MPI_Comm cc) {              \
+    (void)cc;                                                                                               \
+    return static_cast<PREFIX##plan>(plan_create<R>(d, N, n, x_max, m, pnfft_flags, pfft_flags, true));     \
   }                                                                                                        \
-  void* PREFIX##init_adv(int d, const ptrdiff_t* N, unsigned pnfft_flags, unsigned pfft_flags,             \
-                         MPI_Comm cc) {                                                                     \
+  PREFIX##plan PREFIX##init_adv(int d, const ptrdiff_t* N, unsigned pnfft_flags, unsigned pfft_flags,      \
+                                MPI_Comm cc) {                                                              \
     if (d != 3) { set_error("d != 3 not yet implemented"); return nullptr; }                                \
     ptrdiff_t n[3] = {2 * N[0], 2 * N[1], 2 * N[2]}; R xm[3] = {R(0.5), R(0.5), R(0.5)};                    \
     return PREFIX##init_guru(d, N, n, xm, 6, pnfft_flags, pfft_flags, cc);                                  \
   }                                                                                                        \
-  void* PREFIX##init_adv_c2r(int d, const ptrdiff_t* N, unsigned pnfft_flags, unsigned pfft_flags,         \
-                             MPI_Comm cc) {                                                                 \
+  PREFIX##plan PREFIX##init_adv_c2r(int d, const ptrdiff_t* N, unsigned pnfft_flags, unsigned pfft_flags,  \
+                                    MPI_Comm cc) {                                                          \
     if (d != 3) { set_error("d != 3 not yet implemented"); return nullptr; }                                \
     ptrdiff_t n[3] = {2 * N[0], 2 * N[1], 2 * N[2]}; R xm[3] = {R(0.5), R(0.5), R(0.5)};                    \
     return PREFIX##init_guru_c2r(d, N, n, xm, 6, pnfft_flags, pfft_flags, cc);                              \
   }                                                                                                        \
-  void* PREFIX##init_3d(const ptrdiff_t* N, MPI_Comm cc) {                                                  \
+  PREFIX##plan PREFIX##init_3d(const ptrdiff_t* N, MPI_Comm cc) {                                          \
     return PREFIX##init_adv(3, N, fl::MALLOC_F_HAT, 0, cc);                                                 \
   }                                                                                                        \
-  void* PREFIX##init_3d_c2r(const ptrdiff_t* N, MPI_Comm cc) {                                              \
+  PREFIX##plan PREFIX##init_3d_c2r(const ptrdiff_t* N, MPI_Comm cc) {                                      \
     return PREFIX##init_adv_c2r(3, N, fl::MALLOC_F_HAT, 0, cc);                                             \
   }                                                                                                        \
-  void* PREFIX##init_nodes(ptrdiff_t M, unsigned mf) { return nodes_create<R>(M, mf, false); }               \
-  void* PREFIX##init_nodes_device(ptrdiff_t M, unsigned mf) { return nodes_create<R>(M, mf, true); }         \
-  void PREFIX##free_nodes(void* h, unsigned ff) {                                                           \
-    auto* D = static_cast<NodesT<R>*>(h);                                                                   \
-    if (!D) return;                                                                                        \
-    (void)hipDeviceSynchronize();                                                                           \
-    if (ff & fl::MALLOC_GRAD_F) mem_free(D->grad_f);                                                        \
-    if (ff & fl::MALLOC_HESSIAN_F) mem_free(D->hessian_f);                                                  \
-    if (ff & fl::MALLOC_F) mem_free(D->f);                                                                  \
-    if (ff & fl::MALLOC_X) mem_free(D->x);                                                                  \
-    nodes_release_device(D);                                                                                \
-    delete D;                                                                                               \
+  PREFIX##nodes PREFIX##init_nodes(ptrdiff_t M, unsigned mf) {                                              \
+    return static_cast<PREFIX##nodes>(nodes_create<R>(M, mf, false));                                       \
   }                                                                                                        \
-  void PREFIX##precompute_psi(void* p, void* h, unsigned pf) {                                              \
-    precompute_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), pf);                          \
+  PREFIX##nodes PREFIX##init_nodes_device(ptrdiff_t M, unsigned mf) {                                       \
+    return static_cast<PREFIX##nodes>(nodes_create<R>(M, mf, true));                                        \
   }                                                                                                        \
-  void PREFIX##precompute_one_psi(void* p, void* h, unsigned pf) { PREFIX##precompute_psi(p, h, pf); }       \
-  void PREFIX##set_f(void* f, void* h) { if (h) static_cast<NodesT<R>*>(h)->f = (R*)f; }                    \
-  void PREFIX##set_grad_f(void* g, void* h) { if (h) static_cast<NodesT<R>*>(h)->grad_f = (R*)g; }          \
-  void PREFIX##set_hessian_f(void* g, void* h) { if (h) static_cast<NodesT<R>*>(h)->hessian_f = (R*)g; }    \
-  void PREFIX##set_f_real(R* f, void* h) { if (h) static_cast<NodesT<R>*>(h)->f = f; }                      \
-  void PREFIX##set_grad_f_real(R* g, void* h) { if (h) static_cast<NodesT<R>*>(h)->grad_f = g; }            \
-  void PREFIX##set_hessian_f_real(R* g, void* h) { if (h) static_cast<NodesT<R>*>(h)->hessian_f = g; }      \
-  void PREFIX##set_x(R* x, void* h) {                                                                       \
+  void PREFIX##free_nodes(PREFIX##nodes h, unsigned ff) {                                                   \
     if (!h) return;                                                                                        \
-    auto* D = static_cast<NodesT<R>*>(h);                                                                   \
-    D->x = x; D->bins_valid = false; D->pre_valid = false;                                                  \
+    (void)hipDeviceSynchronize();                                                                           \
+    if (ff & fl::MALLOC_GRAD_F) mem_free(h->grad_f);                                                        \
+    if (ff & fl::MALLOC_HESSIAN_F) mem_free(h->hessian_f);                                                  \
+    if (ff & fl::MALLOC_F) mem_free(h->f);                                                                  \
+    if (ff & fl::MALLOC_X) mem_free(h->x);                                                                  \
+    nodes_destroy<R>(h);                                                                                    \
   }                                                                                                        \
-  void PREFIX##set_f_hat(void* fh, void* p) { if (p) static_cast<PlanT<R>*>(p)->f_hat = fh; }               \
-  void PREFIX##set_f_hat_real(R* fh, void* p) { if (p) static_cast<PlanT<R>*>(p)->f_hat = fh; }             \
-  void PREFIX##set_b(R b0, R b1, R b2, void* p) {                                                           \
-    auto* P = static_cast<PlanT<R>*>(p); if (!P) return;                                                    \
-    P->b[0] = b0; P->b[1] = b1; P->b[2] = b2;                                                               \
-    for (int t = 0; t < 3; t++) P->geom.b[t] = P->b[t];                                                     \
-    tables_compute(P);                                                                                      \
+  void PREFIX##precompute_psi(PREFIX##plan p, PREFIX##nodes h, unsigned pf) {                               \
+    precompute_impl<R>(PREFIX##P(p), PREFIX##D(h), pf);                                                     \
   }                                                                                                        \
-  void* PREFIX##get_f(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->f : nullptr; }           \
-  void* PREFIX##get_grad_f(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->grad_f : nullptr; } \
-  void* PREFIX##get_hessian_f(const void* h) {                                                              \
-    return h ? static_cast<const NodesT<R>*>(h)->hessian_f : nullptr;                                       \
+  void PREFIX##precompute_one_psi(PREFIX##plan p, PREFIX##nodes h, unsigned pf) {                           \
+    PREFIX##precompute_psi(p, h, pf);                                                                       \
   }                                                                                                        \
-  R* PREFIX##get_f_real(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->f : nullptr; }         \
-  R* PREFIX##get_grad_f_real(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->grad_f : nullptr; } \
-  R* PREFIX##get_hessian_f_real(const void* h) {                                                            \
-    return h ? static_cast<const NodesT<R>*>(h)->hessian_f : nullptr;                                       \
+  void PREFIX##set_f(CT* f, PREFIX##nodes h) { if (h) h->f = (R*)f; }                                       \
+  void PREFIX##set_grad_f(CT* g, PREFIX##nodes h) { if (h) h->grad_f = (R*)g; }                             \
+  void PREFIX##set_hessian_f(CT* g, PREFIX##nodes h) { if (h) h->hessian_f = (R*)g; }                       \
+  void PREFIX##set_f_real(R* f, PREFIX##nodes h) { if (h) h->f = f; }                                       \
+  void PREFIX##set_grad_f_real(R* g, PREFIX##nodes h) { if (h) h->grad_f = g; }                             \
+  void PREFIX##set_hessian_f_real(R* g, PREFIX##nodes h) { if (h) h->hessian_f = g; }                       \
+  void PREFIX##set_x(R* x, PREFIX##nodes h) {                                                               \
+    if (!h) return;                                                                                        \
+    h->x = x; h->bins_valid = false; h->pre_valid = false;                                                  \
   }                                                                                                        \
-  R* PREFIX##get_x(const void* h) { return h ? static_cast<const NodesT<R>*>(h)->x : nullptr; }              \
-  void* PREFIX##get_f_hat(const void* p) { return p ? static_cast<const PlanT<R>*>(p)->f_hat : nullptr; }    \
-  R* PREFIX##get_f_hat_real(const void* p) {                                                                \
-    return p ? (R*)static_cast<const PlanT<R>*>(p)->f_hat : nullptr;                                        \
+  void PREFIX##set_f_hat(CT* fh, PREFIX##plan p) { if (p) p->f_hat = fh; }                                  \
+  void PREFIX##set_f_hat_real(R* fh, PREFIX##plan p) { if (p) p->f_hat = fh; }                              \
+  void PREFIX##set_b(R b0, R b1, R b2, PREFIX##plan p) {                                                    \
+    if (!p) return;                                                                                        \
+    p->b[0] = b0; p->b[1] = b1; p->b[2] = b2;                                                               \
+    for (int t = 0; t < 3; t++) p->geom.b[t] = p->b[t];                                                     \
+    tables_compute(PREFIX##P(p));                                                                           \
   }                                                                                                        \
-  int PREFIX##get_d(const void* p) { return static_cast<const PlanT<R>*>(p)->d; }                           \
-  int PREFIX##get_m(const void* p) { return static_cast<const PlanT<R>*>(p)->m; }                           \
-  void PREFIX##get_x_max(const void* p, R* xm) {                                                            \
-    for (int t = 0; t < 3; t++) xm[t] = (R) static_cast<const PlanT<R>*>(p)->x_max[t];                      \
+  CT* PREFIX##get_f(const PREFIX##nodes h) { return h ? (CT*)h->f : nullptr; }                           \
+  CT* PREFIX##get_grad_f(const PREFIX##nodes h) { return h ? (CT*)h->grad_f : nullptr; }                 \
+  CT* PREFIX##get_hessian_f(const PREFIX##nodes h) { return h ? (CT*)h->hessian_f : nullptr; }           \
+  R* PREFIX##get_f_real(const PREFIX##nodes h) { return h ? h->f : nullptr; }                            \
+  R* PREFIX##get_grad_f_real(const PREFIX##nodes h) { return h ? h->grad_f : nullptr; }                  \
+  R* PREFIX##get_hessian_f_real(const PREFIX##nodes h) { return h ? h->hessian_f : nullptr; }            \
+  R* PREFIX##get_x(const PREFIX##nodes h) { return h ? h->x : nullptr; }                                 \
+  CT* PREFIX##get_f_hat(const PREFIX##plan p) { return p ? (CT*)p->f_hat : nullptr; }                    \
+  R* PREFIX##get_f_hat_real(const PREFIX##plan p) { return p ? (R*)p->f_hat : nullptr; }                 \
+  int PREFIX##get_d(const PREFIX##plan p) { return p->d; }                                               \
+  int PREFIX##get_m(const PREFIX##plan p) { return p->m; }                                               \
+  void PREFIX##get_x_max(const PREFIX##plan p, R* xm) {                                                  \
+    for (int t = 0; t < 3; t++) xm[t] = (R)p->x_max[t];                                                     \
   }                                                                                                        \
-  void PREFIX##get_N(const void* p, ptrdiff_t* N) {                                                         \
-    for (int t = 0; t < 3; t++) N[t] = static_cast<const PlanT<R>*>(p)->N[t];                               \
+  void PREFIX##get_N(const PREFIX##plan p, ptrdiff_t* N) {                                               \
+    for (int t = 0; t < 3; t++) N[t] = p->N[t];                                                             \
   }                                                                                                        \
-  void PREFIX##get_n(const void* p, ptrdiff_t* n) {                                                         \
-    for (int t = 0; t < 3; t++) n[t] = static_cast<const PlanT<R>*>(p)->n[t];                               \
+  void PREFIX##get_n(const PREFIX##plan p, ptrdiff_t* n) {                                               \
+    for (int t = 0; t < 3; t++) n[t] = p->n[t];                                                             \
   }                                                                                                        \
-  unsigned PREFIX##get_pnfft_flags(const void* p) { return static_cast<const PlanT<R>*>(p)->pnfft_flags; }  \
-  unsigned PREFIX##get_pfft_flags(const void* p) { return static_cast<const PlanT<R>*>(p)->pfft_flags; }    \
-  void PREFIX##get_b(const void* p, R* b0, R* b1, R* b2) {                                                  \
-    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
-    *b0 = (R)P->b[0]; *b1 = (R)P->b[1]; *b2 = (R)P->b[2];                                                   \
+  unsigned PREFIX##get_pnfft_flags(const PREFIX##plan p) { return p->pnfft_flags; }                      \
+  unsigned PREFIX##get_pfft_flags(const PREFIX##plan p) { return p->pfft_flags; }                        \
+  void PREFIX##get_b(const PREFIX##plan p, R* b0, R* b1, R* b2) {                                        \
+    *b0 = (R)p->b[0]; *b1 = (R)p->b[1]; *b2 = (R)p->b[2];                                                   \
   }                                                                                                        \
-  void PREFIX##finalize(void* p, unsigned ff) {                                                             \
-    auto* P = static_cast<PlanT<R>*>(p);                                                                    \
-    if (!P) return;                                                                                        \
-    if ((ff & fl::MALLOC_F_HAT) && P->f_hat) mem_free(P->f_hat);                                            \
-    plan_destroy(P);                                                                                        \
+  void PREFIX##finalize(PREFIX##plan p, unsigned ff) {                                                      \
+    if (!p) return;                                                                                        \
+    if ((ff & fl::MALLOC_F_HAT) && p->f_hat) mem_free(p->f_hat);                                            \
+    plan_destroy<R>(p);                                                                                     \
   }                                                                                                        \
-  void PREFIX##trafo(void* p, void* h, unsigned cf) {                                                       \
-    trafo_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), cf);                               \
+  void PREFIX##trafo(PREFIX##plan p, PREFIX##nodes h, unsigned cf) { trafo_impl<R>(p, h, cf); }             \
+  void PREFIX##adj(PREFIX##plan p, PREFIX##nodes h, unsigned cf) { adj_impl<R>(p, h, cf); }                 \
+  void PREFIX##trafo_grad_x_vjp(PREFIX##plan p, PREFIX##nodes h, const CT* w, R* gx) {                      \
+    vjp_impl<R>(p, h, (const Cplx<R>*)w, gx);                                                               \
   }                                                                                                        \
-  void PREFIX##adj(void* p, void* h, unsigned cf) {                                                         \
-    adj_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), cf);                                 \
+  int PREFIX##node_bins(PREFIX##plan p, PREFIX##nodes h, int* order, unsigned* keys) {                      \
+    return node_bins_impl<R>(p, h, order, keys);                                                            \
   }                                                                                                        \
-  void PREFIX##trafo_grad_x_vjp(void* p, void* h, const void* w, R* gx) {                                   \
-    vjp_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), (const Cplx<R>*)w, gx);              \
-  }                                                                                                        \
-  int PREFIX##node_bins(void* p, void* h, int* order, unsigned* keys) {                                      \
-    return node_bins_impl<R>(static_cast<PlanT<R>*>(p), static_cast<NodesT<R>*>(h), order, keys);           \
-  }                                                                                                        \
+  CT* PREFIX##oversampled_grid(PREFIX##plan p) { return p ? (CT*)p->d_grid : nullptr; }                     \
   void PREFIX##init(void) {}                                                                                \
   void PREFIX##cleanup(void) {}                                                                             \
   void* PREFIX##malloc(size_t n) { return host_alloc(n); }                                                  \
   R* PREFIX##alloc_real(size_t n) { return (R*)host_alloc(sizeof(R) * n); }                                  \
-  void* PREFIX##alloc_complex(size_t n) { return host_alloc(2 * sizeof(R) * n); }                            \
+  CT* PREFIX##alloc_complex(size_t n) { return (CT*)host_alloc(2 * sizeof(R) * n); }                         \
   void PREFIX##free(void* ptr) { mem_free(ptr); }                                                           \
   void* PREFIX##malloc_device(size_t n) { return device_alloc(n); }                                         \
   void PREFIX##init_f_hat_3d(const ptrdiff_t* N, const ptrdiff_t* lN, const ptrdiff_t* lNs, unsigned pf,    \
-                             void* data) {                                                                  \
+                             CT* data) {                                                                    \
     /* deterministic fill; the reference delegates to PFFT's init_input_complex_3d */                      \
     (void)pf; R* d = (R*)data; long k = 0;                                                                  \
     for (long k0 = lNs[0]; k0 < lNs[0] + lN[0]; k0++)                                                       \
@@ -1245,7 +1268,7 @@ using namespace pnfft_amd;
           d[2 * k] = (R)(1000.0 / (2.0 * g + 1.0)); d[2 * k + 1] = (R)(1000.0 / (2.0 * g + 2.0));           \
         }                                                                                                  \
   }                                                                                                        \
-  void PREFIX##init_f(ptrdiff_t M, void* data) {                                                            \
+  void PREFIX##init_f(ptrdiff_t M, CT* data) {                                                              \
     R* d = (R*)data;                                                                                        \
     for (long j = 0; j < M; j++) {                                                                          \
       R re = (R)(100.0 * (R)rand() / RAND_MAX); R im = (R)(100.0 * (R)rand() / RAND_MAX);                   \
@@ -1258,39 +1281,34 @@ using namespace pnfft_amd;
   void PREFIX##init_x_3d(const R* lo, const R* up, ptrdiff_t M, R* x) {                                     \
     R xm[3] = {R(0.5), R(0.5), R(0.5)}; init_x_3d_adv<R>(lo, up, xm, M, x);                                  \
   }                                                                                                        \
-  void PREFIX##zero_f_hat(void* p) {                                                                        \
-    auto* P = static_cast<PlanT<R>*>(p); if (!P || !P->f_hat) return;                                       \
-    if (is_device_ptr(P->f_hat)) {                                                                          \
-      fill_zero<R>((Cplx<R>*)P->f_hat, P->fhat_elems, P->stream); (void)hipStreamSynchronize(P->stream);     \
-    } else memset(P->f_hat, 0, sizeof(Cplx<R>) * P->fhat_elems);                                            \
+  void PREFIX##zero_f_hat(PREFIX##plan p) {                                                                 \
+    if (!p || !p->f_hat) return;                                                                           \
+    if (is_device_ptr(p->f_hat)) {                                                                          \
+      fill_zero<R>((Cplx<R>*)p->f_hat, p->fhat_elems, p->stream); (void)hipStreamSynchronize(p->stream);     \
+    } else memset(p->f_hat, 0, sizeof(Cplx<R>) * p->fhat_elems);                                            \
   }                                                                                                        \
-  R PREFIX##inv_phi_hat(const void* p, int dim, ptrdiff_t k) {                                               \
-    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
-    return (R)host::inv_phi_hat(P->win, k, P->n[dim], P->b[dim], P->m);                                     \
+  R PREFIX##inv_phi_hat(const PREFIX##plan p, int dim, ptrdiff_t k) {                                    \
+    return (R)host::inv_phi_hat(p->win, k, p->n[dim], p->b[dim], p->m);                                     \
   }                                                                                                        \
-  R PREFIX##phi_hat(const void* p, int dim, ptrdiff_t k) {                                                   \
-    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
-    return (R)host::phi_hat(P->win, k, P->n[dim], P->b[dim], P->m);                                         \
+  R PREFIX##phi_hat(const PREFIX##plan p, int dim, ptrdiff_t k) {                                        \
+    return (R)host::phi_hat(p->win, k, p->n[dim], p->b[dim], p->m);                                         \
   }                                                                                                        \
-  R PREFIX##psi(const void* p, int dim, R x) {                                                              \
-    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
-    return (R)host::psi(P->win, P->n[dim], P->b[dim], P->m, (double)x);                                     \
+  R PREFIX##psi(const PREFIX##plan p, int dim, R x) {                                                    \
+    return (R)host::psi(p->win, p->n[dim], p->b[dim], p->m, (double)x);                                     \
   }                                                                                                        \
-  R PREFIX##dpsi(const void* p, int dim, R x) {                                                             \
-    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
-    return (R)host::dpsi(P->win, P->n[dim], P->b[dim], P->m, (double)x);                                    \
+  R PREFIX##dpsi(const PREFIX##plan p, int dim, R x) {                                                   \
+    return (R)host::dpsi(p->win, p->n[dim], p->b[dim], p->m, (double)x);                                    \
   }                                                                                                        \
-  R PREFIX##ddpsi(const void* p, int dim, R x) {                                                            \
-    auto* P = static_cast<const PlanT<R>*>(p);                                                              \
-    return (R)host::ddpsi(P->win, P->n[dim], P->b[dim], P->m, (double)x);                                   \
+  R PREFIX##ddpsi(const PREFIX##plan p, int dim, R x) {                                                  \
+    return (R)host::ddpsi(p->win, p->n[dim], p->b[dim], p->m, (double)x);                                   \
   }                                                                                                        \
-  double* PREFIX##get_timer_trafo(void* p) {                                                                 \
+  double* PREFIX##get_timer_trafo(PREFIX##plan p) {                                                         \
     double* t = (double*)malloc(sizeof(double) * T_LEN);                                                    \
-    memcpy(t, static_cast<PlanT<R>*>(p)->timer_trafo, sizeof(double) * T_LEN); return t;                    \
+    memcpy(t, p->timer_trafo, sizeof(double) * T_LEN); return t;                                            \
   }                                                                                                        \
-  double* PREFIX##get_timer_adj(void* p) {                                                                   \
+  double* PREFIX##get_timer_adj(PREFIX##plan p) {                                                           \
     double* t = (double*)malloc(sizeof(double) * T_LEN);                                                    \
-    memcpy(t, static_cast<PlanT<R>*>(p)->timer_adj, sizeof(double) * T_LEN); return t;                      \
+    memcpy(t, p->timer_adj, sizeof(double) * T_LEN); return t;                                              \
   }                                                                                                        \
   void PREFIX##timer_average(double* t) {                                                                    \
     if (t[T_ITER] > 0) for (int i = 1; i < T_LEN; i++) t[i] /= t[T_ITER];                                   \
@@ -1304,17 +1322,15 @@ using namespace pnfft_amd;
     for (int i = 0; i < T_LEN; i++) t[i] = a[i] + b[i]; return t;                                           \
   }                                                                                                        \
   void PREFIX##timer_free(double* t) { free(t); }                                                           \
-  void PREFIX##reset_timer(void* p) {                                                                       \
-    auto* P = static_cast<PlanT<R>*>(p);                                                                    \
-    for (int i = 0; i < T_LEN; i++) P->timer_trafo[i] = P->timer_adj[i] = 0;                                 \
-    for (int i = 0; i < PNFFT_KT_COUNT; i++) { P->ktime[i] = 0; P->kcount[i] = 0; }                          \
+  void PREFIX##reset_timer(PREFIX##plan p) {                                                                \
+    for (int i = 0; i < T_LEN; i++) p->timer_trafo[i] = p->timer_adj[i] = 0;                                 \
+    for (int i = 0; i < PNFFT_KT_COUNT; i++) { p->ktime[i] = 0; p->kcount[i] = 0; }                          \
   }                                                                                                        \
-  int PREFIX##kernel_times(void* p, double* ms, long* launches, int n) {                                     \
-    auto* P = static_cast<PlanT<R>*>(p);                                                                    \
-    if (!P) return PNFFT_KT_COUNT;                                                                          \
+  int PREFIX##kernel_times(PREFIX##plan p, double* ms, long* launches, int n) {                              \
+    if (!p) return PNFFT_KT_COUNT;                                                                          \
     for (int i = 0; i < n && i < PNFFT_KT_COUNT; i++) {                                                     \
-      if (ms) ms[i] = P->ktime[i];                                                                          \
-      if (launches) launches[i] = P->kcount[i];                                                             \
+      if (ms) ms[i] = p->ktime[i];                                                                          \
+      if (launches) launches[i] = p->kcount[i];                                                             \
     }                                                                                                      \
     return PNFFT_KT_COUNT;                                                                                  \
   }                                                                                                        \
@@ -1325,14 +1341,15 @@ using namespace pnfft_amd;
            name, t[T_ITER], t[T_WHOLE] / it, t[T_D] / it, t[T_F] / it, t[T_B] / it, t[T_LOOP_B] / it,       \
            t[T_SORT] / it);                                                                                 \
   }                                                                                                        \
-  void PREFIX##print_average_timer(const void* p, MPI_Comm c) {                                              \
-    (void)c; auto* P = static_cast<const PlanT<R>*>(p);                                                     \
-    PREFIX##print_timer(P->timer_trafo, "PNFFT trafo"); PREFIX##print_timer(P->timer_adj, "PNFFT adj");      \
+  void PREFIX##print_average_timer(const PREFIX##plan p, MPI_Comm c) {                                    \
+    (void)c;                                                                                                \
+    PREFIX##print_timer(p->timer_trafo, "PNFFT trafo"); PREFIX##print_timer(p->timer_adj, "PNFFT adj");      \
   }                                                                                                        \
-  void PREFIX##print_average_timer_adv(const void* p, MPI_Comm c) { PREFIX##print_average_timer(p, c); }     \
+  void PREFIX##print_average_timer_adv(const PREFIX##plan p, MPI_Comm c) {                                \
+    PREFIX##print_average_timer(p, c);                                                                      \
+  }                                                                                                        \
   const char* PREFIX##last_error(void) { return g_last_error.empty() ? nullptr : g_last_error.c_str(); }     \
   }
 
-typedef int MPI_Comm;
-PNFFT_DEFINE_IMPL(pnfft_, double)
-PNFFT_DEFINE_IMPL(pnfftf_, float)
+PNFFT_DEFINE_IMPL(pnfft_, double, pnfft_complex)
+PNFFT_DEFINE_IMPL(pnfftf_, float, pnfftf_complex)

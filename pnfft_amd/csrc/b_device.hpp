@@ -53,8 +53,8 @@ __device__ inline void scale_windows(const GridGeom& g, int c, NodeWin<R, CM>& w
 // index is static, so NodeWin lives in registers; the values come from the
 // precomputed tables (pnfft_precompute_psi) when they exist (with the
 // derivatives, if WG) and the node's coordinates still equal the ones they were
-// computed from.  MT == 0 (generic
-// m): runtime width m, arrays of kMaxC entries, always evaluated.
+// computed from.  MT == 0: the runtime-width loops of the FP64 m = 7, 8
+// kernels (width m, arrays of CM entries), always evaluated.
 template <class R, int MT, int CM, bool WG>
 __device__ inline void node_windows(const BArgs<R>& A, long p, long j, int m, NodeWin<R, CM>& w) {
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];

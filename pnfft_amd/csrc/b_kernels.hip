@@ -44,8 +44,8 @@ __global__ __launch_bounds__(kB) void gather_direct_k(BArgs<R> A) {
   if (p >= A.M) return;
   if (skip_dense(A, p)) return;
   const long j = A.perm ? (long)A.perm[p] : p;
-  const int m = MT > 0 ? MT : A.g.m;
-  constexpr int CM = MT > 0 ? 2 * MT + 1 : kMaxC;
+  constexpr int m = MT;
+  constexpr int CM = 2 * MT + 1;
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
   NodeWin<R, CM> w;
   constexpr bool ST = static_body<R, MT>();
@@ -59,8 +59,8 @@ __global__ __launch_bounds__(kB) void scatter_direct_k(BArgs<R> A) {
   if (p >= A.M) return;
   if (skip_dense(A, p)) return;
   const long j = A.perm ? (long)A.perm[p] : p;
-  const int m = MT > 0 ? MT : A.g.m;
-  constexpr int CM = MT > 0 ? 2 * MT + 1 : kMaxC;
+  constexpr int m = MT;
+  constexpr int CM = 2 * MT + 1;
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
   NodeWin<R, CM> w;
   constexpr bool ST = static_body<R, MT>();
@@ -728,14 +728,8 @@ int b_launch(const BArgs<R>& A, hipStream_t s) {
     PNFFT_B_CASE(7)
     PNFFT_B_CASE(8)
 #undef PNFFT_B_CASE
-    default: {
-      D.tile_dense = nullptr;
-      if (A.mid) (void)hipEventRecord(A.mid, s);
-      if (A.mid2) (void)hipEventRecord(A.mid2, s);
-      if constexpr (KIND == 0) hipLaunchKernelGGL((gather_direct_k<R, 0, MODE>), grid, block, 0, s, D);
-      else hipLaunchKernelGGL((scatter_direct_k<R, 0, SMODE>), grid, block, 0, s, D);
-      break;
-    }
+    default:  // plans are limited to m = 2..8
+      return kUnsupported;
   }
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }

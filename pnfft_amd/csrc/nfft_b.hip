@@ -392,7 +392,7 @@ int build_tile_order(const int* tile_start, long ntiles, unsigned* work, unsigne
 template <class R>
 int gather(const BArgs<R>& A, int mode, hipStream_t s) {
   if (A.M == 0) return kOk;
-  if (A.g.m < 1 || A.g.m > kMaxM) return kUnsupported;
+  if (A.g.m < kMinPlanM || A.g.m > kMaxPlanM) return kUnsupported;
   switch (mode) {
     case B_F: return b_launch<R, 0, B_F>(A, s);
     case B_GRAD: return b_launch<R, 0, B_GRAD>(A, s);
@@ -405,7 +405,7 @@ int gather(const BArgs<R>& A, int mode, hipStream_t s) {
 template <class R>
 int scatter(const BArgs<R>& A, int mode, hipStream_t s) {
   if (A.M == 0) return kOk;
-  if (A.g.m < 1 || A.g.m > kMaxM) return kUnsupported;
+  if (A.g.m < kMinPlanM || A.g.m > kMaxPlanM) return kUnsupported;
   if (A.slots) {  // column path: writes every grid cell through the fold
     int st = kUnsupported;
     switch (mode) {

@@ -17,8 +17,10 @@ namespace pnfft_amd {
 
 enum WindowId : int { WIN_KB = 0, WIN_GAUSS = 1, WIN_BSPLINE = 2, WIN_SINCPOW = 3, WIN_BESSEL_I0 = 4 };
 
-constexpr int kMaxM = 16;                 // largest supported cutoff parameter m
-constexpr int kMaxC = 2 * kMaxM + 1;      // largest stencil width
+constexpr int kMaxM = 16;                 // largest cutoff the window formulas handle (host)
+constexpr int kMaxC = 2 * kMaxM + 1;      // largest stencil width (host)
+constexpr int kMinPlanM = 2;              // cutoffs the B kernels are compiled for
+constexpr int kMaxPlanM = 8;
 
 template <class R> struct Consts;
 template <> struct Consts<double> {

@@ -147,3 +147,18 @@ def test_kernel_time_classes_match_library(tmp_path):
     for prec in ("double", "float"):
         assert PA.lib().api(prec).kernel_times(None, None, None, 0) == count
     assert len(DevicePlan.KT_NAMES) == count
+
+
+@pytest.mark.parametrize("prec,m,c2r", [("double", 1, False), ("double", 9, False), ("float", 12, False),
+                                        ("double", 7, True), ("double", 8, True)])
+def test_unsupported_plans_are_refused(prec, m, c2r):
+    """Cutoffs outside the compiled 2..8 and double-precision c2r plans with m >= 7 give a NULL
+    plan and a message (validated before any device call)."""
+    import ctypes
+    api = pnfft_amd.lib().api(prec)
+    R = api.real
+    init = api.init_guru_c2r if c2r else api.init_guru
+    p = init(3, api.arr3((16, 16, 16)), api.arr3((32, 32, 32)), (R * 3)(0.5, 0.5, 0.5), m, 0, 0,
+             pnfft_amd.MPI_COMM_WORLD)
+    assert not p
+    assert api.last_error() is not None

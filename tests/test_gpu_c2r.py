@@ -13,10 +13,11 @@ from pnfft_amd.workloads import SplitMix64, make
 
 pytestmark = pytest.mark.gpu
 
-# FP64 plans with m >= 7 stalled a GPU run after a kernel change (now reverted
-# for FP64 m >= 7); the cause was not identified, so these cases stay off the
-# GPU until it is.
-FP64_M8_STALL = "FP64 m=8 stalled a GPU run after a kernel change; cause not identified yet"
+# An FP64 Gaussian m = 8 c2r run stalled the GPU under an earlier kernel build
+# (static-width FP64 m = 7, 8 kernels, since reverted).  No cause was shown, so
+# the library refuses FP64 c2r plans with m >= 7 and these cases stay skipped.
+FP64_M8_STALL = ("FP64 c2r m=8 stalled a GPU run of an earlier kernel build with no cause shown; "
+                 "such plans are refused (pnfft_init_guru_c2r returns NULL)")
 
 
 def rel(a, b):
