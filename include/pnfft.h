@@ -183,12 +183,21 @@ extern "C" {
    * The plan's D factors and window values carry reciprocal powers of two per               \
    * axis (2^e and 2^-e, e = exponent of psi(0)), so the grid is 2^e times the               \
    * reference's g2 when D was applied (exact scaling; B F D is unchanged).                   \
-   * last_error: message of the last rejected call, or NULL. */                               \
+   * last_error: message of the last rejected call, or NULL.                                  \
+   * node_bins_noalloc: no counterpart in the reference; for callers that cannot               \
+   * allocate device memory (code under HIP graph capture, tight serving loops).              \
+   * Like node_bins for M nodes whose coordinates x (R[3M]) and outputs order                 \
+   * (int[M]) and keys (unsigned[M]) are device arrays of the caller; allocates               \
+   * nothing: the sort's scratch is a static device array of the library                      \
+   * (about 17 MB per GPU, held until the process ends) for at most 2^20 nodes;              \
+   * larger M is rejected (-1, last_error set).  Calls that overlap on one GPU                \
+   * share that array and race.  Returns 0 on success. */                                     \
   void *PNX(malloc_device)(size_t n);                                                         \
   PNX(nodes) PNX(init_nodes_device)(INT local_M, unsigned malloc_flags);                      \
   void PNX(trafo_grad_x_vjp)(PNX(plan) ths, PNX(nodes) nodes, const C *w, R *grad_x);         \
   int PNX(node_bins)(PNX(plan) ths, PNX(nodes) nodes, int *order, unsigned *keys);            \
   int PNX(kernel_times)(PNX(plan) ths, double *ms, long *launches, int n);                     \
+  int PNX(node_bins_noalloc)(PNX(plan) ths, const R *x, INT M, int *order, unsigned *keys);   \
   C *PNX(oversampled_grid)(PNX(plan) ths);                                                    \
   const char *PNX(last_error)(void);
 

@@ -134,6 +134,7 @@ class _Api:
         fn("timer_free", None, _DBLP)
         fn("reset_timer", None, _P)
         fn("node_bins", _I, _P, _P, _P, _P)
+        fn("node_bins_noalloc", _I, _P, _P, _PD, _P, _P)
         fn("kernel_times", _I, _P, _P, _P, _I)
         fn("oversampled_grid", _P, _P)
         fn("last_error", ctypes.c_char_p)

@@ -1052,6 +1052,33 @@ static int node_bins_impl(PlanT<R>* P, NodesT<R>* D, int* order, unsigned* keys)
   return 0;
 }
 
+// node_bins without allocation: x, order and keys are device arrays of the
+// caller; the sort runs in the library's static scratch (bin_nodes_noalloc).
+template <class R>
+static int node_bins_noalloc_impl(PlanT<R>* P, const R* x, long M, int* order, unsigned* keys) {
+  if (!P) return -1;
+  if (M < 0 || M > kNoAllocMaxNodes) {
+    set_error("node_bins_noalloc: M=%ld outside [0, %ld] (the static scratch holds %ld nodes)", M, kNoAllocMaxNodes,
+              kNoAllocMaxNodes);
+    return -1;
+  }
+  if (M == 0) return 0;
+  if (!x || !order || !keys || !is_device_ptr(x) || !is_device_ptr(order) || !is_device_ptr(keys)) {
+    set_error("node_bins_noalloc: x, order and keys must be device arrays");
+    return -1;
+  }
+  const int st = bin_nodes_noalloc<R>(x, M, P->geom, P->tiles, order, keys, P->stream);
+  if (st != kOk) {
+    set_error("node_bins_noalloc: binning failed (%d)", st);
+    return -1;
+  }
+  if (hipStreamSynchronize(P->stream) != hipSuccess) {
+    set_error("node_bins_noalloc: HIP error %s", hipGetErrorString(hipGetLastError()));
+    return -1;
+  }
+  return 0;
+}
+
 // glibc-rand based test data (api/api-basic.c:681-704, api/api-adv.c:35-86)
 template <class R>
 static void init_x_3d_adv(const R* lo, const R* up, const R* x_max, long M, R* x) {
@@ -1248,6 +1275,9 @@ using namespace pnfft_amd;
   }                                                                                                        \
   int PREFIX##node_bins(PREFIX##plan p, PREFIX##nodes h, int* order, unsigned* keys) {                      \
     return node_bins_impl<R>(p, h, order, keys);                                                            \
+  }                                                                                                        \
+  int PREFIX##node_bins_noalloc(PREFIX##plan p, const R* x, ptrdiff_t M, int* order, unsigned* keys) {       \
+    return node_bins_noalloc_impl<R>(p, x, (long)M, order, keys);                                           \
   }                                                                                                        \
   CT* PREFIX##oversampled_grid(PREFIX##plan p) { return p ? (CT*)p->d_grid : nullptr; }                     \
   void PREFIX##init(void) {}                                                                                \

@@ -13,6 +13,7 @@
 // both precisions; the window is evaluated in the compute precision from the
 // FP64 fractional part.
 #include <cstdio>
+#include <mutex>
 
 #include "device_common.hpp"
 #include "b_common.hpp"
@@ -312,13 +313,12 @@ static int scan_exclusive(unsigned* d, long n, unsigned* partial, hipStream_t s)
   return kOk;
 }
 
+// Bin keys of the M nodes sorted stably (LSD radix, 8-bit digits) with the
+// scratch of W: perm_out = node ids in key order, keys_out = the sorted keys.
 template <class R>
-int bin_nodes(const R* x, long M, const GridGeom& g, const TileLayout& T, SortWork& W, int* perm_out,
-              unsigned* keys_out, int* tile_start, hipStream_t s) {
-  const long ntiles = T.ntiles();
-  hipLaunchKernelGGL(fill_int_k, dim3(grid1d(ntiles + 1, 256)), dim3(256), 0, s, tile_start, ntiles + 1, (int)M);
-  if (M == 0) return hipGetLastError() == hipSuccess ? kOk : kHipError;
-  const unsigned long long key_limit = (unsigned long long)ntiles * (unsigned long long)T.tile_vol();
+static int sort_bins(const R* x, long M, const GridGeom& g, const TileLayout& T, SortWork& W, int* perm_out,
+                     unsigned* keys_out, hipStream_t s) {
+  const unsigned long long key_limit = (unsigned long long)T.ntiles() * (unsigned long long)T.tile_vol();
   if (key_limit > (1ull << 32)) return kUnsupported;
   const int passes = sort_passes(key_limit);
   const int nblk = (int)((M + kSortIPB - 1) / kSortIPB);
@@ -336,9 +336,80 @@ int bin_nodes(const R* x, long M, const GridGeom& g, const TileLayout& T, SortWo
     return kHipError;
   if (hipMemcpyAsync(keys_out, W.keys[cur], sizeof(unsigned) * M, hipMemcpyDeviceToDevice, s) != hipSuccess)
     return kHipError;
+  return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
+template <class R>
+int bin_nodes(const R* x, long M, const GridGeom& g, const TileLayout& T, SortWork& W, int* perm_out,
+              unsigned* keys_out, int* tile_start, hipStream_t s) {
+  const long ntiles = T.ntiles();
+  hipLaunchKernelGGL(fill_int_k, dim3(grid1d(ntiles + 1, 256)), dim3(256), 0, s, tile_start, ntiles + 1, (int)M);
+  if (M == 0) return hipGetLastError() == hipSuccess ? kOk : kHipError;
+  const int st = sort_bins<R>(x, M, g, T, W, perm_out, keys_out, s);
+  if (st != kOk) return st;
   hipLaunchKernelGGL(tile_bounds_k, dim3(grid1d(M, kB)), dim3(kB), 0, s, keys_out, M, (unsigned)T.tile_vol(),
                      tile_start);
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
+// ---------------------------------------------------------------------------
+// binning without allocation: the sort's scratch (two key and two id buffers,
+// the digit histograms and their scan partials) lives in one file-scope device
+// array per GPU, of kNoAllocMaxNodes nodes at most.  Its address is taken once
+// per device with hipGetSymbolAddress (the host-side name is not a device
+// address); every call on a GPU shares that GPU's array, so calls that overlap
+// on one GPU race.  Every scratch word a call reads was written earlier by the
+// same call (the histograms are rebuilt per pass), so nothing needs clearing.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr long kNaBlocks = (kNoAllocMaxNodes + kSortIPB - 1) / kSortIPB;
+constexpr long kNaHist = 256L * kNaBlocks;
+constexpr long kNaPartial = (kNaHist + kScanChunk - 1) / kScanChunk + 1;
+constexpr long kNaWords = 4 * kNoAllocMaxNodes + kNaHist + kNaPartial;
+}  // namespace
+
+static __device__ __align__(256) unsigned scratch[kNaWords];
+
+static constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_mu;
+static unsigned* g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of `scratch` on the current device (resolved on the first call there).
+static unsigned* noalloc_scratch() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNaMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_na_ptr[dev] = static_cast<unsigned*>(ptr);
+  }
+  return g_na_ptr[dev];
+}
+
+size_t noalloc_scratch_bytes() { return sizeof(unsigned) * (size_t)kNaWords; }
+
+template <class R>
+int bin_nodes_noalloc(const R* x, long M, const GridGeom& g, const TileLayout& T, int* perm_out, unsigned* keys_out,
+                      hipStream_t s) {
+  if (M < 0 || M > kNoAllocMaxNodes) return kUnsupported;
+  if (M == 0) return kOk;
+  unsigned* base = noalloc_scratch();
+  if (!base) return kHipError;
+  SortWork W;
+  W.keys[0] = base;
+  W.keys[1] = base + kNoAllocMaxNodes;
+  W.vals[0] = reinterpret_cast<int*>(base + 2 * kNoAllocMaxNodes);
+  W.vals[1] = reinterpret_cast<int*>(base + 3 * kNoAllocMaxNodes);
+  W.hist = base + 4 * kNoAllocMaxNodes;
+  W.partial = W.hist + kNaHist;
+  W.cap = kNoAllocMaxNodes;
+  W.hist_cap = kNaHist;
+  W.partial_cap = kNaPartial;
+  return sort_bins<R>(x, M, g, T, W, perm_out, keys_out, s);
 }
 
 int build_items(const int* tile_start, long ntiles, int dense_min, unsigned* chunks, unsigned* partial,
@@ -441,6 +512,10 @@ template int bin_nodes<float>(const float*, long, const GridGeom&, const TileLay
                               int*, hipStream_t);
 template int bin_nodes<double>(const double*, long, const GridGeom&, const TileLayout&, SortWork&, int*, unsigned*,
                                int*, hipStream_t);
+template int bin_nodes_noalloc<float>(const float*, long, const GridGeom&, const TileLayout&, int*, unsigned*,
+                                      hipStream_t);
+template int bin_nodes_noalloc<double>(const double*, long, const GridGeom&, const TileLayout&, int*, unsigned*,
+                                       hipStream_t);
 template int gather<float>(const BArgs<float>&, int, hipStream_t);
 template int gather<double>(const BArgs<double>&, int, hipStream_t);
 template int scatter<float>(const BArgs<float>&, int, hipStream_t);

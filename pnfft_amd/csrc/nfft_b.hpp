@@ -135,6 +135,18 @@ int fill_zero(Cplx<R>* p, size_t n, hipStream_t s);
 template <class R>
 int precompute_tables(const BArgs<R>& A, R* psi, R* dpsi, R* x_out, hipStream_t s);
 
+// Binning that allocates nothing: the sort's scratch is a file-scope device
+// array (noalloc_scratch_bytes() bytes per GPU, held until the process ends)
+// sized for kNoAllocMaxNodes nodes.  Writes perm_out / keys_out (device arrays
+// of M entries) as bin_nodes does; no tile bounds.  Returns kUnsupported for
+// M > kNoAllocMaxNodes, kHipError if the array's address cannot be resolved.
+// Calls that overlap on one GPU share the array and race.
+constexpr long kNoAllocMaxNodes = 1L << 20;
+template <class R>
+int bin_nodes_noalloc(const R* x, long M, const GridGeom& g, const TileLayout& T, int* perm_out, unsigned* keys_out,
+                      hipStream_t s);
+size_t noalloc_scratch_bytes();
+
 // Number of sort passes for keys below key_limit.
 int sort_passes(unsigned long long key_limit);
 
