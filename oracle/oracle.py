@@ -25,9 +25,14 @@ PNFFT_WINDOW_GAUSSIAN = 1 << 13
 PNFFT_WINDOW_BSPLINE = 1 << 14
 PNFFT_WINDOW_SINC_POWER = 1 << 15
 PNFFT_WINDOW_BESSEL_I0 = 1 << 16
+PNFFT_INTERLACED = 1 << 8
+PNFFT_DIFF_IK = 1 << 12
 PNFFT_COMPUTE_F = 1 << 0
 PNFFT_COMPUTE_GRAD_F = 1 << 1
+PNFFT_COMPUTE_HESSIAN_F = 1 << 2
 PNFFT_COMPUTE_ACCUMULATED = 1 << 4
+# Hessian component order (kernel/assign.c:905-921): xx, xy, xz, yy, yz, zz
+HESS_PAIRS = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
 
 W_KB, W_GAUSS, W_BSPLINE, W_SINCPOW, W_BESSEL_I0 = range(5)
 
@@ -85,6 +90,14 @@ def _load(path):
     L.orc_psi.restype = ctypes.c_double
     L.orc_dpsi.argtypes = L.orc_psi.argtypes
     L.orc_dpsi.restype = ctypes.c_double
+    L.orc_ddpsi.argtypes = L.orc_psi.argtypes
+    L.orc_ddpsi.restype = ctypes.c_double
+    L.orc_psi_axis2.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_double, dp, dp, dp]
+    L.orc_B_trafo_hess.argtypes = [lp, lp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_long, dp, dp,
+                                   ctypes.c_int, dp]
+    L.orc_ndft_hessian_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp]
+    L.orc_ndft_hessian_c2r.argtypes = [lp, ctypes.c_long, dp, dp, dp]
     L.orc_ndft_trafo_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
     L.orc_ndft_adj_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
     L.orc_ndft_trafo_c2r.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
@@ -180,6 +193,19 @@ def dpsi(window, n, b, m, x):
     return lib().orc_dpsi(window, int(n), float(b), int(m), float(x))
 
 
+def ddpsi(window, n, b, m, x):
+    """Scalar psi''(x) (PNX(ddpsi), kernel/ndft-parallel.c:2320-2460)."""
+    return lib().orc_ddpsi(window, int(n), float(b), int(m), float(x))
+
+
+def ddpsi_axis(window, n, b, m, nx):
+    """(psi, psi', psi'') on one axis's stencil (pre_ddpsi_tensor_*, kernel/ndft-parallel.c:1956-2105)."""
+    c = 2 * m + 1
+    out = [np.zeros(c) for _ in range(3)]
+    lib().orc_psi_axis2(window, int(n), float(b), int(m), float(nx), *[_dp(a) for a in out])
+    return tuple(out)
+
+
 # ---------------------------------------------------------------------------
 # NDFT
 # ---------------------------------------------------------------------------
@@ -196,6 +222,28 @@ def ndft_trafo(N, x, fhat, compute_grad=False, compute_f=True):
     lib().orc_ndft_trafo_c2c(_lp(N), M, _dp(x), _dp(fhat.view(np.float64)), _dp(f.view(np.float64)),
                              _dp(grad.view(np.float64)) if grad is not None else None, fl)
     return (f if compute_f else None), grad
+
+
+def ndft_hessian(N, x, fhat):
+    """NDFT Hessian, c2c (kernel/ndft-parallel.c:450-510, :604-612): C[6M] in HESS_PAIRS order."""
+    N = _longs(N)
+    x = _f64(x).reshape(-1)
+    M = x.size // 3
+    fhat = _c128(fhat).reshape(-1)
+    h = np.zeros(6 * M, np.complex128)
+    lib().orc_ndft_hessian_c2c(_lp(N), M, _dp(x), _dp(fhat.view(np.float64)), _dp(h.view(np.float64)))
+    return h
+
+
+def ndft_hessian_c2r(N, x, fhat_half):
+    """NDFT Hessian of a c2r plan (kernel/ndft-parallel.c:465-476, :604-609): R[6M]."""
+    N = _longs(N)
+    x = _f64(x).reshape(-1)
+    M = x.size // 3
+    fh = _c128(fhat_half).reshape(-1)
+    h = np.zeros(6 * M)
+    lib().orc_ndft_hessian_c2r(_lp(N), M, _dp(x), _dp(fh.view(np.float64)), _dp(h))
+    return h
 
 
 def ndft_adj(N, x, f=None, grad=None, fhat_in=None):
@@ -490,6 +538,160 @@ class Plan:
         g1 = self.FH(g2)
         fhat = np.zeros(int(np.prod(self.N)), np.complex128) if fhat_in is None else _c128(fhat_in).reshape(-1).copy()
         self.DT_accumulate(g1, fhat)
+        return fhat
+
+
+    # -----------------------------------------------------------------------
+    # interlacing, ik-differentiation and Hessians (c2c and c2r plans)
+    # -----------------------------------------------------------------------
+    def _kvecs(self):
+        """Frequencies of f_hat's axes (local_N_start = -N/2; the c2r half spectrum has k2 <= 0)."""
+        ks = [np.arange(-(v // 2), v - v // 2, dtype=np.float64) for v in self.N]
+        if self.c2r:
+            ks[2] = np.arange(-(self.N[2] // 2), 1, dtype=np.float64)
+        return ks
+
+    def _k(self, t):
+        shp = [1, 1, 1]
+        shp[t] = -1
+        return self._kvecs()[t].reshape(shp)
+
+    def _fshape(self):
+        return tuple(self.Nh) if self.c2r else tuple(self.N)
+
+    def interlace_phase(self, sign):
+        """convolution_due_to_interlacing (kernel/matrix_D.c:282-317): the factor
+        e^{-sign pi i sum_t k_t/n_t}; trafo_D passes FFTW_FORWARD (-1), adjoint_D FFTW_BACKWARD (+1)."""
+        ks = self._kvecs()
+        h = ((ks[0] / self.n[0])[:, None, None] + (ks[1] / self.n[1])[None, :, None]
+             + (ks[2] / self.n[2])[None, None, :])
+        return np.exp(-sign * np.pi * 1j * h)
+
+    def shifted_x(self, x):
+        """Nodes of the interlaced pass: x_t + 1/(2 n_t), folded back into [-1/2, 1/2)
+        (loop_over_particles_trafo / _adj, kernel/ndft-parallel.c:2732-2753, :2914-2934)."""
+        xs = _f64(x).reshape(-1, 3) + 0.5 / np.asarray(self.n, np.float64)
+        xs = np.where(xs >= 0.5, xs - 1.0, xs)
+        return np.ascontiguousarray(xs.reshape(-1))
+
+    def _D_any(self, fhat):
+        if self.c2r:
+            return self._D_half(fhat)
+        return self.D(fhat).reshape(self.N)
+
+    def _DT_any(self, g1, fhat):
+        if self.c2r:
+            fhat += self._D_half(g1).reshape(-1)
+        else:
+            self.DT_accumulate(np.asarray(g1).reshape(-1), fhat)
+
+    def _grid_fwd(self, g1):
+        """F and the ghost exchange of g1 (f_hat layout): the padded grid B reads."""
+        if self.c2r:
+            return np.ascontiguousarray(np.pad(self._F_c2r(g1), [(self.m, self.m)] * 3, mode="wrap"))
+        return np.ascontiguousarray(self.ghost_exchange(self.F(np.asarray(g1).reshape(-1))))
+
+    def _gather(self, x, gpad, f=None, grad=None, hess=None):
+        """B at the nodes x (adds onto the given outputs)."""
+        M = x.size // 3
+        isr = 1 if self.c2r else 0
+
+        def v(a):
+            return None if a is None else _dp(a if isr else a.view(np.float64))
+
+        fl = (PNFFT_COMPUTE_F if f is not None else 0) | (PNFFT_COMPUTE_GRAD_F if grad is not None else 0)
+        args = (_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M, _dp(x),
+                _dp(gpad if isr else gpad.view(np.float64)), isr)
+        if fl:
+            lib().orc_B_trafo(*args, v(f), v(grad), fl)
+        if hess is not None:
+            lib().orc_B_trafo_hess(*args, v(hess))
+
+    def _grid_adj(self, x, f=None, grad=None):
+        """B^T, ghost reduce and F^H of node values (f_hat layout)."""
+        M = x.size // 3
+        isr = 1 if self.c2r else 0
+        ng = [v + 2 * self.m for v in self.no]
+        gpad = np.zeros(ng, np.float64 if isr else np.complex128)
+        fl = (PNFFT_COMPUTE_F if f is not None else 0) | (PNFFT_COMPUTE_GRAD_F if grad is not None else 0)
+
+        def v(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64 if isr else np.complex128)
+            return _dp(a if isr else a.view(np.float64)), a
+
+        fv, gv = v(f), v(grad)
+        lib().orc_B_adj(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M, _dp(x),
+                        fv[0] if fv else None, gv[0] if gv else None, isr,
+                        _dp(gpad if isr else gpad.view(np.float64)), fl)
+        g2 = self.ghost_reduce(gpad)
+        return self._FH_r2c(g2) if self.c2r else self.FH(g2).reshape(self.N)
+
+    def trafo_general(self, x, fhat, compute_f=True, compute_grad=False, compute_hess=False):
+        """pnfft_trafo with the plan's PNFFT_INTERLACED / PNFFT_DIFF_IK flags and Hessians
+        (api/api-basic.c:100-244): returns (f, grad, hess), complex for c2c plans, real for c2r.
+
+        Interlacing (api/api-basic.c:233-240): the average of the plain pass and a pass at the
+        nodes shifted by half a cell with D followed by the phase e^{+pi i sum k_t/n_t}
+        (kernel/matrix_D.c:227-251); the 1/2 of kernel/assign.c:688 is applied to g1 here.
+        ik-differentiation (:100-167): f from B F D f_hat; grad_t from B F (-2 pi i k_t D f_hat);
+        Hessian (a, b) from B F (-4 pi^2 k_a k_b D f_hat), all with the f-gather.  Without
+        DIFF_IK the window-derivative gathers give grad and Hessian (kernel/assign.c:763-1027)."""
+        x = _f64(x).reshape(-1)
+        M = x.size // 3
+        dt = np.float64 if self.c2r else np.complex128
+        f = np.zeros(M, dt) if compute_f else None
+        grad = np.zeros(3 * M, dt) if compute_grad else None
+        hess = np.zeros(6 * M, dt) if compute_hess else None
+        passes = [(False, 0.5), (True, 0.5)] if self.flags & PNFFT_INTERLACED else [(False, 1.0)]
+        for il, sc in passes:
+            xx = self.shifted_x(x) if il else x
+            g1 = self._D_any(fhat) * sc
+            if il:
+                g1 = g1 * self.interlace_phase(-1)
+            if not self.flags & PNFFT_DIFF_IK:
+                self._gather(xx, self._grid_fwd(g1), f, grad, hess)
+                continue
+            if compute_f:
+                self._gather(xx, self._grid_fwd(g1), f=f)
+            if compute_grad:  # trafo_scale_ik_diff_c2c, kernel/ndft-parallel.c:3034-3054
+                for t in range(3):
+                    tmp = np.zeros(M, dt)
+                    self._gather(xx, self._grid_fwd(-2j * np.pi * self._k(t) * g1), f=tmp)
+                    grad[t::3] += tmp
+            if compute_hess:  # trafo_scale_ik_diff2_c2c, :3056-3087
+                for q, (a, b) in enumerate(HESS_PAIRS):
+                    tmp = np.zeros(M, dt)
+                    self._gather(xx, self._grid_fwd(-4.0 * np.pi ** 2 * self._k(a) * self._k(b) * g1), f=tmp)
+                    hess[q::6] += tmp
+        return f, grad, hess
+
+    def adj_general(self, x, f=None, grad=None, fhat_in=None):
+        """pnfft_adj with the plan's PNFFT_INTERLACED / PNFFT_DIFF_IK flags (api/api-basic.c:247-378):
+        f_hat (+)= sum over passes of D^T [phase] (1/2) g1, with g1 = F^H B^T (f, grad) (plain) or
+        F^H B^T f + sum_t 2 pi i k_t F^H B^T grad_t (ik, adjoint_scale_ik_diff_c2c :3012-3032)."""
+        x = _f64(x).reshape(-1)
+        shape = self._fshape()
+        fhat = (np.zeros(int(np.prod(shape)), np.complex128) if fhat_in is None
+                else _c128(fhat_in).reshape(-1).copy())
+        passes = [(False, 0.5), (True, 0.5)] if self.flags & PNFFT_INTERLACED else [(False, 1.0)]
+        for il, sc in passes:
+            xx = self.shifted_x(x) if il else x
+            if not self.flags & PNFFT_DIFF_IK:
+                g1 = self._grid_adj(xx, f=f, grad=grad)
+            else:
+                g1 = np.zeros(shape, np.complex128)
+                if f is not None:
+                    g1 = g1 + self._grid_adj(xx, f=f)
+                if grad is not None:
+                    gr = np.asarray(grad).reshape(-1, 3)
+                    for t in range(3):
+                        g1 = g1 + 2j * np.pi * self._k(t) * self._grid_adj(xx, f=gr[:, t].copy())
+            g1 = g1 * sc
+            if il:
+                g1 = g1 * self.interlace_phase(+1)
+            self._DT_any(g1, fhat)
         return fhat
 
 

@@ -33,6 +33,7 @@
 #define F_USE_FK_GAUSS_T   (1u << 17)
 #define C_COMPUTE_F        (1u << 0)
 #define C_COMPUTE_GRAD_F   (1u << 1)
+#define C_COMPUTE_HESSIAN  (1u << 2)
 
 enum { W_KB = 0, W_GAUSS = 1, W_BSPLINE = 2, W_SINCPOW = 3, W_BESSEL_I0 = 4 };
 
@@ -251,6 +252,80 @@ static double bi0_d1d(double xn, long n, double b, int m)
                  : -(0.5 * b * n) * (0.5 * b * n) * xn / (double)n;
 }
 
+/* Second derivatives on the support (kernel/ndft-parallel.c:2220-2237 for the
+ * Bessel-I0 window, :2272-2285 for Kaiser-Bessel), xn = x*n. */
+static double bi0_dd1d(double xn, long n, double b, int m)
+{
+  double d = (double)m * m - xn * xn;
+  if (d < 0) return 0.0;
+  if (d > 0) {
+    double r = sqrt(d), y = xn * xn;
+    return 0.5 * b * (double)n * n / d * (b * y * orc_bessel_i0(b * r) - orc_bessel_i1(b * r) / r * (y + (double)m * m));
+  }
+  double q = b * b * m * (double)n, h = b * (double)n;
+  return q * q / 16.0 - h * h / 4.0;
+}
+
+static double kb_dd1d(double xn, long n, double b, int m, double psi, double dpsi)
+{
+  double d = (double)m * m - xn * xn;
+  double r = sqrt(fabs(d));
+  double nn = (double)n * n, bx = b * xn;
+  if (d == 0) {
+    double bn = b * n, bm = b * m;
+    return b * bn * bn / (15.0 * ORC_PI) * (bm * bm - 5.0);
+  }
+  double hyp = (d < 0) ? cos(b * r) : cosh(b * r);
+  return 3.0 * n * xn * dpsi / d + nn * psi / d * (1.0 + bx * bx) - b * nn / (ORC_PI * d) * hyp;
+}
+
+/* psi'' on the stencil, pre_ddpsi_tensor_* of kernel/ndft-parallel.c:1956-2105
+ * (psi and dpsi of the same node as inputs, as the reference passes them). */
+static void ddpsi_axis(int window, long n, double b, int m, double nx, const double *psi,
+                       const double *dpsi, double *ddpsi)
+{
+  const int c = 2 * m + 1;
+  const double fl = floor(nx);
+  const double uj = nx - fl + m;
+  const double nn = (double)n * n;
+  switch (window) {
+    case W_GAUSS: /* :2010-2024 */
+      for (int s = 0; s < c; s++)
+        ddpsi[s] = 2.0 * nn / b * (2.0 / b * (uj - s) * (uj - s) - 1.0) * psi[s];
+      break;
+    case W_BSPLINE: { /* :2026-2049: n^2 (M_{2m-2}(t) - 2 M_{2m-2}(t-1) + M_{2m-2}(t-2)) */
+      double Q[64];
+      bspline_column(2 * m - 2, 1.0 - (nx - fl), Q); /* Q[d] = M_{2m-2}(1 - frac + d) */
+      for (int s = 0; s < c; s++) {
+        double q3 = (s >= 3 && s - 3 <= 2 * m - 3) ? Q[s - 3] : 0.0;
+        double q2 = (s >= 2 && s - 2 <= 2 * m - 3) ? Q[s - 2] : 0.0;
+        double q1 = (s >= 1 && s - 1 <= 2 * m - 3) ? Q[s - 1] : 0.0;
+        ddpsi[s] = nn * (q3 - 2.0 * q2 + q1);
+      }
+      break;
+    }
+    case W_SINCPOW: /* :2052-2071 */
+      for (int s = 0; s < c; s++) {
+        double y = ORC_PI * (uj - s) / b;
+        double a = ORC_PI * (double)n / b;
+        if (fabs(y) > ORC_EPS) {
+          double t = tan(y);
+          ddpsi[s] = 2.0 * m * a * (1.0 / t - 1.0 / y) * dpsi[s] +
+                     2.0 * m * a * a * (1.0 / (y * y) - 1.0 - 1.0 / (t * t)) * psi[s];
+        } else {
+          ddpsi[s] = -2.0 * m * a * a / (3.0 * b);
+        }
+      }
+      break;
+    case W_BESSEL_I0: /* :2073-2089 */
+      for (int s = 0; s < c; s++) ddpsi[s] = bi0_dd1d(uj - s, n, b, m);
+      break;
+    default: /* Kaiser-Bessel, :2091-2105 */
+      for (int s = 0; s < c; s++) ddpsi[s] = kb_dd1d(uj - s, n, b, m, psi[s], dpsi[s]);
+      break;
+  }
+}
+
 void orc_psi_axis(int window, long n, double b, int m, double nx,
                   double *psi, double *dpsi)
 {
@@ -325,6 +400,14 @@ void orc_psi_axis(int window, long n, double b, int m, double nx,
   }
 }
 
+/* psi, psi' and psi'' of one axis's stencil (ddpsi may not be null). */
+void orc_psi_axis2(int window, long n, double b, int m, double nx, double *psi, double *dpsi,
+                   double *ddpsi)
+{
+  orc_psi_axis(window, n, b, m, nx, psi, dpsi);
+  ddpsi_axis(window, n, b, m, nx, psi, dpsi, ddpsi);
+}
+
 /* Scalar window psi(x) and psi'(x) (PNX(psi), PNX(dpsi): :2288-2318). */
 double orc_psi(int window, long n, double b, int m, double x)
 {
@@ -335,6 +418,38 @@ double orc_psi(int window, long n, double b, int m, double x)
     case W_SINCPOW: return pow(orc_sinc(ORC_PI * xn / b), 2.0 * m) / b;
     case W_BESSEL_I0: return bi0_1d(xn, b, m);
     default: return kb_1d(xn, b, m);
+  }
+}
+
+double orc_dpsi(int window, long n, double b, int m, double x);
+
+/* Scalar psi''(x) (PNX(ddpsi), kernel/ndft-parallel.c:2320-2340, 2350-2460). */
+double orc_ddpsi(int window, long n, double b, int m, double x)
+{
+  double xn = (double)n * x;
+  double nn = (double)n * n;
+  switch (window) {
+    case W_GAUSS: {
+      double y = xn * xn / b;
+      return 2.0 * n / b * n * exp(-y) / sqrt(ORC_PI * b) * (2.0 * y - 1.0);
+    }
+    case W_BSPLINE:
+      return nn * (orc_bspline(2 * m - 2, xn + m) - 2.0 * orc_bspline(2 * m - 2, xn + m - 1) +
+                   orc_bspline(2 * m - 2, xn + m - 2));
+    case W_SINCPOW: {
+      double y = ORC_PI * xn / b, a = ORC_PI * (double)n / b;
+      if (fabs(y) > ORC_EPS) {
+        double t = tan(y);
+        return 2.0 * m * a * (1.0 / t - 1.0 / y) * orc_dpsi(window, n, b, m, x) +
+               2.0 * m * a * a * (1.0 / (y * y) - 1.0 - 1.0 / (t * t)) * orc_psi(window, n, b, m, x);
+      }
+      return -2.0 * m / (3.0 * b) * a * a;
+    }
+    case W_BESSEL_I0: return bi0_dd1d(xn, n, b, m);
+    default: {
+      double p = kb_1d(xn, b, m);
+      return kb_dd1d(xn, n, b, m, p, kb_d1d(xn, n, b, m, p));
+    }
   }
 }
 
@@ -487,6 +602,34 @@ void orc_ndft_adj_c2c(const long *N, long M, const double *x, const double *f,
   (void)Ntot;
 }
 
+/* NDFT Hessian, c2c (kernel/ndft-parallel.c:450-510, scaled by -4 pi^2 at
+ * :604-612): hess[6j + c] = -4 pi^2 sum_k k_a k_b f_hat_k e^{-2 pi i k x_j}
+ * for (a, b) = xx, xy, xz, yy, yz, zz; overwrites hess. */
+void orc_ndft_hessian_c2c(const long *N, long M, const double *x, const double *fhat, double *hess)
+{
+  const long N0 = N[0], N1 = N[1], N2 = N[2];
+  static const int A[6] = {0, 0, 0, 1, 1, 2}, B[6] = {0, 1, 2, 1, 2, 2};
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long j = 0; j < M; j++) {
+    double h[12] = {0};
+    for (long i0 = 0; i0 < N0; i0++)
+      for (long i1 = 0; i1 < N1; i1++)
+        for (long i2 = 0; i2 < N2; i2++) {
+          double k[3] = {(double)(i0 - N0 / 2), (double)(i1 - N1 / 2), (double)(i2 - N2 / 2)};
+          const double *fh = fhat + 2 * ((i0 * N1 + i1) * N2 + i2);
+          double ph = -2.0 * ORC_PI * (k[0] * x[3 * j] + k[1] * x[3 * j + 1] + k[2] * x[3 * j + 2]);
+          double er = cos(ph), ei = sin(ph);
+          double pr = fh[0] * er - fh[1] * ei, pi = fh[0] * ei + fh[1] * er;
+          for (int q = 0; q < 6; q++) {
+            double kk = k[A[q]] * k[B[q]];
+            h[2 * q] += kk * pr;
+            h[2 * q + 1] += kk * pi;
+          }
+        }
+    for (int q = 0; q < 12; q++) hess[12 * j + q] = -4.0 * ORC_PI * ORC_PI * h[q];
+  }
+}
+
 /* ---------------------------------------------------------------------------
  * c2r NDFT (kernel/ndft-parallel.c:356-374, :524-533, :569-573, :594-603).
  * f_hat holds the half spectrum [k0][k1][k2], k2 in [-N2/2, 0] (N2/2+1 entries).
@@ -529,6 +672,31 @@ void orc_ndft_trafo_c2r(const long *N, long M, const double *x, const double *fh
     if ((cflags & C_COMPUTE_F) && f) f[j] = fr;
     if ((cflags & C_COMPUTE_GRAD_F) && grad)
       for (int t = 0; t < 3; t++) grad[3 * j + t] = -2.0 * ORC_PI * g[t];
+  }
+}
+
+/* c2r NDFT Hessian (kernel/ndft-parallel.c:465-476, :494-499, :604-609):
+ * -4 pi^2 sum over the half spectrum of 2 k_a k_b Re(f_hat_k e^{-2 pi i k x}),
+ * skipping k = 0 and the entries is_hermitian() marks; overwrites hess. */
+void orc_ndft_hessian_c2r(const long *N, long M, const double *x, const double *fhat, double *hess)
+{
+  const long N0 = N[0], N1 = N[1], N2 = N[2], H2 = N2 / 2 + 1;
+  static const int A[6] = {0, 0, 0, 1, 1, 2}, B[6] = {0, 1, 2, 1, 2, 2};
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long j = 0; j < M; j++) {
+    double h[6] = {0};
+    for (long i0 = 0; i0 < N0; i0++)
+      for (long i1 = 0; i1 < N1; i1++)
+        for (long i2 = 0; i2 < H2; i2++) {
+          long k0 = i0 - N0 / 2, k1 = i1 - N1 / 2, k2 = i2 - N2 / 2;
+          if ((k0 == 0 && k1 == 0 && k2 == 0) || is_hermitian(k0, k1, k2, N0, N1, N2)) continue;
+          double k[3] = {(double)k0, (double)k1, (double)k2};
+          const double *fh = fhat + 2 * ((i0 * N1 + i1) * H2 + i2);
+          double ph = -2.0 * ORC_PI * (k[0] * x[3 * j] + k[1] * x[3 * j + 1] + k[2] * x[3 * j + 2]);
+          double pr = fh[0] * cos(ph) - fh[1] * sin(ph);
+          for (int q = 0; q < 6; q++) h[q] += 2.0 * k[A[q]] * k[B[q]] * pr;
+        }
+    for (int q = 0; q < 6; q++) hess[6 * j + q] = -4.0 * ORC_PI * ORC_PI * h[q];
   }
 }
 
@@ -625,6 +793,43 @@ void orc_B_trafo(const long *n, const long *no, int m, int window, const double 
       if (want_g) for (int t = 0; t < 3; t++) {
         grad[6 * j + 2 * t] += acc[2 + 2 * t]; grad[6 * j + 2 * t + 1] += acc[3 + 2 * t];
       }
+    }
+  }
+}
+
+/* Hessian gather (assign_hessian_f_c2c_pre_psi, kernel/assign.c:881-921, and
+ * its r2r twin): hess[6j + c], c = xx, xy, xz, yy, yz, zz, added onto the
+ * output as the reference's kernels do. */
+void orc_B_trafo_hess(const long *n, const long *no, int m, int window, const double *b, long M,
+                      const double *x, const double *grid, int is_real, double *hess)
+{
+  const int c = 2 * m + 1;
+  const long g1 = no[1] + 2 * m, g2 = no[2] + 2 * m;
+  const int cplx = is_real ? 1 : 2;
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < M; j++) {
+    double psi[3 * 64], dpsi[3 * 64], dd[3 * 64];
+    long u[3];
+    node_windows(window, n, b, m, x + 3 * j, psi, dpsi, u, no);
+    for (int t = 0; t < 3; t++)
+      ddpsi_axis(window, n[t], b[t], m, (double)n[t] * x[3 * j + t], psi + c * t, dpsi + c * t, dd + c * t);
+    double acc[12] = {0};
+    for (int l0 = 0; l0 < c; l0++)
+      for (int l1 = 0; l1 < c; l1++) {
+        double xy = psi[l0] * psi[c + l1], dxy = dpsi[l0] * psi[c + l1], xdy = psi[l0] * dpsi[c + l1];
+        double dxdy = dpsi[l0] * dpsi[c + l1], ddxy = dd[l0] * psi[c + l1], xddy = psi[l0] * dd[c + l1];
+        long row = ((u[0] + l0) * g1 + (u[1] + l1)) * g2 + u[2];
+        for (int l2 = 0; l2 < c; l2++) {
+          const double *gp = grid + cplx * (row + l2);
+          double gr = gp[0], gi = is_real ? 0.0 : gp[1];
+          double z = psi[2 * c + l2], dz = dpsi[2 * c + l2], ddz = dd[2 * c + l2];
+          double w[6] = {ddxy * z, dxdy * z, dxy * dz, xddy * z, xdy * dz, xy * ddz};
+          for (int q = 0; q < 6; q++) { acc[2 * q] += w[q] * gr; acc[2 * q + 1] += w[q] * gi; }
+        }
+      }
+    for (int q = 0; q < 6; q++) {
+      if (is_real) hess[6 * j + q] += acc[2 * q];
+      else { hess[12 * j + 2 * q] += acc[2 * q]; hess[12 * j + 2 * q + 1] += acc[2 * q + 1]; }
     }
   }
 }

@@ -1,0 +1,200 @@
+"""Pins of the oracle's second derivatives, Hessians, interlacing and
+ik-differentiation (CPU only).
+
+Each check compares the oracle with something independent of the formula it
+restates: central differences, the direct sums of the NDFT, numpy brute force,
+an exact adjointness identity, or the c2c transform of the Hermitian extension.
+"""
+import numpy as np
+import pytest
+
+from pnfft_amd.workloads import SplitMix64
+
+WINDOWS = {"kb": 0, "gauss": 1 << 13, "bspline": 1 << 14, "sincpow": 1 << 15, "besseli0": 1 << 16}
+INTERLACED = 1 << 8
+DIFF_IK = 1 << 12
+
+
+def rel(a, b):
+    a = np.asarray(a).reshape(-1)
+    b = np.asarray(b).reshape(-1)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ---------------------------------------------------------------- psi''
+@pytest.mark.parametrize("wname", list(WINDOWS))
+def test_ddpsi_is_derivative_of_dpsi(oracle, wname):
+    flags = WINDOWS[wname]
+    m, n = 5, 32
+    w = oracle.window_id(flags)
+    b = oracle.window_b([16] * 3, [n] * 3, m, flags)[0]
+    h = 1e-6
+    # (an even count keeps x = 0 out: there the reference's sinc-power psi' formula,
+    # 1/tan(y) - 1/y, cancels and a difference quotient of it is noise)
+    for x in np.linspace(-(m - 0.3) / n, (m - 0.3) / n, 22):
+        fd = (oracle.dpsi(w, n, b, m, x + h) - oracle.dpsi(w, n, b, m, x - h)) / (2 * h)
+        d = oracle.ddpsi(w, n, b, m, x)
+        scale = max(1.0, abs(d), abs(oracle.dpsi(w, n, b, m, x)) * n)
+        assert abs(fd - d) <= 2e-6 * scale, (wname, x, fd, d)
+    # second difference of psi itself (independent of psi')
+    h = 1e-4
+    for x in np.linspace(-(m - 0.5) / n, (m - 0.5) / n, 11):
+        fd = (oracle.psi(w, n, b, m, x + h) - 2 * oracle.psi(w, n, b, m, x) + oracle.psi(w, n, b, m, x - h)) / h ** 2
+        d = oracle.ddpsi(w, n, b, m, x)
+        assert abs(fd - d) <= 1e-4 * max(1.0, abs(d)), (wname, x, fd, d)
+
+
+@pytest.mark.parametrize("wname", list(WINDOWS))
+@pytest.mark.parametrize("m", [2, 4, 6])
+def test_ddpsi_axis_matches_scalar(oracle, wname, m):
+    """stencil value s of psi'' equals the scalar psi''(x - l_s/n), l_s = floor(n x) - m + s."""
+    flags = WINDOWS[wname]
+    n = 32
+    w = oracle.window_id(flags)
+    b = oracle.window_b([16] * 3, [n] * 3, m, flags)[0]
+    rng = np.random.default_rng(11)
+    for x in list(rng.uniform(-0.5, 0.5, 15)) + [0.0, 3.0 / 32]:
+        nx = n * x
+        fl = np.floor(nx)
+        psi, dpsi, dd = oracle.ddpsi_axis(w, n, b, m, nx)
+        p2, d2 = oracle.psi_axis(w, n, b, m, nx)
+        assert np.array_equal(psi, p2) and np.array_equal(dpsi, d2)
+        for s in range(2 * m + 1):
+            if wname == "kb" and s == 0 and nx - fl > 0:
+                continue  # outside the support (sin continuation)
+            ref = oracle.ddpsi(w, n, b, m, x - (fl - m + s) / n)
+            assert abs(dd[s] - ref) <= 1e-9 * max(1.0, abs(ref)), (wname, s, x, dd[s], ref)
+
+
+def test_kb_ddpsi_at_support_edge(oracle):
+    """d = 0 (|n x| = m): the limit b (b n)^2 ((b m)^2 - 5) / (15 pi) of the regular branch."""
+    m, n = 4, 64
+    b = np.pi * (2 - 1 / 2.0)
+    x = m / n
+    edge = oracle.ddpsi(0, n, b, m, x)
+    assert abs(edge - b * (b * n) ** 2 * ((b * m) ** 2 - 5) / (15 * np.pi)) < 1e-9 * abs(edge)
+    near = oracle.ddpsi(0, n, b, m, x * (1 - 1e-7))
+    assert abs(near - edge) < 1e-4 * abs(edge)
+
+
+# ---------------------------------------------------------------- NDFT Hessian
+def test_ndft_hessian_brute_force_and_finite_difference(oracle):
+    N = (6, 4, 8)
+    rng = SplitMix64(41)
+    M = 7
+    x = rng.uniform(3 * M) - 0.5
+    fh = rng.complex_normal(int(np.prod(N)))
+    h = oracle.ndft_hessian(N, x, fh).reshape(M, 6)
+    # numpy brute force from the definition
+    k = np.stack(np.meshgrid(*[np.arange(-(v // 2), v - v // 2) for v in N], indexing="ij"), -1).reshape(-1, 3)
+    e = np.exp(-2j * np.pi * x.reshape(M, 3) @ k.T) * fh[None, :]
+    for q, (a, b) in enumerate(((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))):
+        ref = -4 * np.pi ** 2 * (e * (k[:, a] * k[:, b])[None, :]).sum(1)
+        assert rel(h[:, q], ref) < 1e-13
+    # central difference of the NDFT gradient
+    eps = 1e-6
+    for a in range(3):
+        xp = x.copy().reshape(M, 3)
+        xm = xp.copy()
+        xp[:, a] += eps
+        xm[:, a] -= eps
+        _, gp = oracle.ndft_trafo(N, xp, fh, compute_grad=True)
+        _, gm = oracle.ndft_trafo(N, xm, fh, compute_grad=True)
+        fd = ((gp - gm) / (2 * eps)).reshape(M, 3)
+        for b in range(3):
+            q = {(0, 0): 0, (0, 1): 1, (0, 2): 2, (1, 1): 3, (1, 2): 4, (2, 2): 5}[tuple(sorted((a, b)))]
+            assert rel(fd[:, b], h[:, q]) < 1e-6
+
+
+def test_ndft_hessian_c2r_is_c2c_of_hermitian_extension(oracle):
+    N = (8, 6, 8)
+    rng = SplitMix64(42)
+    M = 9
+    x = rng.uniform(3 * M) - 0.5
+    full = oracle.semirandom_hermitian(N)
+    half = oracle.semirandom_hermitian(N, half=True)
+    hc = oracle.ndft_hessian(N, x, full)
+    hr = oracle.ndft_hessian_c2r(N, x, half)
+    assert np.abs(hc.imag).max() < 1e-9 * np.abs(hc.real).max()
+    assert rel(hr, hc.real) < 1e-12
+
+
+# ---------------------------------------------------------------- fast transforms
+def _case(seed, N=(12, 8, 10), n=(24, 16, 20), M=60):
+    rng = SplitMix64(seed)
+    x = rng.uniform(3 * M) - 0.5
+    fh = rng.complex_normal(int(np.prod(N)))
+    return N, n, x, fh, M
+
+
+@pytest.mark.parametrize("wname", ["kb", "gauss", "bspline", "sincpow", "besseli0"])
+def test_hessian_gather_matches_ndft(oracle, wname):
+    N, n, x, fh, M = _case(7, N=(16, 16, 16), n=(32, 32, 32))
+    ho = oracle.ndft_hessian(N, x, fh)
+    errs = []
+    for m in (4, 6):
+        P = oracle.Plan(N, n, m, WINDOWS[wname])
+        _, _, h = P.trafo_general(x, fh, compute_f=False, compute_hess=True)
+        errs.append(rel(h, ho))
+    tol = {"kb": 1e-7, "gauss": 1e-4, "bspline": 1e-2, "sincpow": 1e-3, "besseli0": 1e-6}[wname]
+    assert errs[1] < tol and errs[1] < errs[0], errs
+
+
+@pytest.mark.parametrize("flags", [DIFF_IK, INTERLACED, INTERLACED | DIFF_IK])
+def test_interlaced_and_ik_match_ndft(oracle, flags):
+    N, n, x, fh, M = _case(8)
+    fo, go = oracle.ndft_trafo(N, x, fh, compute_grad=True)
+    ho = oracle.ndft_hessian(N, x, fh)
+    for m in (3, 5):
+        f, g, h = oracle.Plan(N, n, m, flags).trafo_general(x, fh, True, True, True)
+        f0, g0, h0 = oracle.Plan(N, n, m, 0).trafo_general(x, fh, True, True, True)
+        tol = 2e-3 if m == 3 else 2e-6
+        assert rel(f, fo) < tol and rel(g, go) < 10 * tol and rel(h, ho) < 100 * tol
+        if flags & INTERLACED:  # interlacing cancels the leading aliasing error
+            assert rel(f, fo) < 0.5 * rel(f0, fo)
+        if flags & DIFF_IK:  # ik: the derivative comes from the grid, not the window
+            assert rel(g, go) < rel(g0, go)
+    # the plain-flag general path equals the plain transform
+    P = oracle.Plan(N, n, 4, 0)
+    f1, g1 = P.trafo(x, fh, compute_grad=True)
+    f2, g2, _ = P.trafo_general(x, fh, True, True, False)
+    assert rel(f2, f1) < 1e-15 and rel(g2, g1) < 1e-15
+
+
+@pytest.mark.parametrize("flags", [0, DIFF_IK, INTERLACED, INTERLACED | DIFF_IK])
+def test_general_adjointness(oracle, flags):
+    """<trafo f_hat, (f, g)> == <f_hat, adj (f, g)> for f and the gradient."""
+    N, n, x, fh, M = _case(9)
+    rng = SplitMix64(10)
+    fv = rng.complex_normal(M)
+    gv = rng.complex_normal(3 * M)
+    P = oracle.Plan(N, n, 4, flags)
+    f, g, _ = P.trafo_general(x, fh, True, True, False)
+    a_f = P.adj_general(x, f=fv)
+    a_g = P.adj_general(x, grad=gv)
+    assert abs(np.vdot(fv, f) - np.vdot(a_f, fh)) < 1e-12 * abs(np.vdot(fv, f))
+    assert abs(np.vdot(gv, g) - np.vdot(a_g, fh)) < 1e-12 * abs(np.vdot(gv, g))
+    a_fg = P.adj_general(x, f=fv, grad=gv)
+    assert rel(a_fg, a_f + a_g) < 1e-13
+    # adjoint against the adjoint NDFT
+    ref = oracle.ndft_adj(N, x, f=fv, grad=gv)
+    assert rel(a_fg, ref) < 1e-4
+
+
+@pytest.mark.parametrize("flags", [DIFF_IK, INTERLACED, INTERLACED | DIFF_IK])
+def test_general_c2r_matches_c2c_of_hermitian_extension(oracle, flags):
+    N, n = (8, 8, 8), (16, 16, 16)
+    rng = SplitMix64(12)
+    M = 40
+    x = rng.uniform(3 * M) - 0.5
+    full = oracle.semirandom_hermitian(N)
+    half = oracle.semirandom_hermitian(N, half=True)
+    Pc = oracle.Plan(N, n, 4, flags)
+    Pr = oracle.Plan(N, n, 4, flags, c2r=True)
+    fc, gc, hc = Pc.trafo_general(x, full, True, True, True)
+    fr, gr, hr = Pr.trafo_general(x, half, True, True, True)
+    assert rel(fr, fc.real) < 1e-12 and rel(gr, gc.real) < 1e-12 and rel(hr, hc.real) < 1e-12
+    fo, go = oracle.ndft_trafo_c2r(N, x, half, compute_grad=True)
+    # the reference's c2r NDFT gradient is minus d/dx of its c2r f (kernel/ndft-parallel.c:530-532,
+    # :596-599); the fast gradients, window-derivative or ik, are d/dx
+    assert rel(fr, fo) < 1e-3 and rel(gr, -go) < 1e-2
