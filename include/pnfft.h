@@ -182,8 +182,10 @@ extern "C" {
    * trafo it holds F D f_hat (F f_hat with PNFFT_OMIT_DECONV), after adj B^T f.              \
    * The plan's D factors and window values carry reciprocal powers of two per               \
    * axis (2^e and 2^-e, e = exponent of psi(0)), so the grid is 2^e times the               \
-   * reference's g2 when D was applied (exact scaling; B F D is unchanged).                   \
-   * last_error: message of the last rejected call, or NULL.                                  \
+   * reference's g2 (exact scaling; B F D and, with PNFFT_OMIT_DECONV, B F are                \
+   * unchanged).                                                                              \
+   * last_error: why the most recent plan creation, trafo, adj, precompute_psi,               \
+   * trafo_grad_x_vjp or node binning call failed, or NULL if it succeeded.                   \
    * node_bins_noalloc: no counterpart in the reference; for callers that cannot               \
    * allocate device memory (code under HIP graph capture, tight serving loops).              \
    * Like node_bins for M nodes whose coordinates x (R[3M]) and outputs order                 \

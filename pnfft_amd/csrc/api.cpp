@@ -70,6 +70,10 @@ namespace pnfft_amd {
 // ---------------------------------------------------------------------------
 static thread_local std::string g_last_error;
 
+// Each API call that can fail starts with a clean state, so last_error names
+// the failure of the most recent such call (or is NULL after a success).
+static void clear_error() { g_last_error.clear(); }
+
 static void set_error(const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -220,6 +224,11 @@ struct PlanT {
   Cplx<R>* d_buf2 = nullptr;
   double* d_tab = nullptr;
   std::vector<double> h_tab;
+  double* d_tab_scale = nullptr;  // 2^e per axis and frequency: D of PNFFT_OMIT_DECONV (window scale only)
+  // complex per-axis factors of interlaced / ik-differentiated passes (kCtabSlots
+  // tables of N0 + N1 + N2 (re, im) pairs, filled per call in pinned host memory)
+  double* d_ctab = nullptr;
+  double* h_ctab = nullptr;
   GridGeom geom{};
   TileLayout tiles{};
   EventTimer evt;
@@ -243,7 +252,14 @@ struct NodesT {
   R* d_x = nullptr;
   Cplx<R>* d_f = nullptr;
   Cplx<R>* d_grad = nullptr;
+  Cplx<R>* d_hess = nullptr;
   R* d_gx = nullptr;
+  // scratch of the ik-differentiated transforms (one component of node values)
+  // and of the Hessian gather (window table, stencil starts)
+  Cplx<R>* d_tmp = nullptr;
+  R* d_hess_tab = nullptr;
+  int* d_hess_st = nullptr;
+  size_t tmp_cap = 0, hess_tab_cap = 0, hess_st_cap = 0;
 
   // binning
   int* d_perm = nullptr;
@@ -299,6 +315,64 @@ static void tables_compute(PlanT<R>* P) {
       P->h_tab[off++] = std::ldexp(host::inv_phi_hat(P->win, i - P->N[t] / 2, P->n[t], P->b[t], P->m), e[t]);
   if (P->d_tab)
     (void)hipMemcpy(P->d_tab, P->h_tab.data(), sizeof(double) * P->h_tab.size(), hipMemcpyHostToDevice);
+  std::vector<double> sc(P->h_tab.size());
+  off = 0;
+  for (int t = 0; t < 3; t++)
+    for (long i = 0; i < P->N[t]; i++) sc[off++] = std::ldexp(1.0, e[t]);
+  if (P->d_tab_scale)
+    (void)hipMemcpy(P->d_tab_scale, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice);
+}
+
+// Slots of the complex factor tables: per pass (interlaced plans have two), the
+// transform itself (0), the three gradient components (1..3) and the six
+// Hessian components (4..9, order xx, xy, xz, yy, yz, zz).
+constexpr int kCtabComps = 10;
+constexpr int kCtabSlots = 2 * kCtabComps;
+static const int kHessA[6] = {0, 0, 0, 1, 1, 2}, kHessB[6] = {0, 1, 2, 1, 2, 2};
+
+// Complex per-axis factors of one pass (FP64, f_hat's index order, N0 + N1 + N2
+// (re, im) pairs), their product being the pass's factor at (k0, k1, k2):
+//  * D (matrix_D.c:227-280) with the window scale 2^e of the plan, or the scale
+//    alone under PNFFT_OMIT_DECONV;
+//  * interlaced plans: 1/2 (assign.c:688, spread :481) and, for the shifted pass,
+//    the phase e^{+pi i k_t/n_t} (trafo) / e^{-pi i k_t/n_t} (adj) of
+//    convolution_due_to_interlacing (matrix_D.c:282-317);
+//  * ik-differentiation (ndft-parallel.c:3012-3087): -2 pi i k_t (trafo) or
+//    +2 pi i k_t (adj) on the derivative's axes (comp 1..3: axis comp-1; comp
+//    4..9: the Hessian pair, -4 pi^2 k_a k_b when both are taken).
+template <class R>
+static void ctab_fill(const PlanT<R>* P, bool shifted, bool adjoint, int comp, bool omit_d, double* out) {
+  const bool il = (P->pnfft_flags & fl::INTERLACED) != 0;
+  int da = -1, db = -1;
+  if (comp >= 1 && comp <= 3) da = comp - 1;
+  if (comp >= 4) {
+    da = kHessA[comp - 4];
+    db = kHessB[comp - 4];
+  }
+  const double pi = 3.14159265358979323846;
+  long off = 0;
+  for (int t = 0; t < 3; t++) {
+    const double e = std::ldexp(1.0, std::ilogb(1.0 / P->geom.ws[t]));
+    for (long i = 0; i < P->N[t]; i++, off++) {
+      const double k = (double)(i - P->N[t] / 2);
+      double re = omit_d ? e : P->h_tab[off], im = 0.0;
+      if (il && t == 0) re *= 0.5;
+      auto mul = [&](double cr, double ci) {
+        const double r = re * cr - im * ci;
+        im = re * ci + im * cr;
+        re = r;
+      };
+      if (shifted) {
+        const double ph = (adjoint ? -pi : pi) * k / (double)P->n[t];
+        mul(std::cos(ph), std::sin(ph));
+      }
+      const double dk = (adjoint ? 2.0 : -2.0) * pi * k;
+      if (t == da) mul(0.0, dk);
+      if (t == db) mul(0.0, dk);
+      out[2 * off] = re;
+      out[2 * off + 1] = im;
+    }
+  }
 }
 
 // Tile layout for binning: std_tile cells per axis (8 or 4, fewer on shorter axes).
@@ -323,6 +397,9 @@ static void plan_destroy(PlanT<R>* P) {
   dfree(P->d_buf1);
   dfree(P->d_buf2);
   dfree(P->d_tab);
+  dfree(P->d_tab_scale);
+  dfree(P->d_ctab);
+  if (P->h_ctab) (void)hipHostFree(P->h_ctab);
   P->evt.release();
   if (P->stream) (void)hipStreamDestroy(P->stream);
   delete static_cast<typename Handles<R>::plan*>(P);
@@ -332,6 +409,7 @@ static void plan_destroy(PlanT<R>* P) {
 template <class R>
 static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, const R* x_max, int m,
                              unsigned pnfft_flags, unsigned pfft_flags, bool c2r) {
+  clear_error();
   if (d != 3) {
     set_error("d != 3 not yet implemented");
     return nullptr;
@@ -340,11 +418,10 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
     set_error("c2r plans need an even N[2] (half spectrum N[2]/2+1)");
     return nullptr;
   }
-  const unsigned unsupported = fl::INTERLACED | fl::DIFF_IK | fl::TRANSPOSED_F_HAT | fl::WINDOW_BESSEL_I0 |
-                               fl::USE_FK_GAUSSIAN_T;
+  const unsigned unsupported = fl::TRANSPOSED_F_HAT | fl::WINDOW_BESSEL_I0 | fl::USE_FK_GAUSSIAN_T;
   if (pnfft_flags & unsupported) {
-    set_error("plan flags 0x%x (interlacing, ik-differentiation, transposed f_hat, Bessel-I0 or truncated "
-              "Gaussian window) are not supported by this build",
+    set_error("plan flags 0x%x (transposed f_hat, Bessel-I0 or truncated Gaussian window) are not supported "
+              "by this build",
               pnfft_flags & unsupported);
     return nullptr;
   }
@@ -415,7 +492,11 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
     P->d_buf1 = dalloc<Cplx<R>>(P->fft.buf1_elems());
     P->d_buf2 = dalloc<Cplx<R>>(P->fft.buf2_elems());
     P->d_tab = dalloc<double>(N[0] + N[1] + N[2]);
-    ok = P->d_fhat && P->d_grid && P->d_buf1 && P->d_buf2 && P->d_tab;
+    P->d_tab_scale = dalloc<double>(N[0] + N[1] + N[2]);
+    const size_t ct = (size_t)kCtabSlots * 2 * (N[0] + N[1] + N[2]);
+    P->d_ctab = dalloc<double>(ct);
+    if (hipHostMalloc((void**)&P->h_ctab, sizeof(double) * ct, 0) != hipSuccess) P->h_ctab = nullptr;
+    ok = P->d_fhat && P->d_grid && P->d_buf1 && P->d_buf2 && P->d_tab && P->d_tab_scale && P->d_ctab && P->h_ctab;
   }
   if (!ok) {
     (void)hipGetLastError();
@@ -437,7 +518,12 @@ static void nodes_release_device(NodesT<R>* D) {
   dfree(D->d_x);
   dfree(D->d_f);
   dfree(D->d_grad);
+  dfree(D->d_hess);
   dfree(D->d_gx);
+  dfree(D->d_tmp);
+  dfree(D->d_hess_tab);
+  dfree(D->d_hess_st);
+  D->tmp_cap = D->hess_tab_cap = D->hess_st_cap = 0;
   dfree(D->d_perm);
   dfree(D->d_keys);
   dfree(D->d_tile_start);
@@ -652,17 +738,18 @@ static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false)
 }
 
 static bool same_geom(const GridGeom& a, const GridGeom& b) {
-  return a.n[0] == b.n[0] && a.n[1] == b.n[1] && a.n[2] == b.n[2] && a.m == b.m;
+  return a.n[0] == b.n[0] && a.n[1] == b.n[1] && a.n[2] == b.n[2] && a.m == b.m && a.shift == b.shift;
 }
 
-// Bin the nodes of D for plan P (on P's stream).
+// Bin the nodes of D for plan P (on P's stream) in geometry g (the plan's, or
+// the half-cell shifted one of an interlaced pass).
 template <class R>
-static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx) {
+static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx, const GridGeom& g) {
   if (!ensure_sort_capacity(D, P->tiles)) {
     set_error("device allocation for node binning failed");
     return kHipError;
   }
-  int st = bin_nodes<R>(dx, D->M, P->geom, P->tiles, D->sw, D->d_perm, D->d_keys, D->d_tile_start, P->stream);
+  int st = bin_nodes<R>(dx, D->M, g, P->tiles, D->sw, D->d_perm, D->d_keys, D->d_tile_start, P->stream);
   if (st != kOk) {
     set_error("node binning failed (%d)", st);
     return st;
@@ -701,7 +788,7 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx) {
     // without slot memory the scatter takes the LDS-atomic path
     D->n_torder = D->d_slots ? nt : 0;
   }
-  D->bin_geom = P->geom;
+  D->bin_geom = g;
   D->bin_tiles = P->tiles;
   D->bins_valid = true;
   return kOk;
@@ -740,37 +827,199 @@ static void finish_timers(PlanT<R>* P, double* timer, bool fwd, bool had_sort, b
   }
 }
 
+template <class T>
+static T* grow(T*& p, size_t& cap, size_t need) {
+  if (cap < need) {
+    dfree(p);
+    cap = 0;
+    p = dalloc<T>(need);
+    if (p) cap = need;
+    (void)hipGetLastError();
+  }
+  return p;
+}
+
+// B stage arguments for geometry g: bins the nodes unless the bins of
+// pnfft_precompute_psi are valid for g (device-resident x), then the tile lists
+// and, with valid bins, the window tables.  false on a binning failure.
+template <class R>
+static bool b_args(PlanT<R>* P, NodesT<R>* D, R* dx, const GridGeom& g, bool host_x, bool for_scatter,
+                   bool timed, BArgs<R>& A, bool* had_sort) {
+  const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x && same_geom(D->bin_geom, g);
+  if (!reuse) {
+    if (timed) P->evt.rec(0, P->stream);
+    if (bin(P, D, dx, g) != kOk) return false;
+    if (timed) P->evt.rec(1, P->stream);
+    if (had_sort) *had_sort = true;
+  }
+  A = BArgs<R>{};
+  A.g = g;
+  A.M = D->M;
+  A.x = dx;
+  A.perm = D->d_perm;
+  A.grid = P->d_grid;
+  A.c2r = P->c2r;
+  set_tiles(A, D, for_scatter);
+  if (reuse) set_tables(A, P, D);
+  return true;
+}
+
+// scratch of the ik-differentiated transforms and the Hessian gather
+template <class R>
+static Cplx<R>* node_tmp(NodesT<R>* D) {
+  return grow(D->d_tmp, D->tmp_cap, (size_t)D->M);
+}
+
+template <class R>
+static int hessian_stage(PlanT<R>* P, NodesT<R>* D, BArgs<R> A, Cplx<R>* dh, int acc) {
+  const size_t cm = 2 * (size_t)P->m + 1, M = (size_t)D->M;
+  R* tab = grow(D->d_hess_tab, D->hess_tab_cap, 9 * cm * M);
+  int* st = grow(D->d_hess_st, D->hess_st_cap, 3 * M);
+  if (!tab || !st) {
+    set_error("device allocation for the Hessian gather failed");
+    return kHipError;
+  }
+  A.accumulate = acc;
+  A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
+  return hessian_gather<R>(A, dh, tab, st, P->stream);
+}
+
+// Complex factor tables of this call (ctab_fill) uploaded to the device:
+// slot (pass, comp) at d_ctab + (pass * kCtabComps + comp) * 2 (N0 + N1 + N2).
+template <class R>
+static const double* ctab_upload(PlanT<R>* P, bool adjoint, bool omit_d, int npass) {
+  const size_t per = 2 * (size_t)(P->N[0] + P->N[1] + P->N[2]);
+  for (int ps = 0; ps < npass; ps++)
+    for (int c = 0; c < kCtabComps; c++)
+      ctab_fill(P, ps == 1, adjoint, c, omit_d, P->h_ctab + (size_t)(ps * kCtabComps + c) * per);
+  (void)hipMemcpyAsync(P->d_ctab, P->h_ctab, sizeof(double) * per * npass * kCtabComps, hipMemcpyHostToDevice,
+                       P->stream);
+  return P->d_ctab;
+}
+
+template <class R>
+static const double* ctab_slot(const PlanT<R>* P, int pass, int comp) {
+  return P->d_ctab + (size_t)(pass * kCtabComps + comp) * 2 * (P->N[0] + P->N[1] + P->N[2]);
+}
+
+// Interlaced and/or ik-differentiated trafo (api/api-basic.c:100-244): per pass
+// (the plain nodes, then for interlaced plans the nodes shifted by half a cell,
+// kernel/ndft-parallel.c:2732-2753) F with the pass's complex factors, then B;
+// with DIFF_IK one F and an f-gather per requested component (f, grad_t,
+// Hessian (a, b)), the components landing in grad / hess through a strided add.
+// Outputs are written by the first pass and added to by the second.
+template <class R>
+static bool trafo_passes(PlanT<R>* P, NodesT<R>* D, unsigned cf, R* dx, Cplx<R>* dfh, Cplx<R>* df, Cplx<R>* dg,
+                         Cplx<R>* dh, bool host_x, bool accumulate) {
+  hipStream_t s = P->stream;
+  const bool il = P->pnfft_flags & fl::INTERLACED, ik = P->pnfft_flags & fl::DIFF_IK;
+  const bool want_f = cf & fl::COMPUTE_F, want_g = cf & fl::COMPUTE_GRAD_F, want_h = cf & fl::COMPUTE_HESSIAN_F;
+  const bool conv = !(cf & fl::OMIT_CONV) && D && D->M > 0;
+  const int npass = il ? 2 : 1;
+  const int comps = P->c2r ? 1 : 2;
+  const long M = D ? D->M : 0;
+  ctab_upload(P, false, cf & fl::OMIT_DECONV, npass);
+  auto fwd = [&](int ps, int comp) {
+    if (cf & fl::OMIT_FFT) return true;
+    if (fft3d_forward<R>(P->fft, dfh, nullptr, P->d_buf1, P->d_buf2, P->d_grid, s, P->c2r, nullptr, nullptr,
+                         ctab_slot(P, ps, comp)) != kOk) {
+      set_error("FFT launch failed");
+      return false;
+    }
+    return true;
+  };
+  for (int ps = 0; ps < npass; ps++) {
+    GridGeom g = P->geom;
+    g.shift = ps ? 0.5 : 0.0;
+    const int acc = (accumulate || ps > 0) ? 1 : 0;
+    BArgs<R> A{};
+    if (conv && !b_args(P, D, dx, g, host_x, false, false, A, nullptr)) return false;
+    A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
+    if (!ik) {
+      if (!fwd(ps, 0)) return false;
+      if (!conv) continue;
+      if (want_f || want_g) {
+        A.f = df;
+        A.grad = dg;
+        A.accumulate = acc;
+        const int mode = (want_f && want_g) ? B_FGRAD : (want_f ? B_F : B_GRAD);
+        if (gather<R>(A, mode, s) != kOk) {
+          set_error("gather launch failed");
+          return false;
+        }
+      }
+      if (want_h && hessian_stage(P, D, A, dh, acc) != kOk) return false;
+      continue;
+    }
+    Cplx<R>* tmp = conv && (want_g || want_h) ? node_tmp(D) : nullptr;
+    if (conv && (want_g || want_h) && !tmp) {
+      set_error("device allocation for ik-differentiation failed");
+      return false;
+    }
+    // one transform and f-gather per component: (comp slot, output, stride, offset)
+    auto component = [&](int comp, Cplx<R>* out, int stride, int offset) {
+      if (!fwd(ps, comp)) return false;
+      if (!conv) return true;
+      A.f = stride == 1 ? out : tmp;
+      A.grad = nullptr;
+      A.accumulate = stride == 1 ? acc : 0;
+      if (gather<R>(A, B_F, s) != kOk) {
+        set_error("gather launch failed");
+        return false;
+      }
+      if (stride > 1 && strided_put<R>(reinterpret_cast<R*>(out), reinterpret_cast<const R*>(tmp), M, comps, stride,
+                                       offset, acc, s) != kOk) {
+        set_error("strided copy launch failed");
+        return false;
+      }
+      return true;
+    };
+    if (want_f && !component(0, df, 1, 0)) return false;
+    if (want_g)
+      for (int t = 0; t < 3; t++)
+        if (!component(1 + t, dg, 3, t)) return false;
+    if (want_h)
+      for (int q = 0; q < 6; q++)
+        if (!component(4 + q, dh, 6, q)) return false;
+  }
+  return true;
+}
+
 // api/api-basic.c:199-244
 template <class R>
 static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
+  clear_error();
   if (!P) return;
   if (!D && !(cf & fl::OMIT_CONV)) return;
   auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = P->stream;
   const long M = D ? D->M : 0;
-  const bool want_f = cf & fl::COMPUTE_F, want_g = cf & fl::COMPUTE_GRAD_F;
-  if (cf & fl::COMPUTE_HESSIAN_F) set_error("PNFFT_COMPUTE_HESSIAN_F is not supported by this build");
+  const bool want_f = cf & fl::COMPUTE_F, want_g = cf & fl::COMPUTE_GRAD_F, want_h = cf & fl::COMPUTE_HESSIAN_F;
   const bool accumulate = (cf & fl::COMPUTE_ACCUMULATED) != 0;
-  bool host_x = false, host_f = false, host_g = false, host_fh = false;
+  bool host_x = false, host_f = false, host_g = false, host_h = false, host_fh = false;
 
-  // c2r plans keep f and grad_f as reals (kernel/ndft-parallel.c:2826-2847)
+  // c2r plans keep f, grad_f and hessian_f as reals (kernel/ndft-parallel.c:2826-2847)
   const size_t vsz = P->c2r ? sizeof(R) : sizeof(Cplx<R>);
   R* dx = D ? stage_in<R>(D->x, D->d_x, 3 * M, true, s, &host_x) : nullptr;
   Cplx<R>* dfh = stage_in<Cplx<R>>(P->f_hat, P->d_fhat, P->N_total, true, s, &host_fh,
                                    sizeof(Cplx<R>) * P->fhat_elems);
   const bool direct = cf & fl::COMPUTE_DIRECT;
   // outputs: uploaded only when accumulating (the NDFT overwrites them regardless, :400-414)
+  const bool up = accumulate && !direct;
   Cplx<R>* df = nullptr;
   Cplx<R>* dg = nullptr;
-  if (D && want_f) df = stage_in<Cplx<R>>(D->f, D->d_f, M, accumulate && !direct, s, &host_f, vsz * M);
-  if (D && want_g)
-    dg = stage_in<Cplx<R>>(D->grad_f, D->d_grad, 3 * M, accumulate && !direct, s, &host_g, vsz * 3 * M);
-  if ((want_f && M > 0 && !df) || (want_g && M > 0 && !dg) || (M > 0 && !dx && !(cf & fl::OMIT_CONV)) || !dfh) {
+  Cplx<R>* dh = nullptr;
+  if (D && want_f) df = stage_in<Cplx<R>>(D->f, D->d_f, M, up, s, &host_f, vsz * M);
+  if (D && want_g) dg = stage_in<Cplx<R>>(D->grad_f, D->d_grad, 3 * M, up, s, &host_g, vsz * 3 * M);
+  if (D && want_h) dh = stage_in<Cplx<R>>(D->hessian_f, D->d_hess, 6 * M, up, s, &host_h, vsz * 6 * M);
+  if ((want_f && M > 0 && !df) || (want_g && M > 0 && !dg) || (want_h && M > 0 && !dh) ||
+      (M > 0 && !dx && !(cf & fl::OMIT_CONV)) || !dfh) {
     set_error("trafo: missing node or f_hat arrays");
     return;
   }
 
   bool had_sort = false, had_f = false, had_b = false;
+  const bool general = P->pnfft_flags & (fl::INTERLACED | fl::DIFF_IK);
   if (direct) {
     NdftArgs<R> A{};
     for (int t = 0; t < 3; t++) A.N[t] = (int)P->N[t];
@@ -779,15 +1028,19 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     A.fhat = dfh;
     A.f = df;
     A.grad = dg;
+    A.hess = dh;
     A.want_f = want_f;
     A.want_grad = want_g;
+    A.want_hess = want_h;
     A.c2r = P->c2r;
     if (ndft_trafo<R>(A, s) != kOk) set_error("NDFT trafo launch failed");
+  } else if (general) {
+    if (!trafo_passes(P, D, cf, dx, dfh, df, dg, dh, host_x, accumulate)) return;
   } else {
     // D and F (fused): grid <- F D f_hat
     if (!(cf & fl::OMIT_FFT)) {
       P->evt.rec(2, s);
-      const double* tab = (cf & fl::OMIT_DECONV) ? nullptr : P->d_tab;
+      const double* tab = (cf & fl::OMIT_DECONV) ? P->d_tab_scale : P->d_tab;
       if (fft3d_forward<R>(P->fft, dfh, tab, P->d_buf1, P->d_buf2, P->d_grid, s, P->c2r, P->evt.mark(8),
                            P->evt.mark(9)) != kOk)
         set_error("FFT launch failed");
@@ -795,39 +1048,31 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
       had_f = true;
     }
     // B
-    if (!(cf & fl::OMIT_CONV) && M > 0 && (want_f || want_g)) {
-      const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x &&
-                         same_geom(D->bin_geom, P->geom);
-      if (!reuse) {
-        P->evt.rec(0, s);
-        if (bin(P, D, dx) != kOk) return;
-        P->evt.rec(1, s);
-        had_sort = true;
-      }
+    if (!(cf & fl::OMIT_CONV) && M > 0 && (want_f || want_g || want_h)) {
       BArgs<R> A{};
-      A.g = P->geom;
-      A.M = M;
-      A.x = dx;
-      A.perm = D->d_perm;
-      A.grid = P->d_grid;
+      if (!b_args(P, D, dx, P->geom, host_x, false, true, A, &had_sort)) return;
       A.f = df;
       A.grad = dg;
       A.accumulate = accumulate;
       A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
-      A.c2r = P->c2r;
-      set_tiles(A, D);
-      if (reuse) set_tables(A, P, D);
       A.mid = P->evt.mark(10);
       A.mid2 = P->evt.mark(13);
-      int mode = (want_f && want_g) ? B_FGRAD : (want_f ? B_F : B_GRAD);
       P->evt.rec(4, s);
-      if (gather<R>(A, mode, s) != kOk) set_error("gather launch failed");
+      if (want_f || want_g) {
+        const int mode = (want_f && want_g) ? B_FGRAD : (want_f ? B_F : B_GRAD);
+        if (gather<R>(A, mode, s) != kOk) set_error("gather launch failed");
+      } else {
+        P->evt.rec(10, s);
+        P->evt.rec(13, s);
+      }
       P->evt.rec(5, s);
       had_b = true;
+      if (want_h && hessian_stage(P, D, A, dh, accumulate) != kOk) return;
     }
   }
   if (host_f) (void)hipMemcpyAsync(D->f, df, vsz * M, hipMemcpyDeviceToHost, s);
   if (host_g) (void)hipMemcpyAsync(D->grad_f, dg, vsz * 3 * M, hipMemcpyDeviceToHost, s);
+  if (host_h) (void)hipMemcpyAsync(D->hessian_f, dh, vsz * 6 * M, hipMemcpyDeviceToHost, s);
   if (hipStreamSynchronize(s) != hipSuccess) {
     set_error("trafo: HIP error %s", hipGetErrorString(hipGetLastError()));
     return;
@@ -837,9 +1082,77 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
   P->timer_trafo[T_WHOLE] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Interlaced and/or ik-differentiated adjoint (api/api-basic.c:247-378): per
+// pass, B^T of the node values and F^H with the pass's complex factors (D^T,
+// the conjugate interlacing phase, +2 pi i k_t for gradient components), each
+// transform adding onto f_hat.
+template <class R>
+static bool adj_passes(PlanT<R>* P, NodesT<R>* D, unsigned cf, R* dx, Cplx<R>* dfh, Cplx<R>* df, Cplx<R>* dg,
+                       bool host_x) {
+  hipStream_t s = P->stream;
+  const bool il = P->pnfft_flags & fl::INTERLACED, ik = P->pnfft_flags & fl::DIFF_IK;
+  const bool use_f = cf & fl::COMPUTE_F, use_g = cf & fl::COMPUTE_GRAD_F;
+  const bool conv = !(cf & fl::OMIT_CONV) && D && D->M > 0;
+  const bool back = !(cf & fl::OMIT_FFT) && !(cf & fl::OMIT_DECONV);
+  const int npass = il ? 2 : 1;
+  const int comps = P->c2r ? 1 : 2;
+  const long M = D ? D->M : 0;
+  ctab_upload(P, true, false, npass);
+  for (int ps = 0; ps < npass; ps++) {
+    GridGeom g = P->geom;
+    g.shift = ps ? 0.5 : 0.0;
+    BArgs<R> A{};
+    if (conv && (use_f || use_g) && !b_args(P, D, dx, g, host_x, true, false, A, nullptr)) return false;
+    // grid <- B^T (values) for one mode; then f_hat += factors(comp) F^H grid
+    auto stage = [&](int mode, Cplx<R>* f, Cplx<R>* gr, int comp) {
+      if (!(cf & fl::OMIT_CONV)) {
+        if (conv) {
+          A.f = f;
+          A.grad = gr;
+          if (scatter<R>(A, mode, s) != kOk) {
+            set_error("scatter launch failed");
+            return false;
+          }
+        } else {
+          fill_zero<R>(P->d_grid, P->n_total, s);
+        }
+      }
+      if (back && fft3d_backward<R>(P->fft, P->d_grid, nullptr, P->d_buf1, P->d_buf2, dfh, 1, s, P->c2r, nullptr,
+                                    nullptr, ctab_slot(P, ps, comp)) != kOk) {
+        set_error("FFT launch failed");
+        return false;
+      }
+      return true;
+    };
+    if (!ik) {
+      if (use_f || use_g) {
+        if (!stage((use_f && use_g) ? B_FGRAD : (use_f ? B_F : B_GRAD), df, dg, 0)) return false;
+      }
+      continue;
+    }
+    if (use_f && !stage(B_F, df, nullptr, 0)) return false;
+    if (use_g) {
+      Cplx<R>* tmp = conv ? node_tmp(D) : nullptr;
+      if (conv && !tmp) {
+        set_error("device allocation for ik-differentiation failed");
+        return false;
+      }
+      for (int t = 0; t < 3; t++) {
+        if (conv && strided_get<R>(reinterpret_cast<R*>(tmp), reinterpret_cast<const R*>(dg), M, comps, 3, t, s) != kOk) {
+          set_error("strided copy launch failed");
+          return false;
+        }
+        if (!stage(B_F, tmp, nullptr, 1 + t)) return false;
+      }
+    }
+  }
+  return true;
+}
+
 // api/api-basic.c:344-378
 template <class R>
 static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
+  clear_error();
   if (!P) return;
   if (!D && !(cf & fl::OMIT_CONV)) return;
   auto t0 = std::chrono::steady_clock::now();
@@ -875,30 +1188,17 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     A.want_grad = use_g;
     A.c2r = P->c2r;
     if (ndft_adj<R>(A, s) != kOk) set_error("NDFT adj launch failed");
+  } else if (P->pnfft_flags & (fl::INTERLACED | fl::DIFF_IK)) {
+    if (!adj_passes(P, D, cf, dx, dfh, df, dg, host_x)) return;
   } else {
     if (!(cf & fl::OMIT_CONV)) {
       // grid = B^T (f, grad): the reference scatters into the zeroed grid (:2629-2635);
       // scatter() writes every cell itself
       if (M > 0 && (use_f || use_g)) {
-        const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x &&
-                           same_geom(D->bin_geom, P->geom);
-        if (!reuse) {
-          P->evt.rec(0, s);
-          if (bin(P, D, dx) != kOk) return;
-          P->evt.rec(1, s);
-          had_sort = true;
-        }
         BArgs<R> A{};
-        A.g = P->geom;
-        A.M = M;
-        A.x = dx;
-        A.perm = D->d_perm;
-        A.grid = P->d_grid;
+        if (!b_args(P, D, dx, P->geom, host_x, true, true, A, &had_sort)) return;
         A.f = df;
         A.grad = dg;
-        A.c2r = P->c2r;
-        set_tiles(A, D, true);
-        if (reuse) set_tables(A, P, D);
         A.mid = P->evt.mark(10);
         A.mid2 = P->evt.mark(13);
         const int mode = (use_f && use_g) ? B_FGRAD : (use_f ? B_F : B_GRAD);
@@ -933,7 +1233,12 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
 // x-adjoint of f = trafo(f_hat): grad_x[3j+t] = Re(conj(w_j) df_j/dx_t)
 template <class R>
 static void vjp_impl(PlanT<R>* P, NodesT<R>* D, const Cplx<R>* w, R* gx) {
+  clear_error();
   if (!P || !D) return;
+  if (P->pnfft_flags & fl::INTERLACED) {
+    set_error("trafo_grad_x_vjp: interlaced plans are not supported");
+    return;
+  }
   hipStream_t s = P->stream;
   const long M = D->M;
   if (M == 0) return;
@@ -954,7 +1259,7 @@ static void vjp_impl(PlanT<R>* P, NodesT<R>* D, const Cplx<R>* w, R* gx) {
   }
   const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x &&
                      same_geom(D->bin_geom, P->geom);
-  if (!reuse && bin(P, D, dx) != kOk) return;
+  if (!reuse && bin(P, D, dx, P->geom) != kOk) return;
   BArgs<R> A{};
   A.g = P->geom;
   A.M = M;
@@ -974,20 +1279,9 @@ static void vjp_impl(PlanT<R>* P, NodesT<R>* D, const Cplx<R>* w, R* gx) {
 static bool same_full_geom(const GridGeom& a, const GridGeom& b) {
   for (int t = 0; t < 3; t++)
     if (a.n[t] != b.n[t] || a.b[t] != b.b[t] || a.ws[t] != b.ws[t]) return false;
-  return a.m == b.m && a.win == b.win;
+  return a.m == b.m && a.win == b.win && a.shift == b.shift;
 }
 
-template <class T>
-static T* grow(T*& p, size_t& cap, size_t need) {
-  if (cap < need) {
-    dfree(p);
-    cap = 0;
-    p = dalloc<T>(need);
-    if (p) cap = need;
-    (void)hipGetLastError();
-  }
-  return p;
-}
 
 // kernel/ndft-parallel.c:1144-1245: bins the nodes and, for PNFFT_PRE_PSI,
 // tabulates their window factors (PNFFT_PRE_GRAD_PSI: also the derivatives) in
@@ -995,6 +1289,7 @@ static T* grow(T*& p, size_t& cap, size_t need) {
 // B kernels evaluate the windows as without PRE_PSI.
 template <class R>
 static void precompute_impl(PlanT<R>* P, NodesT<R>* D, unsigned flags) {
+  clear_error();
   if (!P || !D) return;
   D->precompute_flags = flags;
   D->pre_valid = false;
@@ -1002,7 +1297,7 @@ static void precompute_impl(PlanT<R>* P, NodesT<R>* D, unsigned flags) {
   bool host_x = false;
   R* dx = stage_in<R>(D->x, D->d_x, 3 * D->M, true, P->stream, &host_x);
   if (!dx) return;
-  if (bin(P, D, dx) != kOk) return;
+  if (bin(P, D, dx, P->geom) != kOk) return;
   if (!host_x && P->m >= 2 && P->m <= 8 && tables_enabled()) {
     const size_t M = (size_t)D->M, cm = 2 * (size_t)P->m + 1;
     const bool want_grad = (flags & fl::PRE_GRAD_PSI) != 0;
@@ -1030,7 +1325,8 @@ static void precompute_impl(PlanT<R>* P, NodesT<R>* D, unsigned flags) {
 // window tables for a B stage that reuses the precomputed bins
 template <class R>
 static void set_tables(BArgs<R>& A, const PlanT<R>* P, const NodesT<R>* D) {
-  if (!D->pre_valid || !same_full_geom(D->pre_geom, P->geom)) return;
+  (void)P;
+  if (!D->pre_valid || !same_full_geom(D->pre_geom, A.g)) return;
   A.pre_psi = D->d_pre_psi;
   A.pre_dpsi = D->pre_grad ? D->d_pre_dpsi : nullptr;
   A.pre_x = D->d_pre_x;
@@ -1038,13 +1334,14 @@ static void set_tables(BArgs<R>& A, const PlanT<R>* P, const NodesT<R>* D) {
 
 template <class R>
 static int node_bins_impl(PlanT<R>* P, NodesT<R>* D, int* order, unsigned* keys) {
+  clear_error();
   if (!P || !D) return -1;
   if (D->M == 0) return 0;
   bool host_x = false;
   R* dx = stage_in<R>(D->x, D->d_x, 3 * D->M, true, P->stream, &host_x);
   if (!dx) return -1;
   if (!(D->bins_valid && !host_x && same_geom(D->bin_geom, P->geom)))
-    if (bin(P, D, dx) != kOk) return -1;
+    if (bin(P, D, dx, P->geom) != kOk) return -1;
   if (order) (void)hipMemcpyAsync(order, D->d_perm, sizeof(int) * D->M, hipMemcpyDefault, P->stream);
   if (keys) (void)hipMemcpyAsync(keys, D->d_keys, sizeof(unsigned) * D->M, hipMemcpyDefault, P->stream);
   if (hipStreamSynchronize(P->stream) != hipSuccess) return -1;
@@ -1056,6 +1353,7 @@ static int node_bins_impl(PlanT<R>* P, NodesT<R>* D, int* order, unsigned* keys)
 // caller; the sort runs in the library's static scratch (bin_nodes_noalloc).
 template <class R>
 static int node_bins_noalloc_impl(PlanT<R>* P, const R* x, long M, int* order, unsigned* keys) {
+  clear_error();
   if (!P) return -1;
   if (M < 0 || M > kNoAllocMaxNodes) {
     set_error("node_bins_noalloc: M=%ld outside [0, %ld] (the static scratch holds %ld nodes)", M, kNoAllocMaxNodes,

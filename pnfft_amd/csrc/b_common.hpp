@@ -15,10 +15,11 @@ struct NodeAxis {
 
 // n is a power of two (plans reject other oversampled lengths): floor(n x) mod n
 // is fl - n floor(fl / n), exact in FP64 for |n x| < 2^53 (the scaling by 1/n is
-// an exponent shift), and the shift by -m wraps with a mask.
+// an exponent shift), and the shift by -m wraps with a mask.  sh (0 or 1/2, the
+// interlaced pass) is added to n x in FP64, where both are exact.
 template <class R>
-__device__ inline NodeAxis<R> node_axis(R xv, int n, int m) {
-  const double nx = (double)n * (double)xv;
+__device__ inline NodeAxis<R> node_axis(R xv, int n, int m, double sh) {
+  const double nx = (double)n * (double)xv + sh;
   const double fl = floor(nx);
   NodeAxis<R> a;
   a.frac = (R)(nx - fl);

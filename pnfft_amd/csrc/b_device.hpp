@@ -59,9 +59,9 @@ template <class R, int MT, int CM, bool WG>
 __device__ inline void node_windows(const BArgs<R>& A, long p, long j, int m, NodeWin<R, CM>& w) {
   const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
   const R x0 = A.x[3 * j + 0], x1 = A.x[3 * j + 1], x2 = A.x[3 * j + 2];
-  const NodeAxis<R> a0 = node_axis<R>(x0, n0, m);
-  const NodeAxis<R> a1 = node_axis<R>(x1, n1, m);
-  const NodeAxis<R> a2 = node_axis<R>(x2, n2, m);
+  const NodeAxis<R> a0 = node_axis<R>(x0, n0, m, A.g.shift);
+  const NodeAxis<R> a1 = node_axis<R>(x1, n1, m, A.g.shift);
+  const NodeAxis<R> a2 = node_axis<R>(x2, n2, m, A.g.shift);
   w.start[0] = a0.start;
   w.start[1] = a1.start;
   w.start[2] = a2.start;
@@ -241,6 +241,11 @@ __device__ inline void gather_store(const BArgs<R>& A, long j, Cplx<R>* v) {
   }
   if (MODE == B_GRAD || MODE == B_FGRAD) {
     Cplx<R>* gp = A.grad + 3 * j;
+    if (A.real_only) {  // assign_grad_f_r2r writes the real parts only (kernel/assign.c:821-851)
+      v[1].y = 0;
+      v[2].y = 0;
+      v[3].y = 0;
+    }
     if (A.accumulate) {
       v[1] = cadd(v[1], gp[0]);
       v[2] = cadd(v[2], gp[1]);

@@ -462,7 +462,7 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
     int l = 0;
     bool bad = false;
     if (lk < 3 && jg >= 0) {
-      const NodeAxis<float> na = node_axis<float>(L.x, nt, MT);
+      const NodeAxis<float> na = node_axis<float>(L.x, nt, MT, A.g.shift);
       l = local_off(na.start, ot, TS, nt);
       bad = l < 0;
       if (tab && L.x == L.px) {

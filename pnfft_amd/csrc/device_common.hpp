@@ -41,6 +41,10 @@ struct GridGeom {
   // kernels (2^-e with e the exponent of the window's peak) and undone in the
   // D factors, so that FP32 intermediates stay near 1; exact in both precisions
   double ws[3];
+  // added to n_t x_t before the stencil start and the window argument are taken:
+  // 0, or 1/2 for the second pass of an interlaced plan (nodes shifted by half a
+  // cell, kernel/ndft-parallel.c:2732-2753)
+  double shift;
 };
 
 // Launch helpers report errors through this code (0 = ok).

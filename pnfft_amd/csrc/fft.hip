@@ -53,6 +53,7 @@ struct PassArgs {
   const double* tab1;
   const double* tab2;
   int N1;
+  int ctab;                  // tab0..2 hold complex factors (re, im pairs) per frequency index
   int accumulate;            // backward store: out += value (D^T onto f_hat)
   // frequency <-> index map of the pruned side: index e holds frequency k = e - h;
   // forward loads take k in [-h, pos); the c2r half spectrum is h = N2/2, pos = 1
@@ -215,6 +216,24 @@ __device__ inline int load_index(const PassArgs<R>& P, int p, int L) {
   return P.backward ? p : pruned_src(p, L, P.h, P.pos);
 }
 
+// w times the product of the three per-axis factors of frequency (i0, i1, e),
+// formed and applied in FP64: real factors, or complex ones (P.ctab) for the
+// phases and derivative factors of interlaced and ik-differentiated plans
+template <class R>
+__device__ inline Cplx<R> apply_factors(const PassArgs<R>& P, Cplx<R> w, int i0, int i1, int e) {
+  if (!P.ctab) {
+    const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
+    return cmk<R>(R((double)w.x * f), R((double)w.y * f));
+  }
+  const double ar = P.tab0[2 * i0], ai = P.tab0[2 * i0 + 1];
+  const double br = P.tab1[2 * i1], bi = P.tab1[2 * i1 + 1];
+  const double cr = P.tab2[2 * e], ci = P.tab2[2 * e + 1];
+  const double abr = ar * br - ai * bi, abi = ar * bi + ai * br;
+  const double fr = abr * cr - abi * ci, fi = abr * ci + abi * cr;
+  const double wr = (double)w.x, wi = (double)w.y;
+  return cmk<R>(R(wr * fr - wi * fi), R(wr * fi + wi * fr));
+}
+
 // value as it enters the transform: D factor and c2r weight (forward rows),
 // conjugation (backward)
 template <class R>
@@ -222,9 +241,7 @@ __device__ inline Cplx<R> on_load(const PassArgs<R>& P, Cplx<R> w, int col, int 
   const bool fwd = !P.backward;
   if (fwd && P.tab0 != nullptr) {
     // row = i0*N1 + i1 (rows pass over f_hat), e = i2
-    const int i0 = col / P.N1, i1 = col % P.N1;
-    const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
-    w = cmk<R>(R((double)w.x * f), R((double)w.y * f));
+    w = apply_factors(P, w, col / P.N1, col % P.N1, e);
   }
   if (P.half && fwd && e < P.h) w = cmk<R>(R(2) * w.x, R(2) * w.y);
   if (!fwd) w = cconj(w);
@@ -236,11 +253,7 @@ template <class R>
 __device__ inline void on_store(const PassArgs<R>& P, Cplx<R> w, int col, int e, long addr) {
   if (P.backward) {
     w = cconj(w);
-    if (P.tab0 != nullptr) {
-      const int i0 = col / P.N1, i1 = col % P.N1;
-      const double f = P.tab0[i0] * P.tab1[i1] * P.tab2[e];
-      w = cmk<R>(R((double)w.x * f), R((double)w.y * f));
-    }
+    if (P.tab0 != nullptr) w = apply_factors(P, w, col / P.N1, col % P.N1, e);
   }
   if (P.accumulate) w = cadd(P.out[addr], w);
   P.out[addr] = w;
@@ -584,9 +597,27 @@ static void launch_pass(bool rows, dim3 grid, const PassArgs<R>& P, hipStream_t 
   });
 }
 
+// per-axis factors of the rows pass: complex ones (ctab, interleaved pairs)
+// take precedence over the real D factors (tab)
+template <class R>
+static void set_factors(PassArgs<R>& P, const double* tab, const double* ctab, int N0, int N1) {
+  if (ctab) {
+    P.tab0 = ctab;
+    P.tab1 = ctab + 2 * N0;
+    P.tab2 = ctab + 2 * (N0 + N1);
+    P.ctab = 1;
+  } else if (tab) {
+    P.tab0 = tab;
+    P.tab1 = tab + N0;
+    P.tab2 = tab + N0 + N1;
+  }
+  P.N1 = N1;
+}
+
 template <class R>
 int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
-                  Cplx<R>* grid, hipStream_t s, bool half, hipEvent_t mark1, hipEvent_t mark2) {
+                  Cplx<R>* grid, hipStream_t s, bool half, hipEvent_t mark1, hipEvent_t mark2,
+                  const double* ctab) {
   const int N0 = F.N[0], N1 = F.N[1], N2 = F.N[2];
   const int n0 = F.n[0], n1 = F.n[1], n2 = F.n[2];
   const int N2in = half ? N2 / 2 + 1 : N2;  // stored length of f_hat rows
@@ -601,8 +632,7 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.h = N2 / 2;
     P.pos = half ? 1 : N2 - N2 / 2;
     P.half = half ? 1 : 0;
-    if (tab) { P.tab0 = tab; P.tab1 = tab + N0; P.tab2 = tab + N0 + N1; }
-    P.N1 = N1;
+    set_factors(P, tab, ctab, N0, N1);
     dim3 grid_dim((P.n_inner + P.C - 1) / P.C, 1);
     launch_pass<R>(true, grid_dim, P, s);
     if (mark1) (void)hipEventRecord(mark1, s);
@@ -640,7 +670,7 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
 template <class R>
 int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
                    Cplx<R>* fhat, int accumulate, hipStream_t s, bool half, hipEvent_t mark1,
-                   hipEvent_t mark2) {
+                   hipEvent_t mark2, const double* ctab) {
   const int N0 = F.N[0], N1 = F.N[1], N2 = F.N[2];
   const int n0 = F.n[0], n1 = F.n[1], n2 = F.n[2];
   const int N2out = half ? N2 / 2 + 1 : N2;
@@ -682,8 +712,7 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.Lin = n2; P.Lout = N2out; P.h = N2 / 2;
     P.backward = 1;
     P.accumulate = accumulate;
-    if (tab) { P.tab0 = tab; P.tab1 = tab + N0; P.tab2 = tab + N0 + N1; }
-    P.N1 = N1;
+    set_factors(P, tab, ctab, N0, N1);
     dim3 grid_dim((P.n_inner + P.C - 1) / P.C, 1);
     launch_pass<R>(true, grid_dim, P, s);
   }
@@ -716,12 +745,16 @@ template struct FftAxis<double>;
 template struct Fft3d<float>;
 template struct Fft3d<double>;
 template int fft3d_forward<float>(const Fft3d<float>&, const Cplx<float>*, const double*, Cplx<float>*,
-                                  Cplx<float>*, Cplx<float>*, hipStream_t, bool, hipEvent_t, hipEvent_t);
+                                  Cplx<float>*, Cplx<float>*, hipStream_t, bool, hipEvent_t, hipEvent_t,
+                                  const double*);
 template int fft3d_forward<double>(const Fft3d<double>&, const Cplx<double>*, const double*, Cplx<double>*,
-                                   Cplx<double>*, Cplx<double>*, hipStream_t, bool, hipEvent_t, hipEvent_t);
+                                   Cplx<double>*, Cplx<double>*, hipStream_t, bool, hipEvent_t, hipEvent_t,
+                                   const double*);
 template int fft3d_backward<float>(const Fft3d<float>&, const Cplx<float>*, const double*, Cplx<float>*,
-                                   Cplx<float>*, Cplx<float>*, int, hipStream_t, bool, hipEvent_t, hipEvent_t);
+                                   Cplx<float>*, Cplx<float>*, int, hipStream_t, bool, hipEvent_t, hipEvent_t,
+                                   const double*);
 template int fft3d_backward<double>(const Fft3d<double>&, const Cplx<double>*, const double*, Cplx<double>*,
-                                    Cplx<double>*, Cplx<double>*, int, hipStream_t, bool, hipEvent_t, hipEvent_t);
+                                    Cplx<double>*, Cplx<double>*, int, hipStream_t, bool, hipEvent_t, hipEvent_t,
+                                    const double*);
 
 }  // namespace pnfft_amd

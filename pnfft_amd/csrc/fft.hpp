@@ -40,15 +40,18 @@ struct Fft3d {
 // mark1 / mark2 (optional) are recorded on s after the first and second pass.
 // half: fhat holds the c2r half spectrum k2 in [-N2/2, 0]; the grid's real part is then the
 // c2r transform (the k2 < 0 coefficients count twice, the k2 = 0 plane once).
+// ctab (optional, overrides tab): complex per-axis factors, [N0 + N1 + N2] (re, im) FP64
+// pairs in f_hat's index order; the value at (i0, i1, i2) is multiplied by their product.
 template <class R>
 int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
                   Cplx<R>* grid, hipStream_t s, bool half = false, hipEvent_t mark1 = nullptr,
-                  hipEvent_t mark2 = nullptr);
+                  hipEvent_t mark2 = nullptr, const double* ctab = nullptr);
 
 // fhat (+)= D^T F^H grid; half: only k2 in [-N2/2, 0] is produced (r2c of a real grid).
+// ctab as for fft3d_forward, applied after the transform.
 template <class R>
 int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cplx<R>* buf1, Cplx<R>* buf2,
                    Cplx<R>* fhat, int accumulate, hipStream_t s, bool half = false, hipEvent_t mark1 = nullptr,
-                   hipEvent_t mark2 = nullptr);
+                   hipEvent_t mark2 = nullptr, const double* ctab = nullptr);
 
 }  // namespace pnfft_amd

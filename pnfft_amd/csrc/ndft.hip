@@ -156,15 +156,110 @@ __global__ __launch_bounds__(256) void ndft_adj_c2r_k(NdftArgs<R> A) {
   A.fhat[kidx] = cadd(A.fhat[kidx], acc);
 }
 
+// Hessian NDFT (:450-510, scaled by -4 pi^2 at :604-612): hess[6j + q] =
+// -4 pi^2 sum_k k_a k_b f_hat_k e^{-2 pi i k x_j}, (a, b) = xx, xy, xz, yy, yz, zz;
+// c2c nested per axis as ndft_trafo_k, c2r over the half spectrum with weight 2
+// and the skips of is_hermitian.  Overwrites hess.
+template <class R>
+__global__ __launch_bounds__(256) void ndft_hess_k(NdftArgs<R> A) {
+  const long j = (long)blockIdx.x * 256 + threadIdx.x;
+  if (j >= A.M) return;
+  const int N0 = A.N[0], N1 = A.N[1], N2 = A.N[2];
+  const R twopi = R(6.28318530717958647692);
+  const R x0 = A.x[3 * j], x1 = A.x[3 * j + 1], x2 = A.x[3 * j + 2];
+  // level sums: s2[e] = sum_i2 k2^e p, then per i1 and i0 the products with k1, k0
+  Cplx<R> h[6];
+  for (int q = 0; q < 6; q++) h[q] = cmk<R>(0, 0);
+  for (int i0 = 0; i0 < N0; i0++) {
+    const R k0 = R(i0 - N0 / 2);
+    // t[0] = sum e1 s2_0, t[1] = sum k1 e1 s2_0, t[2] = sum k1^2 e1 s2_0,
+    // t[3] = sum e1 s2_1, t[4] = sum k1 e1 s2_1, t[5] = sum e1 s2_2
+    Cplx<R> t[6];
+    for (int q = 0; q < 6; q++) t[q] = cmk<R>(0, 0);
+    for (int i1 = 0; i1 < N1; i1++) {
+      const R k1 = R(i1 - N1 / 2);
+      const Cplx<R>* fh = A.fhat + ((long)i0 * N1 + i1) * N2;
+      Cplx<R> u0 = cmk<R>(0, 0), u1 = cmk<R>(0, 0), u2 = cmk<R>(0, 0);
+      for (int i2 = 0; i2 < N2; i2++) {
+        const R k2 = R(i2 - N2 / 2);
+        R sn, cs;
+        sc(-twopi * k2 * x2, &sn, &cs);
+        const Cplx<R> pv = cmul(fh[i2], cmk<R>(cs, sn));
+        u0 = cadd(u0, pv);
+        u1 = cadd(u1, cscale(pv, k2));
+        u2 = cadd(u2, cscale(pv, k2 * k2));
+      }
+      R sn, cs;
+      sc(-twopi * k1 * x1, &sn, &cs);
+      const Cplx<R> e1 = cmk<R>(cs, sn);
+      const Cplx<R> v0 = cmul(u0, e1), v1 = cmul(u1, e1), v2 = cmul(u2, e1);
+      t[0] = cadd(t[0], v0);
+      t[1] = cadd(t[1], cscale(v0, k1));
+      t[2] = cadd(t[2], cscale(v0, k1 * k1));
+      t[3] = cadd(t[3], v1);
+      t[4] = cadd(t[4], cscale(v1, k1));
+      t[5] = cadd(t[5], v2);
+    }
+    R sn, cs;
+    sc(-twopi * k0 * x0, &sn, &cs);
+    const Cplx<R> e0 = cmk<R>(cs, sn);
+    h[0] = cadd(h[0], cscale(cmul(t[0], e0), k0 * k0));  // xx
+    h[1] = cadd(h[1], cscale(cmul(t[1], e0), k0));       // xy
+    h[2] = cadd(h[2], cscale(cmul(t[3], e0), k0));       // xz
+    h[3] = cadd(h[3], cmul(t[2], e0));                   // yy
+    h[4] = cadd(h[4], cmul(t[4], e0));                   // yz
+    h[5] = cadd(h[5], cmul(t[5], e0));                   // zz
+  }
+  const R c = -twopi * twopi;
+  for (int q = 0; q < 6; q++) A.hess[6 * j + q] = cscale(h[q], c);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void ndft_hess_c2r_k(NdftArgs<R> A) {
+  const long j = (long)blockIdx.x * 256 + threadIdx.x;
+  if (j >= A.M) return;
+  const int N0 = A.N[0], N1 = A.N[1], N2 = A.N[2], H2 = N2 / 2 + 1;
+  const R twopi = R(6.28318530717958647692);
+  const R x0 = A.x[3 * j], x1 = A.x[3 * j + 1], x2 = A.x[3 * j + 2];
+  R h[6] = {0, 0, 0, 0, 0, 0};
+  for (int i0 = 0; i0 < N0; i0++)
+    for (int i1 = 0; i1 < N1; i1++)
+      for (int i2 = 0; i2 < H2; i2++) {
+        const int k0 = i0 - N0 / 2, k1 = i1 - N1 / 2, k2 = i2 - N2 / 2;
+        if ((k0 == 0 && k1 == 0 && k2 == 0) || is_hermitian(k0, k1, k2, N0, N1, N2)) continue;
+        const Cplx<R> cf = A.fhat[((long)i0 * N1 + i1) * H2 + i2];
+        R sn, cs;
+        sc(-twopi * (R(k0) * x0 + R(k1) * x1 + R(k2) * x2), &sn, &cs);
+        const R re = R(2) * cmul(cf, cmk<R>(cs, sn)).x;
+        h[0] += R(k0 * k0) * re;
+        h[1] += R(k0 * k1) * re;
+        h[2] += R(k0 * k2) * re;
+        h[3] += R(k1 * k1) * re;
+        h[4] += R(k1 * k2) * re;
+        h[5] += R(k2 * k2) * re;
+      }
+  R* hr = reinterpret_cast<R*>(A.hess);
+  for (int q = 0; q < 6; q++) hr[6 * j + q] = -twopi * twopi * h[q];
+}
+
 }  // namespace
 
 template <class R>
 int ndft_trafo(const NdftArgs<R>& A, hipStream_t s) {
   if (A.M == 0) return kOk;
-  if (A.c2r)
-    hipLaunchKernelGGL((ndft_trafo_c2r_k<R>), dim3((unsigned)((A.M + 255) / 256)), dim3(256), 0, s, A);
-  else
-    hipLaunchKernelGGL((ndft_trafo_k<R>), dim3((unsigned)((A.M + 255) / 256)), dim3(256), 0, s, A);
+  const dim3 grid((unsigned)((A.M + 255) / 256)), block(256);
+  if (A.want_f || A.want_grad) {
+    if (A.c2r)
+      hipLaunchKernelGGL((ndft_trafo_c2r_k<R>), grid, block, 0, s, A);
+    else
+      hipLaunchKernelGGL((ndft_trafo_k<R>), grid, block, 0, s, A);
+  }
+  if (A.want_hess) {
+    if (A.c2r)
+      hipLaunchKernelGGL((ndft_hess_c2r_k<R>), grid, block, 0, s, A);
+    else
+      hipLaunchKernelGGL((ndft_hess_k<R>), grid, block, 0, s, A);
+  }
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
 

@@ -15,8 +15,10 @@ struct NdftArgs {
   Cplx<R>* fhat;
   Cplx<R>* f;
   Cplx<R>* grad;
+  Cplx<R>* hess;  // C[6M] (R[6M] for c2r), components xx, xy, xz, yy, yz, zz
   int want_f;
   int want_grad;
+  int want_hess;
   int c2r;  // real plan: f_hat is the half spectrum k2 in [-N2/2, 0]; f, grad are real arrays
 };
 

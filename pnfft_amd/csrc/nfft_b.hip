@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kB) void bin_keys_k(const R* __restrict__ x, long M
   if (j >= M) return;
   unsigned long long tile = 0, in = 0;
   for (int t = 0; t < 3; t++) {
-    const NodeAxis<R> a = node_axis<R>(x[3 * j + t], g.n[t], g.m);
+    const NodeAxis<R> a = node_axis<R>(x[3 * j + t], g.n[t], g.m, g.shift);
     tile = tile * T.ntile[t] + (unsigned)(a.start / T.ts[t]);
     in = in * T.ts[t] + (unsigned)(a.start % T.ts[t]);
   }

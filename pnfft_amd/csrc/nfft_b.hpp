@@ -147,6 +147,22 @@ int bin_nodes_noalloc(const R* x, long M, const GridGeom& g, const TileLayout& T
                       hipStream_t s);
 size_t noalloc_scratch_bytes();
 
+// Hessian gather (PNFFT_COMPUTE_HESSIAN_F by window derivatives): tab holds
+// 9 (2m+1) M reals and st 3 M ints of scratch (the node's psi, psi', psi'' per
+// axis and stencil starts); hess[6j + q], q = xx, xy, xz, yy, yz, zz, is written
+// (added onto when A.accumulate; R[6M] for c2r plans).  m = 2..8.
+template <class R>
+int hessian_gather(const BArgs<R>& A, Cplx<R>* hess, R* tab, int* st, hipStream_t s);
+
+// Strided node-value copies of the ik-differentiated transforms (comps = 2 for
+// complex values, 1 for the reals of c2r plans):
+//   put: out[(j stride + offset) comps + c] = (acc ? out[..] : 0) + in[j comps + c]
+//   get: out[j comps + c] = in[(j stride + offset) comps + c]
+template <class R>
+int strided_put(R* out, const R* in, long M, int comps, int stride, int offset, int acc, hipStream_t s);
+template <class R>
+int strided_get(R* out, const R* in, long M, int comps, int stride, int offset, hipStream_t s);
+
 // Number of sort passes for keys below key_limit.
 int sort_passes(unsigned long long key_limit);
 

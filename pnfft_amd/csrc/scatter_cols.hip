@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
       int l = 0;
       bool bad = false;
       if (L.j >= 0) {
-        const NodeAxis<float> na = node_axis<float>(L.x, nt, MT);
+        const NodeAxis<float> na = node_axis<float>(L.x, nt, MT, A.g.shift);
         l = local_off(na.start, ot, TS, nt);
         bad = l < 0;
         if (tab && L.x == L.px) {

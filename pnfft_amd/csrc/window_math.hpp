@@ -215,6 +215,36 @@ __host__ __device__ inline void window_axis(int win, int m, R frac, R nR, R b, R
   }
 }
 
+// cot y - 1/y and 1/y^2 - 1 - cot^2 y, the factors of the sinc-power window's
+// psi' and psi'' (kernel/ndft-parallel.c:1907-1915, :2052-2071).  FP32 takes
+// them from their Taylor series for |y| < 1/2, where the closed forms cancel
+// (-y/3 - y^3/45 - 2y^5/945 - ..., -1/3 - y^2/15 - 2y^4/189 - ...); FP64 keeps
+// the reference's closed forms.
+template <class R>
+__host__ __device__ inline R sincpow_c1(R y) {
+  if (sizeof(R) == sizeof(float) && m_fabs(y) < R(0.5)) {
+    const R y2 = y * y;
+    return -y * (R(1.0 / 3) +
+                 y2 * (R(1.0 / 45) +
+                       y2 * (R(2.0 / 945) + y2 * (R(1.0 / 4725) + y2 * (R(2.0 / 93555) + y2 * R(1382.0 / 638512875))))));
+  }
+  return R(1) / m_tan(y) - R(1) / y;
+}
+
+template <class R>
+__host__ __device__ inline R sincpow_c2(R y) {
+  if (sizeof(R) == sizeof(float) && m_fabs(y) < R(0.5)) {
+    const R y2 = y * y;
+    return -(R(1.0 / 3) +
+             y2 * (R(1.0 / 15) +
+                   y2 * (R(2.0 / 189) +
+                         y2 * (R(1.0 / 675) +
+                               y2 * (R(2.0 / 10395) + y2 * (R(1382.0 / 58046625) + y2 * R(4.0 / 1403325)))))));
+  }
+  const R t = m_tan(y);
+  return R(1) / (y * y) - R(1) - R(1) / (t * t);
+}
+
 // bspline_values with the order K fixed at compile time: same recursion and
 // operation order, every index static so V stays in registers.
 template <int K, class R>
@@ -281,9 +311,7 @@ __host__ __device__ inline void window_axis_s(int win, R frac, R nR, R b, R (&ps
 #pragma unroll
         for (int s = 0; s < CM; s++) {
           const R y = pi * (frac + R(m - s)) / b;
-          dpsi[s] = (m_fabs(y) > Consts<R>::eps)
-                        ? R(2 * m) * pi * nR / b * (R(1) / m_tan(y) - R(1) / y) * psi[s]
-                        : R(0);
+          dpsi[s] = (m_fabs(y) > Consts<R>::eps) ? R(2 * m) * pi * nR / b * sincpow_c1(y) * psi[s] : R(0);
         }
       }
       break;
@@ -295,6 +323,116 @@ __host__ __device__ inline void window_axis_s(int win, R frac, R nR, R b, R (&ps
 #pragma unroll
         for (int s = 0; s < CM; s++) dpsi[s] = kb_dpsi<R>(frac + R(m - s), nR, b, m, psi[s]);
       }
+      break;
+    }
+  }
+}
+
+// k(z) = h'(z)/z for the two KB branches, h as in kb_h_sinh / kb_h_sin:
+// (3 z cosh z - (z^2 + 3) sinh z) / z^5 and (z^2 sin z - 3 sin z + 3 z cos z) / z^5,
+// from their series -sum_{k>=2} (+-1)^k 2k (2k-2) z^{2k-4} / (2k+1)! where the
+// closed forms cancel (z^2 < 4).
+template <class R>
+__host__ __device__ inline R kb_k_sinh(R z) {
+  const R z2 = z * z;
+  if (z2 < R(4)) {
+    R term = R(1), sum = R(0);
+    R fact = R(120);  // (2k+1)! for k = 2
+    for (int k = 2; k <= 12; k++) {
+      sum += R(2 * k * (2 * k - 2)) * term / fact;
+      term *= z2;
+      fact *= R((2 * k + 2) * (2 * k + 3));
+    }
+    return -sum;
+  }
+  return (R(3) * z * m_cosh(z) - (z2 + R(3)) * m_sinh(z)) / (z2 * z2 * z);
+}
+
+template <class R>
+__host__ __device__ inline R kb_k_sin(R z) {
+  const R z2 = z * z;
+  if (z2 < R(4)) {
+    R term = R(1), sum = R(0), sgn = R(-1);
+    R fact = R(120);
+    for (int k = 2; k <= 12; k++) {
+      sum += sgn * R(2 * k * (2 * k - 2)) * term / fact;
+      term *= z2;
+      sgn = -sgn;
+      fact *= R((2 * k + 2) * (2 * k + 3));
+    }
+    return sum;
+  }
+  return (z2 * m_sin(z) - R(3) * m_sin(z) + R(3) * z * m_cos(z)) / (z2 * z2 * z);
+}
+
+// Kaiser-Bessel psi'' at xn = n x - l (kaiser_bessel_second_derivative_1d,
+// kernel/ndft-parallel.c:2272-2285).  FP64 keeps the reference's formula; FP32
+// uses n^2 b^3/pi (h(z) - b^2 xn^2 k(z)) (z = b r), its cancellation-free form,
+// and -n^2 b^3/pi (h_s(z) + b^2 xn^2 k_s(z)) outside the support.
+template <class R>
+__host__ __device__ inline R kb_ddpsi(R xn, R nR, R b, int m, R psi, R dpsi) {
+  const R d = R(m) * R(m) - xn * xn;
+  const R r = m_sqrt(m_fabs(d));
+  const R pi = pi_v(R(0));
+  if (d == R(0)) {
+    const R bn = b * nR, bm = b * R(m);
+    return b * bn * bn / (R(15) * pi) * (bm * bm - R(5));
+  }
+  if (sizeof(R) == sizeof(double)) {
+    const R hyp = (d < R(0)) ? m_cos(b * r) : m_cosh(b * r);
+    return R(3) * nR * xn * dpsi / d + nR * nR * psi / d * (R(1) + b * xn * b * xn) - b * nR * nR / (pi * d) * hyp;
+  }
+  const R z = b * r, c = nR * nR * b * b * b / pi, bx2 = b * b * xn * xn;
+  if (d > R(0)) return c * (kb_h_sinh(z) - bx2 * kb_k_sinh(z));
+  return -c * (kb_h_sin(z) + bx2 * kb_k_sin(z));
+}
+
+// psi'' on the stencil of one axis (pre_ddpsi_tensor_*, kernel/ndft-parallel.c:
+// 1956-2105), from psi and psi' of the same node; width CM = 2m+1 static.
+template <class R, int CM>
+__host__ __device__ inline void window_axis_dd_s(int win, R frac, R nR, R b, const R (&psi)[CM],
+                                                 const R (&dpsi)[CM], R (&dd)[CM]) {
+  constexpr int m = (CM - 1) / 2;
+  const R pi = pi_v(R(0));
+  const R nn = nR * nR;
+  switch (win) {
+    case WIN_GAUSS: {
+#pragma unroll
+      for (int s = 0; s < CM; s++) {
+        const R u = frac + R(m - s);
+        dd[s] = R(2) * nn / b * (R(2) / b * u * u - R(1)) * psi[s];
+      }
+      break;
+    }
+    case WIN_BSPLINE: {
+      // n^2 (M_{2m-2}(t) - 2 M_{2m-2}(t-1) + M_{2m-2}(t-2)), Q[d] = M_{2m-2}(1 - frac + d)
+      R Q[2 * m - 2];
+      bspline_values_s<2 * m - 2, R>(R(1) - frac, Q);
+#pragma unroll
+      for (int s = 0; s < CM; s++) {
+        const R q3 = (s >= 3 && s - 3 <= 2 * m - 3) ? Q[s - 3] : R(0);
+        const R q2 = (s >= 2 && s - 2 <= 2 * m - 3) ? Q[s - 2] : R(0);
+        const R q1 = (s >= 1 && s - 1 <= 2 * m - 3) ? Q[s - 1] : R(0);
+        dd[s] = nn * (q3 - R(2) * q2 + q1);
+      }
+      break;
+    }
+    case WIN_SINCPOW: {
+      const R a = pi * nR / b;
+#pragma unroll
+      for (int s = 0; s < CM; s++) {
+        const R y = pi * (frac + R(m - s)) / b;
+        if (m_fabs(y) > Consts<R>::eps) {
+          dd[s] = R(2 * m) * a * sincpow_c1(y) * dpsi[s] + R(2 * m) * a * a * sincpow_c2(y) * psi[s];
+        } else {
+          dd[s] = -R(2 * m) * a * a / (R(3) * b);
+        }
+      }
+      break;
+    }
+    default: {  // Kaiser-Bessel
+#pragma unroll
+      for (int s = 0; s < CM; s++) dd[s] = kb_ddpsi<R>(frac + R(m - s), nR, b, m, psi[s], dpsi[s]);
       break;
     }
   }

@@ -46,6 +46,7 @@ class DevicePlan:
         self.x = torch.zeros((self.M, 3), dtype=rdt, device=self.device)
         self.f = torch.zeros(self.M, dtype=vdt, device=self.device)
         self.grad = torch.zeros((self.M, 3), dtype=vdt, device=self.device) if alloc_grad else None
+        self.hess = None  # allocated by alloc_hessian()
         self.f_hat = torch.zeros(self.fh_shape, dtype=cdt, device=self.device)
         self.api.set_x(_ptr(self.x), self.nodes)
         self.api.set_f(_ptr(self.f), self.nodes)
@@ -66,6 +67,12 @@ class DevicePlan:
 
     def set_grad(self, g):
         self.grad.copy_(torch.as_tensor(g).reshape(self.M, 3).to(self.grad.dtype))
+
+    def alloc_hessian(self):
+        """hessian_f: (M, 6) components xx, xy, xz, yy, yz, zz (real for c2r plans)."""
+        self.hess = torch.zeros((self.M, 6), dtype=self.f.dtype, device=self.device)
+        self.api.set_hessian_f(_ptr(self.hess), self.nodes)
+        return self.hess
 
     # operations ---------------------------------------------------------------
     def _check(self):

@@ -9,9 +9,9 @@ host cores).  Tolerances: rel-l2 <= 1e-12 for the FP64 plans, <= 1e-5 for the
 FP32 plans (against FP64, as the benchmark's accuracy gate).
 
 The FFT tests compare the pruned pass chain with torch.fft (rocFFT) on the
-device: trafo with PNFFT_OMIT_DECONV | PNFFT_OMIT_CONV leaves F f_hat in the
-plan's oversampled grid, and adj with PNFFT_OMIT_CONV then gives
-D^T F^H (F f_hat).
+device: trafo with PNFFT_OMIT_DECONV | PNFFT_OMIT_CONV leaves F f_hat (times the
+plan's power-of-two window scale) in the plan's oversampled grid, and adj with
+PNFFT_OMIT_CONV then gives D^T F^H of that grid.
 """
 import ctypes
 import sys
@@ -168,7 +168,11 @@ def test_fft_chain_matches_rocfft(gpu, precision, N, n):
     grid = _copy_grid(dp, n, cdt)
     fh = torch.tensor(fh_np, dtype=torch.complex128, device="cuda:0")
     A, idx = _embed(fh, N, n)
-    ref = torch.fft.fftn(A)  # rocFFT, FP64
+    # the plan's window values carry 2^-e per axis (e = exponent of psi(0)) and its D
+    # factors 2^e; with PNFFT_OMIT_DECONV the transform keeps the 2^e, so that B F
+    # is the reference's B F
+    e = sum(int(np.frexp(abs(dp.api.psi(dp.plan, t, 0.0)))[1]) - 1 for t in range(3))
+    ref = torch.fft.fftn(A) * 2.0 ** e  # rocFFT, FP64
     e_fwd = float(torch.linalg.norm(grid.to(torch.complex128) - ref) / torch.linalg.norm(ref))
     # backward: f_hat = D^T F^H grid with the plan's own D factors
     dp.api.adj(dp.plan, None, PA.PNFFT_OMIT_CONV)
@@ -179,9 +183,7 @@ def test_fft_chain_matches_rocfft(gpu, precision, N, n):
     d = [torch.tensor([dp.api.inv_phi_hat(dp.plan, t, k) for k in range(-(N[t] // 2), N[t] - N[t] // 2)],
                       dtype=torch.float64, device="cuda:0") for t in range(3)]
     D = d[0][:, None, None] * d[1][None, :, None] * d[2][None, None, :]
-    # the plan's D factors carry 2^e per axis (e = exponent of psi(0)), which its window
-    # values in B drop again; with PNFFT_OMIT_CONV that power of two stays in f_hat
-    e = sum(int(np.frexp(abs(dp.api.psi(dp.plan, t, 0.0)))[1]) - 1 for t in range(3))
+    # the D factors' 2^e stays in f_hat with PNFFT_OMIT_CONV
     ref_b = back * D * 2.0 ** e
     e_bwd = float(torch.linalg.norm(out - ref_b) / torch.linalg.norm(ref_b))
     dp.close()
