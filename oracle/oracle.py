@@ -25,6 +25,11 @@ PNFFT_WINDOW_GAUSSIAN = 1 << 13
 PNFFT_WINDOW_BSPLINE = 1 << 14
 PNFFT_WINDOW_SINC_POWER = 1 << 15
 PNFFT_WINDOW_BESSEL_I0 = 1 << 16
+PNFFT_PRE_CONST_PSI = 1 << 2
+PNFFT_PRE_LIN_PSI = 1 << 3
+PNFFT_PRE_QUAD_PSI = 1 << 4
+PNFFT_PRE_CUB_PSI = 1 << 5
+PNFFT_USE_FK_GAUSSIAN_T = 1 << 17
 PNFFT_INTERLACED = 1 << 8
 PNFFT_DIFF_IK = 1 << 12
 PNFFT_COMPUTE_F = 1 << 0
@@ -97,6 +102,12 @@ def _load(path):
     L.orc_B_trafo_hess.argtypes = [lp, lp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_long, dp, dp,
                                    ctypes.c_int, dp]
     L.orc_ndft_hessian_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp]
+    L.orc_cerf.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp]
+    L.orc_intpol_table.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_long, ctypes.c_int, dp]
+    L.orc_intpol_set.argtypes = [ctypes.c_int, ctypes.c_long] + [dp] * 9
+    L.orc_intpol_clear.argtypes = []
+    L.orc_intpol_axis.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
     L.orc_ndft_hessian_c2r.argtypes = [lp, ctypes.c_long, dp, dp, dp]
     L.orc_ndft_trafo_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
     L.orc_ndft_adj_c2c.argtypes = [lp, ctypes.c_long, dp, dp, dp, dp, ctypes.c_uint]
@@ -191,6 +202,41 @@ def psi(window, n, b, m, x):
 
 def dpsi(window, n, b, m, x):
     return lib().orc_dpsi(window, int(n), float(b), int(m), float(x))
+
+
+def cerf(z):
+    """erf(z) for Re z > 0 (Abramowitz & Stegun 7.1.29; the reference calls libcerf, cerf/)."""
+    re, im = ctypes.c_double(), ctypes.c_double()
+    lib().orc_cerf(float(z.real), float(z.imag), ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value)
+
+
+def intpol_order(flags):
+    """Interpolation order of kernel/ndft-parallel.c:997-1006 after the flag implications of
+    api/api-guru.c:152-155 (PRE_HESSIAN_PSI -> PRE_GRAD_PSI -> PRE_PSI on the plan flags, whose
+    values alias PRE_LIN_PSI -> PRE_CONST_PSI -> FAST_GAUSSIAN): -1 for none."""
+    if flags & (1 << 3):
+        flags |= 1 << 2
+    if flags & (1 << 2):
+        flags |= 1 << 1
+    for bit, order in ((PNFFT_PRE_CONST_PSI, 0), (PNFFT_PRE_LIN_PSI, 1), (PNFFT_PRE_QUAD_PSI, 2),
+                       (PNFFT_PRE_CUB_PSI, 3)):
+        if flags & bit:
+            return order
+    return -1
+
+
+def intpol_nodes(m):
+    """Interpolation nodes per unit interval, kernel/ndft-parallel.c:1095: ceil(31/(2m+1)) * 2048."""
+    return int(np.ceil(31.0 / (2 * m + 1))) * 2048
+
+
+def intpol_table(window, n, b, m, order, deriv):
+    """init_intpol_table_psi (kernel/ndft-parallel.c:318-352)."""
+    K = intpol_nodes(m)
+    tab = np.zeros(K * (2 * m + 1) * (order + 1))
+    lib().orc_intpol_table(window, int(n), float(b), int(m), int(order), K, int(deriv), _dp(tab))
+    return tab
 
 
 def ddpsi(window, n, b, m, x):
@@ -382,6 +428,26 @@ class Plan:
         self.workers = workers
         self.c2r = bool(c2r)
         self.Nh = [self.N[0], self.N[1], self.N[2] // 2 + 1]  # c2r half spectrum extent
+        # interpolated windows (PNFFT_PRE_*_PSI): tables [derivative][axis], the derivatives
+        # only without ik-differentiation (kernel/ndft-parallel.c:1097-1122)
+        self.ip_order = intpol_order(self.flags)
+        self.ip_tabs = None
+        if self.ip_order >= 0:
+            derivs = 1 if self.flags & PNFFT_DIFF_IK else 3
+            self.ip_tabs = [[intpol_table(self.window, self.n[t], self.b[t], self.m, self.ip_order, d)
+                             for t in range(3)] for d in range(derivs)]
+            zero = np.zeros(1)
+            while len(self.ip_tabs) < 3:
+                self.ip_tabs.append([zero, zero, zero])
+
+    def _ip_on(self):
+        if self.ip_tabs is not None:
+            lib().orc_intpol_set(self.ip_order, intpol_nodes(self.m),
+                                 *[_dp(self.ip_tabs[d][t]) for d in range(3) for t in range(3)])
+
+    def _ip_off(self):
+        if self.ip_tabs is not None:
+            lib().orc_intpol_clear()
 
     # -- c2r: D on the half spectrum, k2 = i2 - N2/2 <= 0 (kernel/matrix_D.c:319-356 over local_N)
     def _D_half(self, fhat_half, accumulate_into=None):
@@ -429,8 +495,10 @@ class Plan:
         f = np.zeros(M) if compute_f else None
         grad = np.zeros(3 * M) if compute_grad else None
         fl = (PNFFT_COMPUTE_F if compute_f else 0) | (PNFFT_COMPUTE_GRAD_F if compute_grad else 0)
+        self._ip_on()
         lib().orc_B_trafo(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
                           _dp(x), _dp(gpad), 1, _dp(f), _dp(grad), fl)
+        self._ip_off()
         return f, grad
 
     def adj_c2r(self, x, f=None, grad=None, parallel=False):
@@ -447,8 +515,10 @@ class Plan:
             ga = _f64(grad).reshape(-1)
             fl |= PNFFT_COMPUTE_GRAD_F
         scatter = lib().orc_B_adj_blocks if parallel else lib().orc_B_adj
+        self._ip_on()
         scatter(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
                 _dp(x), _dp(fa), _dp(ga), 1, _dp(gpad), fl)
+        self._ip_off()
         g2 = self.ghost_reduce(gpad)
         return self._D_half(self._FH_r2c(g2)).reshape(-1)
 
@@ -508,10 +578,12 @@ class Plan:
         f = np.zeros(M, np.complex128) if compute_f else None
         grad = np.zeros(3 * M, np.complex128) if compute_grad else None
         fl = (PNFFT_COMPUTE_F if compute_f else 0) | (PNFFT_COMPUTE_GRAD_F if compute_grad else 0)
+        self._ip_on()
         lib().orc_B_trafo(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
                           _dp(x), _dp(gpad.view(np.float64)), 0,
                           _dp(f.view(np.float64)) if f is not None else None,
                           _dp(grad.view(np.float64)) if grad is not None else None, fl)
+        self._ip_off()
         return f, grad
 
     def adj(self, x, f=None, grad=None, fhat_in=None, parallel=False):
@@ -530,10 +602,12 @@ class Plan:
             ga = _c128(grad).reshape(-1)
             fl |= PNFFT_COMPUTE_GRAD_F
         scatter = lib().orc_B_adj_blocks if parallel else lib().orc_B_adj
+        self._ip_on()
         scatter(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M,
-                        _dp(x), _dp(fa.view(np.float64)) if fa is not None else None,
-                        _dp(ga.view(np.float64)) if ga is not None else None, 0,
-                        _dp(gpad.view(np.float64)), fl)
+                _dp(x), _dp(fa.view(np.float64)) if fa is not None else None,
+                _dp(ga.view(np.float64)) if ga is not None else None, 0,
+                _dp(gpad.view(np.float64)), fl)
+        self._ip_off()
         g2 = self.ghost_reduce(gpad)
         g1 = self.FH(g2)
         fhat = np.zeros(int(np.prod(self.N)), np.complex128) if fhat_in is None else _c128(fhat_in).reshape(-1).copy()
@@ -602,10 +676,12 @@ class Plan:
         fl = (PNFFT_COMPUTE_F if f is not None else 0) | (PNFFT_COMPUTE_GRAD_F if grad is not None else 0)
         args = (_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M, _dp(x),
                 _dp(gpad if isr else gpad.view(np.float64)), isr)
+        self._ip_on()
         if fl:
             lib().orc_B_trafo(*args, v(f), v(grad), fl)
         if hess is not None:
             lib().orc_B_trafo_hess(*args, v(hess))
+        self._ip_off()
 
     def _grid_adj(self, x, f=None, grad=None):
         """B^T, ghost reduce and F^H of node values (f_hat layout)."""
@@ -622,9 +698,11 @@ class Plan:
             return _dp(a if isr else a.view(np.float64)), a
 
         fv, gv = v(f), v(grad)
+        self._ip_on()
         lib().orc_B_adj(_lp(_longs(self.n)), _lp(_longs(self.no)), self.m, self.window, _dp(self.b), M, _dp(x),
                         fv[0] if fv else None, gv[0] if gv else None, isr,
                         _dp(gpad if isr else gpad.view(np.float64)), fl)
+        self._ip_off()
         g2 = self.ghost_reduce(gpad)
         return self._FH_r2c(g2) if self.c2r else self.FH(g2).reshape(self.N)
 

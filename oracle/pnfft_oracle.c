@@ -35,7 +35,7 @@
 #define C_COMPUTE_GRAD_F   (1u << 1)
 #define C_COMPUTE_HESSIAN  (1u << 2)
 
-enum { W_KB = 0, W_GAUSS = 1, W_BSPLINE = 2, W_SINCPOW = 3, W_BESSEL_I0 = 4 };
+enum { W_KB = 0, W_GAUSS = 1, W_BSPLINE = 2, W_SINCPOW = 3, W_BESSEL_I0 = 4, W_GAUSS_T = 5 };
 
 /* Window selection order of kernel/ndft-parallel.c:1650-1675. */
 int orc_window_id(unsigned flags)
@@ -174,6 +174,28 @@ double orc_bspline(int k, double x)
   return V[d0];
 }
 
+/* Complex error function erf(x + i y), x > 0, by Abramowitz & Stegun 7.1.29:
+ *   erf(x) + e^{-x^2}/(2 pi x) [(1 - cos 2xy) + i sin 2xy]
+ *   + (2/pi) e^{-x^2} sum_{k>=1} e^{-k^2/4}/(k^2 + 4x^2) [f_k + i g_k],
+ *   f_k = 2x - 2x cosh(ky) cos(2xy) + k sinh(ky) sin(2xy),
+ *   g_k = 2x cosh(ky) sin(2xy) + k sinh(ky) cos(2xy),
+ * relative error about 1e-16 (the reference calls libcerf's cerf, cerf/). */
+void orc_cerf(double x, double y, double *re, double *im)
+{
+  const double e = exp(-x * x), c = cos(2.0 * x * y), sn = sin(2.0 * x * y);
+  double r = erf(x) + e / (2.0 * ORC_PI * x) * (1.0 - c);
+  double i = e / (2.0 * ORC_PI * x) * sn;
+  double sr = 0.0, si = 0.0;
+  for (int k = 1; k <= 60; k++) {
+    const double w = exp(-0.25 * k * k) / (k * k + 4.0 * x * x);
+    const double ch = cosh(k * y), sh = sinh(k * y);
+    sr += w * (2.0 * x - 2.0 * x * ch * c + k * sh * sn);
+    si += w * (2.0 * x * ch * sn + k * sh * c);
+  }
+  *re = r + 2.0 / ORC_PI * e * sr;
+  *im = i + 2.0 / ORC_PI * e * si;
+}
+
 /* ---------------------------------------------------------------------------
  * Window Fourier coefficients (matrix D), kernel/matrix_D.c:30-225.
  * The factor 1/n of D cancels the factor n of the coefficients (:29).
@@ -190,6 +212,12 @@ double orc_inv_phi_hat(int window, long k, long n, double b, int m)
     case W_SINCPOW: { /* inv_phi_hat_sinc_power, :124-132 */
       double d = fabs((double)k * b / (double)n);
       return (d < m) ? 1.0 / orc_bspline(2 * m, d + m) : 0.0;
+    }
+    case W_GAUSS_T: { /* inv_phi_hat_gauss_t, :37-55: 1 / (Re erf(m/sqrt b + i pi k sqrt b / n) e^{-b (pi k/n)^2}) */
+      double sb = sqrt(b), re, im;
+      orc_cerf((double)m / sb, ORC_PI * (double)k * sb / (double)n, &re, &im);
+      double a = ORC_PI * (double)k / (double)n;
+      return 1.0 / (re * exp(-a * a * b));
     }
     case W_BESSEL_I0: { /* inv_phi_hat_bessel_i0, :91-102 */
       double a = 2.0 * ORC_PI * (double)k / (double)n;
@@ -398,6 +426,76 @@ void orc_psi_axis(int window, long n, double b, int m, double nx,
       break;
     }
   }
+}
+
+/* ---------------------------------------------------------------------------
+ * Interpolated windows, PNFFT_PRE_{CONST,LIN,QUAD,CUB}_PSI
+ * (kernel/ndft-parallel.c:1088-1122, :318-352, :1586-1617, kernel/ipnfft.h:457-497).
+ * A table of axis t holds, for k in [0, K), stencil point c and offsets
+ * i = -order/2 .. (order+1)/2, the value psi^(d)((m + (k+i)/K - c)/n);
+ * a node with fractional part dist = n x - floor(n x) reads the entries of
+ * k = floor(dist K) and interpolates at dist K - k.  orc_intpol_set makes
+ * node_windows (and so every B routine) read tables instead of evaluating.
+ * ------------------------------------------------------------------------- */
+double orc_ddpsi(int window, long n, double b, int m, double x);
+double orc_dpsi(int window, long n, double b, int m, double x);
+double orc_psi(int window, long n, double b, int m, double x);
+
+void orc_intpol_table(int window, long n, double b, int m, int order, long K, int deriv, double *table)
+{
+  const int cutoff = 2 * m + 1;
+  long ind = 0;
+  for (long k = 0; k < K; k++)
+    for (int c = 0; c < cutoff; c++)
+      for (int i = -order / 2; i <= (order + 1) / 2; i++, ind++) {
+        const double x = ((double)m + (double)(k + i) / (double)K - c) / (double)n;
+        table[ind] = deriv == 0 ? orc_psi(window, n, b, m, x)
+                   : deriv == 1 ? orc_dpsi(window, n, b, m, x) : orc_ddpsi(window, n, b, m, x);
+      }
+}
+
+static double intpol_eval(int order, const double *f, double dist)
+{
+  switch (order) {
+    case 0: return f[0];
+    case 1: return f[0] * (1.0 - dist) + f[1] * dist;
+    case 2: {
+      const double c0 = dist + 1.0, c1 = dist, c2 = dist - 1.0;
+      return 0.5 * f[0] * c1 * c2 - c0 * f[1] * c2 + 0.5 * c0 * c1 * f[2];
+    }
+    default: {
+      const double c0 = dist + 1.0, c1 = dist, c2 = dist - 1.0, c3 = dist - 2.0;
+      return (-f[0] * c1 * c2 * c3 + 3.0 * c0 * f[1] * c2 * c3 - 3.0 * c0 * c1 * f[2] * c3 + c0 * c1 * c2 * f[3]) / 6.0;
+    }
+  }
+}
+
+static struct {
+  int order;
+  long K;
+  const double *tab[3][3]; /* [derivative][axis] */
+} g_ip = {-1, 0, {{0}}};
+
+void orc_intpol_set(int order, long K, const double *p0, const double *p1, const double *p2, const double *d0,
+                    const double *d1, const double *d2, const double *dd0, const double *dd1, const double *dd2)
+{
+  g_ip.order = order;
+  g_ip.K = K;
+  const double *t[9] = {p0, p1, p2, d0, d1, d2, dd0, dd1, dd2};
+  for (int q = 0; q < 9; q++) g_ip.tab[q / 3][q % 3] = t[q];
+}
+
+void orc_intpol_clear(void) { g_ip.order = -1; }
+
+/* pre_tensor_intpol (kernel/ndft-parallel.c:1586-1617) for axis t, derivative d */
+void orc_intpol_axis(int t, int d, int m, double nx, double *out)
+{
+  const int cutoff = 2 * m + 1, w = g_ip.order + 1;
+  const double dist = nx - floor(nx);
+  const long k = (long)floor(dist * (double)g_ip.K);
+  const double dk = dist * (double)g_ip.K - (double)k;
+  const double *tab = g_ip.tab[d][t] + k * cutoff * w;
+  for (int s = 0; s < cutoff; s++) out[s] = intpol_eval(g_ip.order, tab + s * w, dk);
 }
 
 /* psi, psi' and psi'' of one axis's stencil (ddpsi may not be null). */
@@ -709,6 +807,8 @@ void orc_inv_phi_hat_tables(const long *N, const long *n, int m, unsigned flags,
 {
   double b[3];
   int w = orc_window_id(flags);
+  /* the truncated Gaussian changes only D (PNX(inv_phi_hat), kernel/matrix_D.c:191-196) */
+  if (w == W_GAUSS && (flags & F_USE_FK_GAUSS_T)) w = W_GAUSS_T;
   orc_window_param_b(N, n, m, flags, b);
   long off = 0;
   for (int t = 0; t < 3; t++)
@@ -745,7 +845,12 @@ static void node_windows(int window, const long *n, const double *b, int m, cons
     double nx = (double)n[t] * xj[t];
     double fl = floor(nx);
     u[t] = (long)fl - m + no[t] / 2 + m;
-    orc_psi_axis(window, n[t], b[t], m, nx, psi + c * t, dpsi ? dpsi + c * t : NULL);
+    if (g_ip.order >= 0) {
+      orc_intpol_axis(t, 0, m, nx, psi + c * t);
+      if (dpsi) orc_intpol_axis(t, 1, m, nx, dpsi + c * t);
+    } else {
+      orc_psi_axis(window, n[t], b[t], m, nx, psi + c * t, dpsi ? dpsi + c * t : NULL);
+    }
   }
 }
 
@@ -811,8 +916,12 @@ void orc_B_trafo_hess(const long *n, const long *no, int m, int window, const do
     double psi[3 * 64], dpsi[3 * 64], dd[3 * 64];
     long u[3];
     node_windows(window, n, b, m, x + 3 * j, psi, dpsi, u, no);
-    for (int t = 0; t < 3; t++)
-      ddpsi_axis(window, n[t], b[t], m, (double)n[t] * x[3 * j + t], psi + c * t, dpsi + c * t, dd + c * t);
+    for (int t = 0; t < 3; t++) {
+      if (g_ip.order >= 0)
+        orc_intpol_axis(t, 2, m, (double)n[t] * x[3 * j + t], dd + c * t);
+      else
+        ddpsi_axis(window, n[t], b[t], m, (double)n[t] * x[3 * j + t], psi + c * t, dpsi + c * t, dd + c * t);
+    }
     double acc[12] = {0};
     for (int l0 = 0; l0 < c; l0++)
       for (int l1 = 0; l1 < c; l1++) {

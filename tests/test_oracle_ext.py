@@ -198,3 +198,116 @@ def test_general_c2r_matches_c2c_of_hermitian_extension(oracle, flags):
     # the reference's c2r NDFT gradient is minus d/dx of its c2r f (kernel/ndft-parallel.c:530-532,
     # :596-599); the fast gradients, window-derivative or ik, are d/dx
     assert rel(fr, fo) < 1e-3 and rel(gr, -go) < 1e-2
+
+
+# ---------------------------------------------------------------- truncated Gaussian
+def test_cerf_vs_mpmath_and_scipy(oracle):
+    import mpmath
+    import scipy.special
+    for z in [complex(2.17, 0.0), complex(2.17, 0.72), complex(3.76, 1.25), complex(4.34, -1.45), complex(0.5, 2.9),
+              complex(1.0, 0.1), complex(6.0, 3.0)]:
+        ref = complex(mpmath.erf(mpmath.mpc(z.real, z.imag)))
+        got = oracle.cerf(z)
+        assert abs(got - ref) <= 2e-15 * abs(ref), (z, got, ref)
+        assert abs(got - complex(scipy.special.erf(z))) <= 1e-13 * abs(ref)
+
+
+@pytest.mark.parametrize("m", [2, 4, 6])
+def test_gaussian_t_phi_hat_is_fourier_transform_of_truncated_window(oracle, m):
+    """1/inv_phi_hat(k) == n * integral_{-m/n}^{m/n} psi(x) e^{-2 pi i k x} dx for the Gaussian
+    truncated at the stencil's half width (kernel/matrix_D.c:37-64)."""
+    import scipy.integrate
+    N, n = 16, 32
+    flags = WINDOWS["gauss"] | (1 << 17)
+    b = oracle.window_b([N] * 3, [n] * 3, m, flags)[0]
+    tab = oracle.inv_phi_hat_tables([N] * 3, [n] * 3, m, flags)[:N]
+    w = oracle.window_id(WINDOWS["gauss"])
+    for i in (0, 3, 8, 15):
+        k = i - N // 2
+        re = scipy.integrate.quad(lambda x: oracle.psi(w, n, b, m, x) * np.cos(2 * np.pi * k * x), -m / n, m / n,
+                                  epsabs=1e-15, epsrel=1e-13, limit=200)[0]
+        assert abs(1.0 / tab[i] - n * re) <= 1e-12 * abs(n * re), (m, k, 1.0 / tab[i], n * re)
+    # the untruncated Gaussian differs (its Fourier transform includes the tails)
+    plain = oracle.inv_phi_hat_tables([N] * 3, [n] * 3, m, WINDOWS["gauss"])[:N]
+    assert np.max(np.abs(plain / tab - 1)) > 1e-8
+
+
+def test_gaussian_t_fast_transform_matches_ndft(oracle):
+    N, n, x, fh, M = _case(13, N=(16, 16, 16), n=(32, 32, 32))
+    fo, go = oracle.ndft_trafo(N, x, fh, compute_grad=True)
+    errs = []
+    for m in (4, 6):
+        ft, _ = oracle.Plan(N, n, m, WINDOWS["gauss"] | (1 << 17)).trafo(x, fh)
+        fg, _ = oracle.Plan(N, n, m, WINDOWS["gauss"]).trafo(x, fh)
+        # the aliasing error dominates: the truncated window is as accurate as the plain one
+        assert rel(ft, fo) < 1.5 * rel(fg, fo), m
+        errs.append(rel(ft, fo))
+    assert errs[1] < 1e-5 and errs[1] < 0.1 * errs[0], errs
+
+
+# ---------------------------------------------------------------- interpolated windows
+PRE = {"const": 1 << 2, "lin": 1 << 3, "quad": 1 << 4, "cub": 1 << 5}
+
+
+def test_intpol_order_follows_flag_aliasing(oracle):
+    """api/api-guru.c:152-155 on the plan flags: LIN implies CONST, so CONST wins (quirk)."""
+    assert [oracle.intpol_order(PRE[k]) for k in ("const", "lin", "quad", "cub")] == [0, 0, 2, 3]
+    assert oracle.intpol_order(0) == -1
+
+
+@pytest.mark.parametrize("wname", ["kb", "gauss", "bspline", "besseli0"])
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+def test_interpolated_window_error_bound(oracle, wname, order):
+    """Interpolation of order p on a grid of spacing h = 1/K reproduces psi at the nodes and is
+    within c h^{p+1} max|psi^{(p+1)}| between them; off-by-one table offsets fail both."""
+    m, n = 4, 32
+    flags = WINDOWS[wname]
+    w = oracle.window_id(flags)
+    b = oracle.window_b([16] * 3, [n] * 3, m, flags)[0]
+    K = oracle.intpol_nodes(m)
+    tabs = [oracle.intpol_table(w, n, b, m, order, d) for d in range(3)]
+    lib = oracle.lib()
+    lib.orc_intpol_set(order, K, *[oracle._dp(tabs[d]) for d in range(3) for _ in range(3)])
+    try:
+        rng = np.random.default_rng(order + 7)
+        worst = 0.0
+        for nx in list(rng.uniform(-16, 16, 40)) + [3.0 + 5.0 / K, -7.0 + 100.0 / K]:
+            fl = np.floor(nx)
+            out = np.zeros(2 * m + 1)
+            lib.orc_intpol_axis(0, 0, m, float(nx), oracle._dp(out))
+            exact, _ = oracle.psi_axis(w, n, b, m, nx)
+            on_node = abs((nx - fl) * K - round((nx - fl) * K)) < 1e-9
+            for s in range(2 * m + 1):
+                if wname == "kb" and s == 0:
+                    continue  # the sin continuation outside the support
+                err = abs(out[s] - exact[s]) / max(abs(exact).max(), 1e-300)
+                if on_node:
+                    assert err < 1e-13, (nx, s, err)
+                worst = max(worst, err)
+        bound = {0: 2e-3, 1: 1e-6, 2: 1e-9, 3: 1e-12}[order] * (30 if wname == "besseli0" else 1)
+        assert worst < bound, (wname, order, worst)
+        if order == 0:
+            assert worst > 1e-6  # constant interpolation is visibly not exact
+    finally:
+        lib.orc_intpol_clear()
+
+
+@pytest.mark.parametrize("key", ["const", "quad", "cub"])
+def test_interpolated_fast_transform_matches_ndft(oracle, key):
+    N, n, x, fh, M = _case(14, N=(16, 16, 16), n=(32, 32, 32))
+    fo, go = oracle.ndft_trafo(N, x, fh, compute_grad=True)
+    P = oracle.Plan(N, n, 6, PRE[key])
+    f, g, h = P.trafo_general(x, fh, True, True, True)
+    P0 = oracle.Plan(N, n, 6, 0)
+    f0, g0, _ = P0.trafo_general(x, fh, True, True, False)
+    tol = {"const": 1e-2, "quad": 1e-8, "cub": 1e-9}[key]
+    assert rel(f, fo) < tol and rel(g, go) < 10 * tol
+    if key == "cub":
+        assert rel(f, f0) < 1e-11  # cubic interpolation at K = 4096 is exact to ~1e-13
+    else:
+        assert rel(f, f0) > 1e-14  # and the tables are really used
+    # adjointness holds for the interpolated windows as for the exact ones
+    rng = SplitMix64(15)
+    fv = rng.complex_normal(M)
+    a = P.adj_general(x, f=fv)
+    assert abs(np.vdot(fv, f) - np.vdot(a, fh)) < 1e-12 * abs(np.vdot(fv, f))
