@@ -29,7 +29,7 @@ COLS_VARIANTS = [
     for r in ("f32", "f64")
     for mi, m in ((1, "f"), (2, "grad"), (3, "fgrad"))
 ]
-# pre_psi.hip and hessian.hip are compiled once per precision
+# pre_psi.hip, hessian.hip and window_tab.hip are compiled once per precision
 PRE_VARIANTS = [(r, [f"-DPNFFT_B_REAL={'float' if r == 'f32' else 'double'}"]) for r in ("f32", "f64")]
 ARCH = os.environ.get("PNFFT_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -65,7 +65,8 @@ def build(verbose=True, force=True):
     jobs = ([("b_kernels.hip", "_" + sfx, d) for sfx, d in B_VARIANTS] +
             [("scatter_cols.hip", "_" + sfx, d) for sfx, d in COLS_VARIANTS] +
             [("pre_psi.hip", "_" + sfx, d) for sfx, d in PRE_VARIANTS] +
-            [("hessian.hip", "_" + sfx, d) for sfx, d in PRE_VARIANTS] + [(s, "", []) for s in SOURCES])
+            [("hessian.hip", "_" + sfx, d) for sfx, d in PRE_VARIANTS] +
+            [("window_tab.hip", "_" + sfx, d) for sfx, d in PRE_VARIANTS] + [(s, "", []) for s in SOURCES])
     with ThreadPoolExecutor(max_workers=max(1, min(len(jobs), os.cpu_count() or 4))) as ex:
         objs = list(ex.map(lambda j: _compile(j, force), jobs))
     tmp = LIB + ".tmp"

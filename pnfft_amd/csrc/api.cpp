@@ -38,7 +38,12 @@ constexpr unsigned MALLOC_F_HAT = 1u << 6;
 constexpr unsigned INTERLACED = 1u << 8;
 constexpr unsigned TRANSPOSED_F_HAT = 1u << 11;
 constexpr unsigned DIFF_IK = 1u << 12;
+constexpr unsigned WINDOW_GAUSSIAN = 1u << 13;
 constexpr unsigned WINDOW_BESSEL_I0 = 1u << 16;
+constexpr unsigned PRE_CONST_PSI = 1u << 2;
+constexpr unsigned PRE_LIN_PSI = 1u << 3;
+constexpr unsigned PRE_QUAD_PSI = 1u << 4;
+constexpr unsigned PRE_CUB_PSI = 1u << 5;
 constexpr unsigned USE_FK_GAUSSIAN_T = 1u << 17;
 constexpr unsigned REAL_F = 1u << 4;
 constexpr unsigned PRE_GRAD_PSI = 1u << 2;
@@ -229,6 +234,12 @@ struct PlanT {
   // tables of N0 + N1 + N2 (re, im) pairs, filled per call in pinned host memory)
   double* d_ctab = nullptr;
   double* h_ctab = nullptr;
+  bool gauss_t = false;   // PNFFT_WINDOW_GAUSSIAN_T: the truncated Gaussian's D (matrix_D.c:37-64)
+  // windows the B kernels take from per-call tables (window_ext.hpp): the
+  // Bessel-I0 window and the interpolated windows of PNFFT_PRE_*_PSI
+  bool tab_only = false;
+  IntpolTabs<R> ip{};
+  R* d_ip = nullptr;      // interpolation tables, 9 of them ([derivative][axis])
   GridGeom geom{};
   TileLayout tiles{};
   EventTimer evt;
@@ -298,6 +309,67 @@ struct NodesT {
   TileLayout bin_tiles{};
 };
 
+// 1/phi_hat and phi_hat of axis t (PNX(inv_phi_hat), PNX(phi_hat), kernel/matrix_D.c:191-225)
+template <class R>
+static double plan_phi_hat(const PlanT<R>* P, int t, long k) {
+  if (P->gauss_t) return host::phi_hat_gauss_t(k, P->n[t], P->b[t], P->m);
+  return host::phi_hat(P->win, k, P->n[t], P->b[t], P->m);
+}
+template <class R>
+static double plan_inv_phi_hat(const PlanT<R>* P, int t, long k) {
+  if (P->gauss_t) return 1.0 / host::phi_hat_gauss_t(k, P->n[t], P->b[t], P->m);
+  return host::inv_phi_hat(P->win, k, P->n[t], P->b[t], P->m);
+}
+
+// Interpolation order of kernel/ndft-parallel.c:997-1006 (on the plan flags
+// after the implications of api/api-guru.c:152-155, so LIN implies CONST).
+static int intpol_order_of(unsigned flags) {
+  if (flags & fl::PRE_CONST_PSI) return 0;
+  if (flags & fl::PRE_LIN_PSI) return 1;
+  if (flags & fl::PRE_QUAD_PSI) return 2;
+  if (flags & fl::PRE_CUB_PSI) return 3;
+  return -1;
+}
+
+// Interpolation tables of init_intpol_table_psi (kernel/ndft-parallel.c:318-352):
+// K = ceil(31/(2m+1)) 2048 nodes per unit interval (:1095); derivative tables
+// only without ik-differentiation (:1106-1121).  Entries shared by neighbouring
+// k are evaluated once, as the reference does.
+template <class R>
+static bool intpol_tables(PlanT<R>* P, int order) {
+  const int m = P->m, cutoff = 2 * m + 1, w = order + 1;
+  const long K = (long)std::ceil(31.0 / cutoff) * 2048;
+  const size_t per = (size_t)K * cutoff * w;
+  const int nderiv = (P->pnfft_flags & fl::DIFF_IK) ? 1 : 3;
+  std::vector<R> h(per * 9, R(0));
+  for (int d = 0; d < nderiv; d++)
+    for (int t = 0; t < 3; t++) {
+      R* tab = h.data() + (size_t)(d * 3 + t) * per;
+      size_t ind = 0;
+      for (long k = 0; k < K; k++)
+        for (int c = 0; c < cutoff; c++)
+          for (int i = -order / 2; i <= (order + 1) / 2; i++, ind++) {
+            if (k > 0 && i < (order + 1) / 2) {
+              tab[ind] = tab[ind - (size_t)cutoff * w + 1];
+              continue;
+            }
+            const double x = ((double)m + (double)(k + i) / (double)K - c) / (double)P->n[t];
+            const double v = d == 0   ? host::psi(P->win, P->n[t], P->b[t], m, x)
+                             : d == 1 ? host::dpsi(P->win, P->n[t], P->b[t], m, x)
+                                      : host::ddpsi(P->win, P->n[t], P->b[t], m, x);
+            tab[ind] = (R)v;
+          }
+    }
+  P->d_ip = dalloc<R>(per * 9);
+  if (!P->d_ip || hipMemcpy(P->d_ip, h.data(), sizeof(R) * per * 9, hipMemcpyHostToDevice) != hipSuccess)
+    return false;
+  P->ip.order = order;
+  P->ip.K = K;
+  for (int d = 0; d < 3; d++)
+    for (int t = 0; t < 3; t++) P->ip.tab[d][t] = P->d_ip + (size_t)(d * 3 + t) * per;
+  return true;
+}
+
 template <class R>
 static void tables_compute(PlanT<R>* P) {
   // window scale 2^-e per axis, e = exponent of psi(0) (the window's peak); the
@@ -312,7 +384,7 @@ static void tables_compute(PlanT<R>* P) {
   long off = 0;
   for (int t = 0; t < 3; t++)
     for (long i = 0; i < P->N[t]; i++)
-      P->h_tab[off++] = std::ldexp(host::inv_phi_hat(P->win, i - P->N[t] / 2, P->n[t], P->b[t], P->m), e[t]);
+      P->h_tab[off++] = std::ldexp(plan_inv_phi_hat(P, t, i - P->N[t] / 2), e[t]);
   if (P->d_tab)
     (void)hipMemcpy(P->d_tab, P->h_tab.data(), sizeof(double) * P->h_tab.size(), hipMemcpyHostToDevice);
   std::vector<double> sc(P->h_tab.size());
@@ -400,6 +472,7 @@ static void plan_destroy(PlanT<R>* P) {
   dfree(P->d_tab_scale);
   dfree(P->d_ctab);
   if (P->h_ctab) (void)hipHostFree(P->h_ctab);
+  dfree(P->d_ip);
   P->evt.release();
   if (P->stream) (void)hipStreamDestroy(P->stream);
   delete static_cast<typename Handles<R>::plan*>(P);
@@ -418,11 +491,9 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
     set_error("c2r plans need an even N[2] (half spectrum N[2]/2+1)");
     return nullptr;
   }
-  const unsigned unsupported = fl::TRANSPOSED_F_HAT | fl::WINDOW_BESSEL_I0 | fl::USE_FK_GAUSSIAN_T;
+  const unsigned unsupported = fl::TRANSPOSED_F_HAT;
   if (pnfft_flags & unsupported) {
-    set_error("plan flags 0x%x (transposed f_hat, Bessel-I0 or truncated Gaussian window) are not supported "
-              "by this build",
-              pnfft_flags & unsupported);
+    set_error("plan flags 0x%x (transposed f_hat) are not supported by this build", pnfft_flags & unsupported);
     return nullptr;
   }
   // the B kernels are compiled for stencils of 2m+1 = 5..17 points
@@ -460,6 +531,9 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
   P->pnfft_flags = pnfft_flags;
   P->pfft_flags = pfft_flags;
   P->win = host::window_from_flags(pnfft_flags);
+  P->gauss_t = (pnfft_flags & fl::WINDOW_GAUSSIAN) && (pnfft_flags & fl::USE_FK_GAUSSIAN_T);
+  const int ip_order = intpol_order_of(pnfft_flags);
+  P->tab_only = P->win == WIN_BESSEL_I0 || ip_order >= 0;
   P->N_total = P->n_total = 1;
   for (int t = 0; t < 3; t++) {
     P->N[t] = N[t];
@@ -506,6 +580,12 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
   }
   P->evt.init();
   tables_compute(P);
+  if (ip_order >= 0 && !intpol_tables(P, ip_order)) {
+    (void)hipGetLastError();
+    set_error("device setup of the interpolation tables failed");
+    plan_destroy(P);
+    return nullptr;
+  }
   if (pnfft_flags & fl::MALLOC_F_HAT) {
     P->f_hat = host_alloc(sizeof(Cplx<R>) * P->fhat_elems);
     P->own_f_hat = true;
@@ -844,8 +924,10 @@ static T* grow(T*& p, size_t& cap, size_t need) {
 // and, with valid bins, the window tables.  false on a binning failure.
 template <class R>
 static bool b_args(PlanT<R>* P, NodesT<R>* D, R* dx, const GridGeom& g, bool host_x, bool for_scatter,
-                   bool timed, BArgs<R>& A, bool* had_sort) {
-  const bool reuse = D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x && same_geom(D->bin_geom, g);
+                   bool timed, BArgs<R>& A, bool* had_sort, bool need_grad = true) {
+  // plans whose windows come from tables bin and tabulate on every call
+  const bool reuse = !P->tab_only && D->bins_valid && (D->precompute_flags & fl::PRE_PSI) && !host_x &&
+                     same_geom(D->bin_geom, g);
   if (!reuse) {
     if (timed) P->evt.rec(0, P->stream);
     if (bin(P, D, dx, g) != kOk) return false;
@@ -861,6 +943,20 @@ static bool b_args(PlanT<R>* P, NodesT<R>* D, R* dx, const GridGeom& g, bool hos
   A.c2r = P->c2r;
   set_tiles(A, D, for_scatter);
   if (reuse) set_tables(A, P, D);
+  if (P->tab_only && D->M > 0) {
+    const size_t M = (size_t)D->M, cm = 2 * (size_t)P->m + 1;
+    R* psi = grow(D->d_pre_psi, D->pre_psi_cap, 3 * cm * M);
+    R* dpsi = need_grad ? grow(D->d_pre_dpsi, D->pre_dpsi_cap, 3 * cm * M) : nullptr;
+    R* xo = grow(D->d_pre_x, D->pre_x_cap, 3 * M);
+    D->pre_valid = false;  // the precomputed tables, if any, are overwritten
+    if (!psi || !xo || (need_grad && !dpsi) || ext_tables<R>(A, P->ip, psi, dpsi, xo, P->stream) != kOk) {
+      set_error("window tables for the plan's window failed (device memory or launch)");
+      return false;
+    }
+    A.pre_psi = psi;
+    A.pre_dpsi = dpsi;
+    A.pre_x = xo;
+  }
   return true;
 }
 
@@ -881,7 +977,7 @@ static int hessian_stage(PlanT<R>* P, NodesT<R>* D, BArgs<R> A, Cplx<R>* dh, int
   }
   A.accumulate = acc;
   A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
-  return hessian_gather<R>(A, dh, tab, st, P->stream);
+  return hessian_gather<R>(A, P->ip, dh, tab, st, P->stream);
 }
 
 // Complex factor tables of this call (ctab_fill) uploaded to the device:
@@ -933,7 +1029,7 @@ static bool trafo_passes(PlanT<R>* P, NodesT<R>* D, unsigned cf, R* dx, Cplx<R>*
     g.shift = ps ? 0.5 : 0.0;
     const int acc = (accumulate || ps > 0) ? 1 : 0;
     BArgs<R> A{};
-    if (conv && !b_args(P, D, dx, g, host_x, false, false, A, nullptr)) return false;
+    if (conv && !b_args(P, D, dx, g, host_x, false, false, A, nullptr, !ik && want_g)) return false;
     A.real_only = (P->pnfft_flags & fl::REAL_F) ? 1 : 0;
     if (!ik) {
       if (!fwd(ps, 0)) return false;
@@ -1050,7 +1146,7 @@ static void trafo_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
     // B
     if (!(cf & fl::OMIT_CONV) && M > 0 && (want_f || want_g || want_h)) {
       BArgs<R> A{};
-      if (!b_args(P, D, dx, P->geom, host_x, false, true, A, &had_sort)) return;
+      if (!b_args(P, D, dx, P->geom, host_x, false, true, A, &had_sort, want_g)) return;
       A.f = df;
       A.grad = dg;
       A.accumulate = accumulate;
@@ -1102,7 +1198,7 @@ static bool adj_passes(PlanT<R>* P, NodesT<R>* D, unsigned cf, R* dx, Cplx<R>* d
     GridGeom g = P->geom;
     g.shift = ps ? 0.5 : 0.0;
     BArgs<R> A{};
-    if (conv && (use_f || use_g) && !b_args(P, D, dx, g, host_x, true, false, A, nullptr)) return false;
+    if (conv && (use_f || use_g) && !b_args(P, D, dx, g, host_x, true, false, A, nullptr, !ik && use_g)) return false;
     // grid <- B^T (values) for one mode; then f_hat += factors(comp) F^H grid
     auto stage = [&](int mode, Cplx<R>* f, Cplx<R>* gr, int comp) {
       if (!(cf & fl::OMIT_CONV)) {
@@ -1196,7 +1292,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
       // scatter() writes every cell itself
       if (M > 0 && (use_f || use_g)) {
         BArgs<R> A{};
-        if (!b_args(P, D, dx, P->geom, host_x, true, true, A, &had_sort)) return;
+        if (!b_args(P, D, dx, P->geom, host_x, true, true, A, &had_sort, use_g)) return;
         A.f = df;
         A.grad = dg;
         A.mid = P->evt.mark(10);
@@ -1235,8 +1331,9 @@ template <class R>
 static void vjp_impl(PlanT<R>* P, NodesT<R>* D, const Cplx<R>* w, R* gx) {
   clear_error();
   if (!P || !D) return;
-  if (P->pnfft_flags & fl::INTERLACED) {
-    set_error("trafo_grad_x_vjp: interlaced plans are not supported");
+  if ((P->pnfft_flags & fl::INTERLACED) || P->tab_only) {
+    set_error("trafo_grad_x_vjp: interlaced plans and plans with the Bessel-I0 or interpolated windows are not "
+              "supported");
     return;
   }
   hipStream_t s = P->stream;
@@ -1298,7 +1395,8 @@ static void precompute_impl(PlanT<R>* P, NodesT<R>* D, unsigned flags) {
   R* dx = stage_in<R>(D->x, D->d_x, 3 * D->M, true, P->stream, &host_x);
   if (!dx) return;
   if (bin(P, D, dx, P->geom) != kOk) return;
-  if (!host_x && P->m >= 2 && P->m <= 8 && tables_enabled()) {
+  // plans whose windows come from tables tabulate on every call (b_args)
+  if (!host_x && P->m >= 2 && P->m <= 8 && tables_enabled() && !P->tab_only) {
     const size_t M = (size_t)D->M, cm = 2 * (size_t)P->m + 1;
     const bool want_grad = (flags & fl::PRE_GRAD_PSI) != 0;
     R* psi = grow(D->d_pre_psi, D->pre_psi_cap, 3 * cm * M);
@@ -1616,10 +1714,10 @@ using namespace pnfft_amd;
     } else memset(p->f_hat, 0, sizeof(Cplx<R>) * p->fhat_elems);                                            \
   }                                                                                                        \
   R PREFIX##inv_phi_hat(const PREFIX##plan p, int dim, ptrdiff_t k) {                                    \
-    return (R)host::inv_phi_hat(p->win, k, p->n[dim], p->b[dim], p->m);                                     \
+    return (R)plan_inv_phi_hat(PREFIX##P(p), dim, k);                                                       \
   }                                                                                                        \
   R PREFIX##phi_hat(const PREFIX##plan p, int dim, ptrdiff_t k) {                                        \
-    return (R)host::phi_hat(p->win, k, p->n[dim], p->b[dim], p->m);                                         \
+    return (R)plan_phi_hat(PREFIX##P(p), dim, k);                                                           \
   }                                                                                                        \
   R PREFIX##psi(const PREFIX##plan p, int dim, R x) {                                                    \
     return (R)host::psi(p->win, p->n[dim], p->b[dim], p->m, (double)x);                                     \

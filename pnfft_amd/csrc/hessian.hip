@@ -19,6 +19,7 @@
 // them.  This is not the headline path; it is written for register bounds that
 // hold at every m = 2..8 in both precisions rather than for speed.
 #include "b_device.hpp"
+#include "window_ext.hpp"
 
 namespace pnfft_amd {
 
@@ -30,7 +31,8 @@ constexpr int kHT = 256;  // threads per block
 
 // tab[((t * 3 + q) * CM + s) * M + p], q = 0 psi, 1 psi', 2 psi''; st[t * M + p]
 template <class R, int MT>
-__global__ __launch_bounds__(kHT) void hess_tab_k(BArgs<R> A, R* __restrict__ tab, int* __restrict__ st) {
+__global__ __launch_bounds__(kHT) void hess_tab_k(BArgs<R> A, IntpolTabs<R> I, R* __restrict__ tab,
+                                                  int* __restrict__ st) {
   constexpr int CM = 2 * MT + 1;
   const long p = (long)blockIdx.x * kHT + threadIdx.x;
   const long M = A.M;
@@ -41,8 +43,7 @@ __global__ __launch_bounds__(kHT) void hess_tab_k(BArgs<R> A, R* __restrict__ ta
     const int nt = A.g.n[t];
     const NodeAxis<R> a = node_axis<R>(A.x[3 * j + t], nt, MT, A.g.shift);
     R psi[CM], dpsi[CM], dd[CM];
-    window_axis_s<R, CM, true>(A.g.win, a.frac, (R)nt, (R)A.g.b[t], psi, dpsi);
-    window_axis_dd_s<R, CM>(A.g.win, a.frac, (R)nt, (R)A.g.b[t], psi, dpsi, dd);
+    ext_window_axis<R, CM, true, true>(A.g, I, t, a.frac, psi, dpsi, dd);
     const R ws = (R)A.g.ws[t];  // the plan's power-of-two window scale (exact)
 #pragma unroll
     for (int s = 0; s < CM; s++) {
@@ -144,9 +145,9 @@ __global__ __launch_bounds__(kHT) void hess_gather_k(BArgs<R> A, Cplx<R>* __rest
 }
 
 template <class R, int MT>
-int launch_hess(const BArgs<R>& A, Cplx<R>* hess, R* tab, int* st, hipStream_t s) {
+int launch_hess(const BArgs<R>& A, const IntpolTabs<R>& I, Cplx<R>* hess, R* tab, int* st, hipStream_t s) {
   const dim3 grid((unsigned)((A.M + kHT - 1) / kHT)), block(kHT);
-  hipLaunchKernelGGL((hess_tab_k<R, MT>), grid, block, 0, s, A, tab, st);
+  hipLaunchKernelGGL((hess_tab_k<R, MT>), grid, block, 0, s, A, I, tab, st);
   hipLaunchKernelGGL((hess_gather_k<R, MT>), grid, block, 0, s, A, hess, tab, st);
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
@@ -175,16 +176,16 @@ __global__ __launch_bounds__(256) void strided_get_k(R* __restrict__ out, const 
 }  // namespace
 
 template <class R>
-int hessian_gather(const BArgs<R>& A, Cplx<R>* hess, R* tab, int* st, hipStream_t s) {
+int hessian_gather(const BArgs<R>& A, const IntpolTabs<R>& I, Cplx<R>* hess, R* tab, int* st, hipStream_t s) {
   if (A.M == 0) return kOk;
   switch (A.g.m) {
-    case 2: return launch_hess<R, 2>(A, hess, tab, st, s);
-    case 3: return launch_hess<R, 3>(A, hess, tab, st, s);
-    case 4: return launch_hess<R, 4>(A, hess, tab, st, s);
-    case 5: return launch_hess<R, 5>(A, hess, tab, st, s);
-    case 6: return launch_hess<R, 6>(A, hess, tab, st, s);
-    case 7: return launch_hess<R, 7>(A, hess, tab, st, s);
-    case 8: return launch_hess<R, 8>(A, hess, tab, st, s);
+    case 2: return launch_hess<R, 2>(A, I, hess, tab, st, s);
+    case 3: return launch_hess<R, 3>(A, I, hess, tab, st, s);
+    case 4: return launch_hess<R, 4>(A, I, hess, tab, st, s);
+    case 5: return launch_hess<R, 5>(A, I, hess, tab, st, s);
+    case 6: return launch_hess<R, 6>(A, I, hess, tab, st, s);
+    case 7: return launch_hess<R, 7>(A, I, hess, tab, st, s);
+    case 8: return launch_hess<R, 8>(A, I, hess, tab, st, s);
     default: return kUnsupported;
   }
 }
@@ -205,8 +206,8 @@ int strided_get(R* out, const R* in, long M, int comps, int stride, int offset, 
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
 
-template int hessian_gather<PNFFT_B_REAL>(const BArgs<PNFFT_B_REAL>&, Cplx<PNFFT_B_REAL>*, PNFFT_B_REAL*, int*,
-                                          hipStream_t);
+template int hessian_gather<PNFFT_B_REAL>(const BArgs<PNFFT_B_REAL>&, const IntpolTabs<PNFFT_B_REAL>&,
+                                          Cplx<PNFFT_B_REAL>*, PNFFT_B_REAL*, int*, hipStream_t);
 template int strided_put<PNFFT_B_REAL>(PNFFT_B_REAL*, const PNFFT_B_REAL*, long, int, int, int, int, hipStream_t);
 template int strided_get<PNFFT_B_REAL>(PNFFT_B_REAL*, const PNFFT_B_REAL*, long, int, int, int, hipStream_t);
 

@@ -76,6 +76,62 @@ inline double bessel_i0(double x) {
   return std::exp(ax) / std::sqrt(2.0 * kPi * ax) * sum;
 }
 
+// I1(x) in FP64 (the reference calls GSL, kernel/bessel_i1.c:567-597): the
+// series sum_k (x/2)^{2k+1} / (k! (k+1)!) for |x| <= 25, beyond it
+// e^x/sqrt(2 pi x) sum_k (-1)^k prod_{j<=k} (4 - (2j-1)^2) / (k! (8x)^k)
+// (Abramowitz & Stegun 9.7.1 with mu = 4); odd in x.
+inline double bessel_i1(double x) {
+  const double ax = std::fabs(x);
+  double v;
+  if (ax <= 25.0) {
+    const double q = 0.25 * ax * ax;
+    double t = 0.5 * ax, sum = t;
+    for (int k = 1; k < 200; k++) {
+      t *= q / (double(k) * double(k + 1));
+      sum += t;
+      if (t < 1e-18 * sum) break;
+    }
+    v = sum;
+  } else {
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 80; k++) {
+      const double c = 2.0 * k - 1.0;
+      const double next = -term * (4.0 - c * c) / (8.0 * k * ax);
+      if (std::fabs(next) > std::fabs(term)) break;
+      term = next;
+      sum += term;
+      if (std::fabs(term) < 1e-17 * std::fabs(sum)) break;
+    }
+    v = std::exp(ax) / std::sqrt(2.0 * kPi * ax) * sum;
+  }
+  return x < 0 ? -v : v;
+}
+
+// erf(x + i y) for x > 0 (Abramowitz & Stegun 7.1.29, relative error about
+// 1e-16), the complex error function of the truncated Gaussian's Fourier
+// coefficients (the reference calls libcerf's cerf, kernel/matrix_D.c:37-64).
+inline void cerf(double x, double y, double* re, double* im) {
+  const double e = std::exp(-x * x), c = std::cos(2.0 * x * y), sn = std::sin(2.0 * x * y);
+  double sr = 0.0, si = 0.0;
+  for (int k = 1; k <= 60; k++) {
+    const double w = std::exp(-0.25 * k * k) / (k * k + 4.0 * x * x);
+    const double ch = std::cosh(k * y), sh = std::sinh(k * y);
+    sr += w * (2.0 * x - 2.0 * x * ch * c + k * sh * sn);
+    si += w * (2.0 * x * ch * sn + k * sh * c);
+  }
+  *re = std::erf(x) + e / (2.0 * kPi * x) * (1.0 - c) + 2.0 / kPi * e * sr;
+  *im = e / (2.0 * kPi * x) * sn + 2.0 / kPi * e * si;
+}
+
+// phi_hat of the Gaussian truncated to the stencil, e^{-b (pi k/n)^2} Re erf(m/sqrt b + i pi k sqrt b/n)
+// (phi_hat_gauss_t, kernel/matrix_D.c:57-64).
+inline double phi_hat_gauss_t(long k, long n, double b, int m) {
+  const double sb = std::sqrt(b), a = kPi * double(k) / double(n);
+  double re, im;
+  cerf(double(m) / sb, kPi * double(k) * sb / double(n), &re, &im);
+  return std::exp(-a * a * b) * re;
+}
+
 // Cardinal B-spline M_k(x) on (0, k), zero elsewhere (kernel/bspline.c:45-91).
 inline double bspline(int k, double x) {
   if (!(x > 0.0 && x < double(k)) || k > 2 * kMaxM) return 0.0;
@@ -152,6 +208,10 @@ inline double psi(int win, long n, double b, int m, double x) {
     case WIN_GAUSS: return std::exp(-(xn * xn) / b) / std::sqrt(kPi * b);
     case WIN_BSPLINE: return bspline(2 * m, xn + m);
     case WIN_SINCPOW: return std::pow(sinc_ref<double>(kPi * xn / b), 2.0 * m) / b;
+    case WIN_BESSEL_I0: {  // window_bessel_i0_1d, kernel/ndft-parallel.c:2195-2203
+      const double d = double(m) * m - xn * xn;
+      return (d < 0) ? 0.0 : 0.5 * bessel_i0(b * std::sqrt(d));
+    }
     default: return kb_psi<double>(xn, b, m);
   }
 }
@@ -166,6 +226,12 @@ inline double dpsi(int win, long n, double b, int m, double x) {
       if (std::fabs(y) > Consts<double>::eps)
         return 2.0 * m * kPi * double(n) / b * (1.0 / std::tan(y) - 1.0 / y) * psi(win, n, b, m, x);
       return 0.0;
+    }
+    case WIN_BESSEL_I0: {  // window_bessel_i0_derivative_1d, :2205-2218
+      const double d = double(m) * m - xn * xn;
+      if (d < 0) return 0.0;
+      const double r = std::sqrt(d);
+      return (d > 0) ? -0.5 * b * double(n) * xn * bessel_i1(b * r) / r : -(0.5 * b * n) * (0.5 * b * n) * xn / n;
     }
     default: return kb_dpsi<double>(xn, double(n), b, m, kb_psi<double>(xn, b, m));
   }
@@ -190,6 +256,16 @@ inline double ddpsi(int win, long n, double b, int m, double x) {
                2.0 * m * (kPi * nd / b) * (kPi * nd / b) * (-1.0 + 1.0 / (y * y) - t * t) * psi(win, n, b, m, x);
       }
       return -2.0 * m / (3.0 * b) * (kPi * nd / b) * (kPi * nd / b);
+    }
+    case WIN_BESSEL_I0: {  // window_bessel_i0_second_derivative_1d, :2220-2237
+      const double d = double(m) * m - xn * xn;
+      if (d < 0) return 0.0;
+      if (d > 0) {
+        const double r = std::sqrt(d), y = xn * xn;
+        return 0.5 * b * nd * nd / d * (b * y * bessel_i0(b * r) - bessel_i1(b * r) / r * (y + double(m) * m));
+      }
+      const double q = b * b * m * nd, h = b * nd;
+      return q * q / 16.0 - h * h / 4.0;
     }
     default: {
       const double p = kb_psi<double>(xn, b, m);

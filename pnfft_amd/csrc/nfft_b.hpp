@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
+#include "window_ext.hpp"
 
 namespace pnfft_amd {
 
@@ -149,10 +150,17 @@ size_t noalloc_scratch_bytes();
 
 // Hessian gather (PNFFT_COMPUTE_HESSIAN_F by window derivatives): tab holds
 // 9 (2m+1) M reals and st 3 M ints of scratch (the node's psi, psi', psi'' per
-// axis and stencil starts); hess[6j + q], q = xx, xy, xz, yy, yz, zz, is written
-// (added onto when A.accumulate; R[6M] for c2r plans).  m = 2..8.
+// axis, from I's tables for interpolated windows, and stencil starts);
+// hess[6j + q], q = xx, xy, xz, yy, yz, zz, is written (added onto when
+// A.accumulate; R[6M] for c2r plans).  m = 2..8.
 template <class R>
-int hessian_gather(const BArgs<R>& A, Cplx<R>* hess, R* tab, int* st, hipStream_t s);
+int hessian_gather(const BArgs<R>& A, const IntpolTabs<R>& I, Cplx<R>* hess, R* tab, int* st, hipStream_t s);
+
+// Window tables of the Bessel-I0 window and of interpolated windows (the
+// layout of precompute_tables: psi[(t CM + s) M + p], dpsi likewise if
+// non-null, x_out[t M + p]), for the nodes in A.perm's order and A.g.
+template <class R>
+int ext_tables(const BArgs<R>& A, const IntpolTabs<R>& I, R* psi, R* dpsi, R* x_out, hipStream_t s);
 
 // Strided node-value copies of the ik-differentiated transforms (comps = 2 for
 // complex values, 1 for the reals of c2r plans):
