@@ -170,7 +170,9 @@ def traffic_file(workload):
     return os.path.join(ROOT, "profiles", "traffic_%s.json" % workload.replace("^", "e"))
 
 
-def run_plan(w, precision, steps, warmup, x_np):
+def run_plan(w, precision, steps, warmup, x_np, tables=True):
+    """tables=False: the node bins are made once but no window tables, so every B kernel
+    evaluates the windows of its nodes (the window-recomputation mode)."""
     import torch
 
     import pnfft_amd as PA
@@ -182,7 +184,10 @@ def run_plan(w, precision, steps, warmup, x_np):
     dp.set_f(w["f"])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    if not tables:
+        os.environ["PNFFT_AMD_TABLES"] = "0"
     dp.precompute(PA.PNFFT_PRE_PSI | PA.PNFFT_PRE_GRAD_PSI)  # window tables of psi and psi' (setup)
+    os.environ.pop("PNFFT_AMD_TABLES", None)
     torch.cuda.synchronize()
     setup_ms = 1e3 * (time.perf_counter() - t0)
     f_in = dp.f.clone()
@@ -360,6 +365,10 @@ def main():
                for nm, a, b in zip(names, res["out"], fp64["out"])}
         log(f"  rel-l2 vs FP64: {acc}")
 
+    # the same steps with the windows evaluated in the B kernels instead of read from tables
+    recompute = run_plan(w, precision, args.steps, 1, x_np, tables=False)
+    log(f"  {precision}, windows recomputed: {recompute['ms_per_step']:.3f} ms/step")
+
     steps = args.steps
     ab = algorithmic_bytes(w, precision)
     af = algorithmic_flops(w)
@@ -403,6 +412,8 @@ def main():
         "accuracy_rel_l2_vs_oracle": acc_oracle,
         # the same rate with the node binning and window tables redone for every step
         "value_incl_setup": w["M"] / ((res["ms_per_step"] + res["setup_ms"]) * 1e-3),
+        # the rate with the windows evaluated in the B kernels (no window tables; bins made once)
+        "value_window_recompute": w["M"] / (recompute["ms_per_step"] * 1e-3),
         "fp64_ms_per_step": fp64["ms_per_step"] if fp64 else None,
         "roofline": roofline,
         "cpu_baseline": cpu,

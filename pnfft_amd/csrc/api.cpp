@@ -765,13 +765,10 @@ static bool cols_enabled() {
   return v;
 }
 // PNFFT_AMD_TABLES=0: precompute_psi bins the nodes but tabulates no windows
-// (the B kernels evaluate them), for comparison
+// (the B kernels evaluate them), for comparison; read on every precompute call
 static bool tables_enabled() {
-  static const bool v = [] {
-    const char* e = getenv("PNFFT_AMD_TABLES");
-    return !(e && strcmp(e, "0") == 0);
-  }();
-  return v;
+  const char* e = getenv("PNFFT_AMD_TABLES");
+  return !(e && strcmp(e, "0") == 0);
 }
 static bool mfma_disabled() {
   static const bool v = [] {
