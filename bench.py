@@ -309,6 +309,32 @@ def copy_probe():
                       "launches": 3, "GB_per_s": round(3 * 2 * (1 << 30) / el / 1e9, 1)}), flush=True)
 
 
+def bw_probe(reps=20):
+    """Achievable HBM bandwidth, unprofiled: torch's device copy, sum (read only) and fill (write
+    only) of a 1 GiB buffer, one warm-up call each, then `reps` calls timed with events."""
+    import torch
+
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    acc = torch.empty((), dtype=torch.float32, device="cuda")
+    ops = {"copy": (lambda: dst.copy_(src), 2), "read": (lambda: torch.sum(src, (0,), out=acc), 1),
+           "write": (lambda: dst.fill_(2.0), 1)}
+    out = {"bytes_per_call": 1 << 30, "reps": reps}
+    for name, (fn, mult) in ops.items():
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[f"{name}_GBps"] = round(mult * (1 << 30) / (ms * 1e-3) / 1e9, 1)
+        out[f"{name}_ms"] = round(ms, 4)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,12 +348,17 @@ def main():
                     help="nodes of the CPU baseline's B stages (a quarter of the headline workload)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (calibrates the HBM byte counters)")
+    ap.add_argument("--bw-probe", action="store_true",
+                    help="only time a 1 GiB device copy, read and fill (achievable HBM bandwidth, unprofiled)")
     ap.add_argument("--pmc-traffic", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="rocprofv3 FETCH_SIZE / WRITE_SIZE outputs of a bench run (--steps 1 --warmup 0: "
                          "two calls of each stage) and of --copy-probe; writes profiles/traffic_<workload>.json")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.bw_probe:
+        bw_probe()
         return
     if args.pmc_traffic:
         from pnfft_amd.workloads import CONFIGS

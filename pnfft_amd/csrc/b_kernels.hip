@@ -493,15 +493,17 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
         amax = max(amax, v + CM - 1);
       }
     }
+    {
+      // the wave's rows are zeroed with consecutive lanes on consecutive 16-byte
+      // chunks (one lane per row would put the lanes 64 bytes apart, a few banks);
+      // the window values below are later stores of the same wave, so they land after
+      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < S::ROWS * 256 / 4 / 64; k++) reinterpret_cast<float4*>(rows)[lane + 64 * k] = z4;
+    }
     if (lk < 3) {
       float* r0 = rows + (lk * 16 + li) * 16;
       float* r1 = rows + ((3 + lk) * 16 + li) * 16;
-      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        reinterpret_cast<float4*>(r0)[e] = z4;
-        if (WG) reinterpret_cast<float4*>(r1)[e] = z4;
-      }
       if (ok) {
         const int sx = sw4(li) << 2;
 #pragma unroll

@@ -233,8 +233,15 @@ struct MfmaShape {
   static constexpr int NBUF = V == 1 ? 2 : 1;
 };
 
-// row position of plane a: wave (a & 3) reads the float4 at 4 (a & 3), entry a >> 2
-__device__ __forceinline__ int xpos(int a) { return ((a & 3) << 2) | (a >> 2); }
+// Window rows of the batch, 16 floats per term v, with positions swizzled per
+// row so that the rows phase, where lane q writes the same position of rows
+// q V + u, spreads its stores over the banks:
+// x rows: plane a in float4 chunk (a & 3) ^ ((v >> 1) & 3), entry a >> 2
+// (wave a & 3 reads its four planes as one float4); y and z rows: position p
+// at p ^ ((v >> 1) & 15).
+__device__ __forceinline__ int xchunk(int v, int w) { return w ^ ((v >> 1) & 3); }
+__device__ __forceinline__ int xpos(int v, int a) { return (xchunk(v, a & 3) << 2) | (a >> 2); }
+__device__ __forceinline__ int yzpos(int v, int p) { return p ^ ((v >> 1) & 15); }
 
 // one thread's share of a node's data for the rows phase: axis `role` (0..2)
 // or the complex weights (role 3)
@@ -260,9 +267,9 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
   // window rows of a batch; with two buffers the rows of batch b + 1 are written
   // while batch b is on the matrix cores (one barrier per batch)
   constexpr int NBUF = MS::NBUF;
-  __shared__ __align__(16) float s_x[NBUF][NV][16];  // X_v by xpos
-  __shared__ __align__(16) float s_y[NBUF][NV][16];  // Y_v[b], zero outside the stencil
-  __shared__ __align__(16) float s_z[NBUF][NV][16];  // Z_v[c]
+  __shared__ __align__(16) float s_x[NBUF][NV][16];  // X_v[a] at xpos(v, a)
+  __shared__ __align__(16) float s_y[NBUF][NV][16];  // Y_v[b] at yzpos(v, b), zero outside the stencil
+  __shared__ __align__(16) float s_z[NBUF][NV][16];  // Z_v[c] at yzpos(v, c)
   __shared__ float2 s_w[NBUF][NV];                   // W_v
   __shared__ unsigned char s_bad[NBUF][3][NB];       // node outside the tile on axis t
 
@@ -360,17 +367,20 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
         }
       }
       float(*srow)[16] = ax == 0 ? s_x[bf] : (ax == 1 ? s_y[bf] : s_z[bf]);
+      // the wave's rows are zeroed with consecutive lanes on consecutive 16-byte
+      // chunks; the window values below are later stores of the same wave
       const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-      for (int u = 0; u < V; u++) {
-        float* row = srow[q * V + u];
+      for (int k = 0; k < NV * 16 / 4 / 64; k++) reinterpret_cast<float4*>(&srow[0][0])[q + 64 * k] = z4;
 #pragma unroll
-        for (int e = 0; e < 4; e++) reinterpret_cast<float4*>(row)[e] = z4;
+      for (int u = 0; u < V; u++) {
+        const int v = q * V + u;
+        float* row = srow[v];
         if (L.j >= 0 && !bad) {
           // term u differentiates axis ax when it is the gradient term of that axis
           const bool d = WG && u == (WF ? 1 : 0) + ax;
 #pragma unroll
-          for (int s = 0; s < CM; s++) row[ax == 0 ? xpos(l + s) : l + s] = d ? dw[s] : w[s];
+          for (int s = 0; s < CM; s++) row[ax == 0 ? xpos(v, l + s) : yzpos(v, l + s)] = d ? dw[s] : w[s];
         }
       }
       s_bad[bf][ax][q] = bad ? 1 : 0;
@@ -414,8 +424,8 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
     const int nq = (nb * V + 3) >> 2;
     for (int q4 = 0; q4 < nq; q4++) {
       const int v = 4 * q4 + lk;
-      const float4 x = reinterpret_cast<const float4*>(s_x[bf][v])[wave];
-      const float y = s_y[bf][v][li], z = s_z[bf][v][li];
+      const float4 x = reinterpret_cast<const float4*>(s_x[bf][v])[xchunk(v, wave)];
+      const float y = s_y[bf][v][yzpos(v, li)], z = s_z[bf][v][yzpos(v, li)];
       const float2 wv = s_w[bf][v];
       const float bre = wv.x * z, bim = wv.y * z;
       const float xs[4] = {x.x, x.y, x.z, x.w};
