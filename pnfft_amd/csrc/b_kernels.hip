@@ -326,6 +326,22 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ int sw4(int r) { return (r >> 2) & 3; }
 __device__ __forceinline__ int rpos(int r, int v) { return v ^ (sw4(r) << 2); }
 
+// Position of cell c (axis 2) in a subgrid row: c = 4 e + k at 4 k + e.  The
+// float4 chunk k of a row then holds c = k, 4 + k, 8 + k, 12 + k, so the e-th
+// product of a plane (component e of every lane's chunk) covers the K-block
+// c = 4 e .. 4 e + 3, and a block that no stencil of the group reaches can be
+// left out.
+__device__ __forceinline__ int kp(int c) { return ((c & 3) << 2) | (c >> 2); }
+
+template <int N>
+struct IntC {
+  static constexpr int value = N;
+};
+
+__device__ __forceinline__ float comp4(const float4& v, int e) {
+  return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
+}
+
 // LDS writes of one lane become visible to the other lanes of its wave
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -365,7 +381,7 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
   constexpr bool WG = S::WG;
   static_assert(P <= 16, "one 16 x 16 block per plane");
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  float* s_gr = reinterpret_cast<float*>(smem_raw);  // [P][16 rows b][16], element c at rpos(b, c)
+  float* s_gr = reinterpret_cast<float*>(smem_raw);  // [P][16 rows b][16], element c at rpos(b, kp(c))
   float* s_gi = s_gr + P * 256;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -405,10 +421,12 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
       const int q = tid + NT * k;
       if (q < NPAIR) {
         const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
-        // cells c, c + 1 (c even) stay adjacent under the swizzle
-        const int r = (cx * 16 + cy) * 16 + rpos(cy, 2 * hz);
-        *reinterpret_cast<float2*>(s_gr + r) = make_float2(v[k].x, v[k].z);
-        *reinterpret_cast<float2*>(s_gi + r) = make_float2(v[k].y, v[k].w);
+        const int r = (cx * 16 + cy) * 16;
+        const int p0 = rpos(cy, kp(2 * hz)), p1 = rpos(cy, kp(2 * hz + 1));
+        s_gr[r + p0] = v[k].x;
+        s_gi[r + p0] = v[k].y;
+        s_gr[r + p1] = v[k].z;
+        s_gi[r + p1] = v[k].w;
       }
     }
   }
@@ -527,65 +545,78 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
     const float* dxr = rows + (48 + li) * 16;
     const int ch = 4 * (lk ^ sw4(li));
     const float4 yb = *reinterpret_cast<const float4*>(rows + (16 + li) * 16 + ch);  // Y_j[4 lk + e]
-    const float4 zc = *reinterpret_cast<const float4*>(rows + (32 + li) * 16 + ch);  // Z_j[4 lk + e]
+    // Z_j[4 e + lk] in component e: the K-slice of this lane group in the e-th product
+    auto zread = [&](const float* row) {
+      return make_float4(row[rpos(li, lk)], row[rpos(li, 4 + lk)], row[rpos(li, 8 + lk)], row[rpos(li, 12 + lk)]);
+    };
+    const float4 zc = zread(rows + (32 + li) * 16);
     float4 dyb = yb, dzc = zc;
     if (WG) {
       dyb = *reinterpret_cast<const float4*>(rows + (64 + li) * 16 + ch);
-      dzc = *reinterpret_cast<const float4*>(rows + (80 + li) * 16 + ch);
+      dzc = zread(rows + (80 + li) * 16);
     }
     float fr = 0.f, fi = 0.f, g0r = 0.f, g0i = 0.f, g1r = 0.f, g1i = 0.f, g2r = 0.f, g2i = 0.f;
-    // plane a + 1's operands are read while plane a is on the matrix cores
-    auto ld_plane = [&](int a, float4& gr, float4& gi, float& xa, float& dxa) {
-      gr = *reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * 16 + ch);
-      gi = *reinterpret_cast<const float4*>(s_gi + (a * 16 + li) * 16 + ch);
-      xa = xr[rpos(li, a)];
-      dxa = WG ? dxr[rpos(li, a)] : 0.f;
-    };
-    float4 gr, gi;
-    float xa, dxa;
-    ld_plane(amax >= amin ? amin : 0, gr, gi, xa, dxa);
-    for (int a = amin; a <= amax; a++) {
-      float4 grn, gin;
-      float xan, dxan;
-      ld_plane(a < amax ? a + 1 : a, grn, gin, xan, dxan);
-      mfma_f4 tr = {0.f, 0.f, 0.f, 0.f}, ti = tr;
-      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.x, zc.x, tr, 0, 0, 0);
-      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.x, zc.x, ti, 0, 0, 0);
-      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.y, zc.y, tr, 0, 0, 0);
-      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.y, zc.y, ti, 0, 0, 0);
-      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.z, zc.z, tr, 0, 0, 0);
-      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.z, zc.z, ti, 0, 0, 0);
-      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.w, zc.w, tr, 0, 0, 0);
-      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.w, zc.w, ti, 0, 0, 0);
-      const float sr = yb.x * tr[0] + yb.y * tr[1] + yb.z * tr[2] + yb.w * tr[3];
-      const float si = yb.x * ti[0] + yb.y * ti[1] + yb.z * ti[2] + yb.w * ti[3];
-      fr += xa * sr;
-      fi += xa * si;
-      if (WG) {
-        mfma_f4 ur = {0.f, 0.f, 0.f, 0.f}, ui = ur;
-        ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.x, dzc.x, ur, 0, 0, 0);
-        ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.x, dzc.x, ui, 0, 0, 0);
-        ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.y, dzc.y, ur, 0, 0, 0);
-        ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.y, dzc.y, ui, 0, 0, 0);
-        ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.z, dzc.z, ur, 0, 0, 0);
-        ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.z, dzc.z, ui, 0, 0, 0);
-        ur = __builtin_amdgcn_mfma_f32_16x16x4f32(gr.w, dzc.w, ur, 0, 0, 0);
-        ui = __builtin_amdgcn_mfma_f32_16x16x4f32(gi.w, dzc.w, ui, 0, 0, 0);
-        const float syr = dyb.x * tr[0] + dyb.y * tr[1] + dyb.z * tr[2] + dyb.w * tr[3];
-        const float syi = dyb.x * ti[0] + dyb.y * ti[1] + dyb.z * ti[2] + dyb.w * ti[3];
-        const float szr = yb.x * ur[0] + yb.y * ur[1] + yb.z * ur[2] + yb.w * ur[3];
-        const float szi = yb.x * ui[0] + yb.y * ui[1] + yb.z * ui[2] + yb.w * ui[3];
-        g0r += dxa * sr;
-        g0i += dxa * si;
-        g1r += xa * syr;
-        g1i += xa * syi;
-        g2r += xa * szr;
-        g2i += xa * szi;
+    // plane a + 1's operands are read while plane a is on the matrix cores;
+    // products e = K0 .. K0 + NK - 1 (the K-blocks the group's stencils reach)
+    auto plane_loop = [&](auto k0c, auto nkc) __attribute__((always_inline)) {
+      constexpr int K0 = decltype(k0c)::value, NK = decltype(nkc)::value;
+      auto ld_plane = [&](int a, float4& gr, float4& gi, float& xa, float& dxa) {
+        gr = *reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * 16 + ch);
+        gi = *reinterpret_cast<const float4*>(s_gi + (a * 16 + li) * 16 + ch);
+        xa = xr[rpos(li, a)];
+        dxa = WG ? dxr[rpos(li, a)] : 0.f;
+      };
+      float4 gr, gi;
+      float xa, dxa;
+      ld_plane(amax >= amin ? amin : 0, gr, gi, xa, dxa);
+      for (int a = amin; a <= amax; a++) {
+        float4 grn, gin;
+        float xan, dxan;
+        ld_plane(a < amax ? a + 1 : a, grn, gin, xan, dxan);
+        mfma_f4 tr = {0.f, 0.f, 0.f, 0.f}, ti = tr;
+#pragma unroll
+        for (int e = K0; e < K0 + NK; e++) {
+          tr = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), comp4(zc, e), tr, 0, 0, 0);
+          ti = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), comp4(zc, e), ti, 0, 0, 0);
+        }
+        const float sr = yb.x * tr[0] + yb.y * tr[1] + yb.z * tr[2] + yb.w * tr[3];
+        const float si = yb.x * ti[0] + yb.y * ti[1] + yb.z * ti[2] + yb.w * ti[3];
+        fr += xa * sr;
+        fi += xa * si;
+        if (WG) {
+          mfma_f4 ur = {0.f, 0.f, 0.f, 0.f}, ui = ur;
+#pragma unroll
+          for (int e = K0; e < K0 + NK; e++) {
+            ur = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), comp4(dzc, e), ur, 0, 0, 0);
+            ui = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), comp4(dzc, e), ui, 0, 0, 0);
+          }
+          const float syr = dyb.x * tr[0] + dyb.y * tr[1] + dyb.z * tr[2] + dyb.w * tr[3];
+          const float syi = dyb.x * ti[0] + dyb.y * ti[1] + dyb.z * ti[2] + dyb.w * ti[3];
+          const float szr = yb.x * ur[0] + yb.y * ur[1] + yb.z * ur[2] + yb.w * ur[3];
+          const float szi = yb.x * ui[0] + yb.y * ui[1] + yb.z * ui[2] + yb.w * ui[3];
+          g0r += dxa * sr;
+          g0i += dxa * si;
+          g1r += xa * syr;
+          g1i += xa * syi;
+          g2r += xa * szr;
+          g2i += xa * szi;
+        }
+        gr = grn;
+        gi = gin;
+        xa = xan;
+        dxa = dxan;
       }
-      gr = grn;
-      gi = gin;
-      xa = xan;
-      dxa = dxan;
+    };
+    if constexpr (MT == 4 && P == 16) {
+      // a 9-point stencil starting at l reaches the K-blocks l / 4 .. l / 4 + 2; the
+      // group needs all four only when its z offsets lie on both sides of 4
+      const bool lo = __ballot(lk == 2 && ok && l < 4) != 0;
+      const bool hi = __ballot(lk == 2 && ok && l >= 4) != 0;
+      if (lo && hi) plane_loop(IntC<0>{}, IntC<4>{});
+      else if (lo) plane_loop(IntC<0>{}, IntC<3>{});
+      else plane_loop(IntC<1>{}, IntC<3>{});
+    } else {
+      plane_loop(IntC<0>{}, IntC<4>{});
     }
     // sum over the four lane groups (rows b = 4 lk + e)
     auto red = [](float v) {

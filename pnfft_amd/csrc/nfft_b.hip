@@ -38,12 +38,21 @@ __global__ __launch_bounds__(kB) void bin_keys_k(const R* __restrict__ x, long M
                                                  unsigned* keys, int* vals) {
   const long j = (long)blockIdx.x * kB + threadIdx.x;
   if (j >= M) return;
-  unsigned long long tile = 0, in = 0;
+  unsigned long long tile = 0;
+  unsigned l[3];
   for (int t = 0; t < 3; t++) {
     const NodeAxis<R> a = node_axis<R>(x[3 * j + t], g.n[t], g.m, g.shift);
     tile = tile * T.ntile[t] + (unsigned)(a.start / T.ts[t]);
-    in = in * T.ts[t] + (unsigned)(a.start % T.ts[t]);
+    l[t] = (unsigned)(a.start % T.ts[t]);
   }
+  // cell order inside a tile: runs of q = min(TS2, 4) cells along axis 2
+  // outermost, then axis 0, axis 1 and the cell within the run, so that
+  // consecutive nodes of a tile share the quarter of axis 2 their stencils start
+  // in (the matrix-core gather then skips one of its four K-blocks for most
+  // 16-node groups)
+  const unsigned q = T.ts[2] < 4 ? (unsigned)T.ts[2] : 4u;
+  const unsigned long long in =
+      ((unsigned long long)(l[2] / q) * (T.ts[0] * T.ts[1]) + l[0] * T.ts[1] + l[1]) * q + l[2] % q;
   keys[j] = (unsigned)(tile * (unsigned long long)(T.ts[0] * T.ts[1] * T.ts[2]) + in);
   vals[j] = (int)j;
 }

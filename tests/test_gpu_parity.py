@@ -228,7 +228,11 @@ def _tile_keys(x, n, m, precision):
     nt_ = [(a + b - 1) // b for a, b in zip(n, ts)]
     st = (np.floor(np.asarray(n, float) * x) - m).astype(np.int64) % np.asarray(n)
     tile = ((st[:, 0] // ts[0]) * nt_[1] + st[:, 1] // ts[1]) * nt_[2] + st[:, 2] // ts[2]
-    inn = ((st[:, 0] % ts[0]) * ts[1] + st[:, 1] % ts[1]) * ts[2] + st[:, 2] % ts[2]
+    # cell order inside a tile: runs of q = min(ts2, 4) cells along axis 2 outermost, then
+    # axis 0, axis 1 and the cell within the run
+    q = min(ts[2], 4)
+    l0, l1, l2 = st[:, 0] % ts[0], st[:, 1] % ts[1], st[:, 2] % ts[2]
+    inn = ((l2 // q) * (ts[0] * ts[1]) + l0 * ts[1] + l1) * q + l2 % q
     return (tile * (ts[0] * ts[1] * ts[2]) + inn).astype(np.int64)
 
 
