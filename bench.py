@@ -344,6 +344,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload's node count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-recompute", action="store_true",
+                    help="skip the run with the windows evaluated in the B kernels (value_window_recompute null)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22,
                     help="nodes of the CPU baseline's B stages (a quarter of the headline workload)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -397,8 +399,10 @@ def main():
         log(f"  rel-l2 vs FP64: {acc}")
 
     # the same steps with the windows evaluated in the B kernels instead of read from tables
-    recompute = run_plan(w, precision, args.steps, 1, x_np, tables=False)
-    log(f"  {precision}, windows recomputed: {recompute['ms_per_step']:.3f} ms/step")
+    recompute = None
+    if not args.no_recompute:
+        recompute = run_plan(w, precision, args.steps, 1, x_np, tables=False)
+        log(f"  {precision}, windows recomputed: {recompute['ms_per_step']:.3f} ms/step")
 
     steps = args.steps
     ab = algorithmic_bytes(w, precision)
@@ -444,7 +448,7 @@ def main():
         # the same rate with the node binning and window tables redone for every step
         "value_incl_setup": w["M"] / ((res["ms_per_step"] + res["setup_ms"]) * 1e-3),
         # the rate with the windows evaluated in the B kernels (no window tables; bins made once)
-        "value_window_recompute": w["M"] / (recompute["ms_per_step"] * 1e-3),
+        "value_window_recompute": w["M"] / (recompute["ms_per_step"] * 1e-3) if recompute else None,
         "fp64_ms_per_step": fp64["ms_per_step"] if fp64 else None,
         "roofline": roofline,
         "cpu_baseline": cpu,

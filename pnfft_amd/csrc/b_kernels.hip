@@ -317,12 +317,12 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
 // ---------------------------------------------------------------------------
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
-// LDS rows of 16 floats (subgrid rows b and the nodes' window rows), element v
-// of row r at v ^ 4 sw(r), sw(r) = (r >> 2) & 3: a float4 chunk k of rows r =
-// 0..15 (read by one lane group) and a scalar of rows 0..15 fall in distinct
-// banks.  Lane group k supplies the MFMA K-slices c = 4k + e, e = 0..3, of the
-// subgrid row and the z-window row as one float4 (chunk k), and reads the rows
-// b = 4k + e of its accumulator with the y-window chunk k.
+// Subgrid rows of 16 floats in LDS, element v of row r at v ^ 4 sw(r),
+// sw(r) = (r >> 2) & 3: the float4 chunks k of rows r = 0..15 (one per lane
+// group) fall in distinct banks.  Lane group k supplies the MFMA K-slices
+// c = 4e + k of the e-th product of a plane from its subgrid chunk (cells in
+// the order kp below) and from the z-window row, and reads the rows b = 4k + e
+// of its accumulator with the y-window chunk k.
 __device__ __forceinline__ int sw4(int r) { return (r >> 2) & 3; }
 __device__ __forceinline__ int rpos(int r, int v) { return v ^ (sw4(r) << 2); }
 
@@ -342,6 +342,15 @@ __device__ __forceinline__ float comp4(const float4& v, int e) {
   return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
 }
 
+// OR of v over each row of 16 lanes (DPP row shifts); lane 15 of a row holds it
+__device__ __forceinline__ unsigned row_or16(unsigned v) {
+  v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  return v;
+}
+
 // LDS writes of one lane become visible to the other lanes of its wave
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -356,10 +365,14 @@ struct GatherMfmaShape {
   static constexpr int CM = 2 * MT + 1, TS = std_tile(MT, false), P = TS + 2 * MT;
   static constexpr bool WF = (MODE == B_F || MODE == B_FGRAD);
   static constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
-  static constexpr int ROWS = WG ? 6 : 3;  // window rows per node: x, y, z (and x', y', z')
+  // window rows of a wave's 16 nodes: x and z rows of 17 floats (read as
+  // scalars), y rows of 20 floats (read as float4 chunks), at X 0, Y YOFF,
+  // Z ZOFF; the derivative rows follow at ROWSZ / 2.  The odd and the 20-float
+  // pitches keep the rows' stores and reads in distinct banks without a swizzle.
+  static constexpr int YOFF = 16 * 17, ZOFF = YOFF + 16 * 20, ROWSZ = (WG ? 2 : 1) * (ZOFF + 16 * 17);
   static constexpr int WAVES = WG ? 2 : 3;  // waves per SIMD the kernel is compiled for
   static constexpr int NT = 64 * NW;        // threads per block
-  static constexpr size_t LDS = sizeof(float) * (2 * P * 256 + NW * ROWS * 256);
+  static constexpr size_t LDS = sizeof(float) * (2 * P * 256 + NW * ROWSZ);
 };
 
 // one lane's share of a node's window data: axis lk of node li of a group
@@ -385,8 +398,7 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
   float* s_gi = s_gr + P * 256;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  // this wave's window rows: row r (0..2: x, y, z; 3..5: x', y', z') of node i at (r * 16 + i) * 16
-  float* rows = s_gi + P * 256 + wave * S::ROWS * 256;
+  float* rows = s_gi + P * 256 + wave * S::ROWSZ;  // this wave's window rows
 
   const int tile = A.torder[first + blockIdx.x];
   const int t2 = tile % A.ntile[2];
@@ -473,10 +485,23 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
   int jn = perm_raw(g + NW < ngroups ? g + NW : g);
   bool jn_ok = g + NW < ngroups && perm_ok(g + NW);
 
+  // this lane's window rows (lane group lk < 3: axis lk of node li)
+  float* const r0 = rows + (lk == 0 ? 0 : (lk == 1 ? S::YOFF + li * 20 : S::ZOFF)) + (lk == 1 ? 0 : li * 17);
+  float* const r1 = r0 + S::ROWSZ / 2;  // the derivative rows (gradient modes)
+
   for (; g < ngroups; g += NW) {
-    // windows of this lane's axis for node li of the group
     const int jg = L.j;
-    float w[CM], dw[CM];
+    {
+      // the wave's rows are zeroed with consecutive lanes on consecutive 16-byte
+      // chunks; the window values below are later stores of the same wave
+      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < (S::ROWSZ / 4 + 63) / 64; k++)
+        if (lane + 64 * k < S::ROWSZ / 4) reinterpret_cast<float4*>(rows)[lane + 64 * k] = z4;
+    }
+    // windows of this lane's axis for node li of the group, written at the
+    // node's offset l in the tile (a node outside its tile on this axis leaves
+    // the row zero, so it adds nothing; it is listed for the stale kernel below)
     int l = 0;
     bool bad = false;
     if (lk < 3 && jg >= 0) {
@@ -484,54 +509,36 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
       l = local_off(na.start, ot, TS, nt);
       bad = l < 0;
       if (tab && L.x == L.px) {
+        if (!bad) {
 #pragma unroll
-        for (int s = 0; s < CM; s++) {
-          w[s] = L.psi[s];
-          dw[s] = WG ? L.dpsi[s] : 0.f;
+          for (int s = 0; s < CM; s++) {
+            r0[l + s] = L.psi[s];
+            if (WG) r1[l + s] = L.dpsi[s];
+          }
         }
       } else {
+        float w[CM], dw[CM];
         // inlined: an out-of-line call would keep w and dw in scratch memory
         [[clang::always_inline]] window_axis_s<float, CM, WG>(A.g.win, na.frac, (float)nt, bt, w, dw);
+        if (!bad) {
 #pragma unroll
-        for (int s = 0; s < CM; s++) {
-          w[s] *= wst;
-          if (WG) dw[s] *= wst;
+          for (int s = 0; s < CM; s++) {
+            r0[l + s] = w[s] * wst;
+            if (WG) r1[l + s] = dw[s] * wst;
+          }
         }
       }
     }
     const unsigned long long badm = __ballot(bad);
     const unsigned stale = (unsigned)(badm | (badm >> 16) | (badm >> 32)) & 0xFFFFu;
     const bool ok = jg >= 0 && !((stale >> li) & 1u);
-    // planes the group's stencils reach, [amin, amax]: from the x lanes' offsets (uniform ballots)
-    int amin = P, amax = -1;
-#pragma unroll
-    for (int v = 0; v < TS; v++) {
-      if (__ballot(lk == 0 && ok && l == v)) {
-        amin = min(amin, v);
-        amax = max(amax, v + CM - 1);
-      }
-    }
-    {
-      // the wave's rows are zeroed with consecutive lanes on consecutive 16-byte
-      // chunks (one lane per row would put the lanes 64 bytes apart, a few banks);
-      // the window values below are later stores of the same wave, so they land after
-      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-      for (int k = 0; k < S::ROWS * 256 / 4 / 64; k++) reinterpret_cast<float4*>(rows)[lane + 64 * k] = z4;
-    }
-    if (lk < 3) {
-      float* r0 = rows + (lk * 16 + li) * 16;
-      float* r1 = rows + ((3 + lk) * 16 + li) * 16;
-      if (ok) {
-        const int sx = sw4(li) << 2;
-#pragma unroll
-        for (int s = 0; s < CM; s++) {
-          const int pos = (l + s) ^ sx;
-          r0[pos] = w[s];
-          if (WG) r1[pos] = dw[s];
-        }
-      }
-    }
+    // the offsets of the group's nodes per axis, as bit masks: an OR over the
+    // 16 lanes of each lane group (lane 15 of the x group, lane 47 of the z group)
+    const unsigned om = row_or16((lk < 3 && ok) ? (1u << l) : 0u);
+    const unsigned xm = (unsigned)__builtin_amdgcn_readlane((int)om, 15);
+    const unsigned zm = (unsigned)__builtin_amdgcn_readlane((int)om, 47);
+    // planes the group's stencils reach, [amin, amax]
+    const int amin = xm ? __builtin_ctz(xm) : P, amax = xm ? 31 - __builtin_clz(xm) + CM - 1 : -1;
     wave_sync();
 
     // loads for the wave's next group, in flight during the matrix-core phase
@@ -541,19 +548,17 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
     const int jnn = perm_raw(g + 2 * NW < ngroups ? g + 2 * NW : g);
     const bool jnn_ok = g + 2 * NW < ngroups && perm_ok(g + 2 * NW);
 
-    const float* xr = rows + li * 16;
-    const float* dxr = rows + (48 + li) * 16;
+    const float* xr = rows + li * 17;
+    const float* dxr = xr + S::ROWSZ / 2;
     const int ch = 4 * (lk ^ sw4(li));
-    const float4 yb = *reinterpret_cast<const float4*>(rows + (16 + li) * 16 + ch);  // Y_j[4 lk + e]
+    const float4 yb = *reinterpret_cast<const float4*>(rows + S::YOFF + li * 20 + 4 * lk);  // Y_j[4 lk + e]
     // Z_j[4 e + lk] in component e: the K-slice of this lane group in the e-th product
-    auto zread = [&](const float* row) {
-      return make_float4(row[rpos(li, lk)], row[rpos(li, 4 + lk)], row[rpos(li, 8 + lk)], row[rpos(li, 12 + lk)]);
-    };
-    const float4 zc = zread(rows + (32 + li) * 16);
+    auto zread = [&](const float* row) { return make_float4(row[lk], row[4 + lk], row[8 + lk], row[12 + lk]); };
+    const float4 zc = zread(rows + S::ZOFF + li * 17);
     float4 dyb = yb, dzc = zc;
     if (WG) {
-      dyb = *reinterpret_cast<const float4*>(rows + (64 + li) * 16 + ch);
-      dzc = zread(rows + (80 + li) * 16);
+      dyb = *reinterpret_cast<const float4*>(rows + S::ROWSZ / 2 + S::YOFF + li * 20 + 4 * lk);
+      dzc = zread(rows + S::ROWSZ / 2 + S::ZOFF + li * 17);
     }
     float fr = 0.f, fi = 0.f, g0r = 0.f, g0i = 0.f, g1r = 0.f, g1i = 0.f, g2r = 0.f, g2i = 0.f;
     // plane a + 1's operands are read while plane a is on the matrix cores;
@@ -563,8 +568,8 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
       auto ld_plane = [&](int a, float4& gr, float4& gi, float& xa, float& dxa) {
         gr = *reinterpret_cast<const float4*>(s_gr + (a * 16 + li) * 16 + ch);
         gi = *reinterpret_cast<const float4*>(s_gi + (a * 16 + li) * 16 + ch);
-        xa = xr[rpos(li, a)];
-        dxa = WG ? dxr[rpos(li, a)] : 0.f;
+        xa = xr[a];
+        dxa = WG ? dxr[a] : 0.f;
       };
       float4 gr, gi;
       float xa, dxa;
@@ -610,8 +615,7 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
     if constexpr (MT == 4 && P == 16) {
       // a 9-point stencil starting at l reaches the K-blocks l / 4 .. l / 4 + 2; the
       // group needs all four only when its z offsets lie on both sides of 4
-      const bool lo = __ballot(lk == 2 && ok && l < 4) != 0;
-      const bool hi = __ballot(lk == 2 && ok && l >= 4) != 0;
+      const bool lo = (zm & 0x0Fu) != 0, hi = (zm & 0xF0u) != 0;
       if (lo && hi) plane_loop(IntC<0>{}, IntC<4>{});
       else if (lo) plane_loop(IntC<0>{}, IntC<3>{});
       else plane_loop(IntC<1>{}, IntC<3>{});
