@@ -574,42 +574,60 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
       float4 gr, gi;
       float xa, dxa;
       ld_plane(amax >= amin ? amin : 0, gr, gi, xa, dxa);
+      // sums over all planes with X_j[a] folded into the products' second operand:
+      // T = sum_a sum_c G[a][b][c] X_j[a] Z_j[c] (f mode), or with Z' (the z
+      // gradient in the gradient modes); their y sums follow after the loop
+      mfma_f4 vr = {0.f, 0.f, 0.f, 0.f}, vi = vr;
       for (int a = amin; a <= amax; a++) {
         float4 grn, gin;
         float xan, dxan;
         ld_plane(a < amax ? a + 1 : a, grn, gin, xan, dxan);
-        mfma_f4 tr = {0.f, 0.f, 0.f, 0.f}, ti = tr;
-#pragma unroll
-        for (int e = K0; e < K0 + NK; e++) {
-          tr = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), comp4(zc, e), tr, 0, 0, 0);
-          ti = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), comp4(zc, e), ti, 0, 0, 0);
-        }
-        const float sr = yb.x * tr[0] + yb.y * tr[1] + yb.z * tr[2] + yb.w * tr[3];
-        const float si = yb.x * ti[0] + yb.y * ti[1] + yb.z * ti[2] + yb.w * ti[3];
-        fr += xa * sr;
-        fi += xa * si;
-        if (WG) {
-          mfma_f4 ur = {0.f, 0.f, 0.f, 0.f}, ui = ur;
+        if (!WG) {
 #pragma unroll
           for (int e = K0; e < K0 + NK; e++) {
-            ur = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), comp4(dzc, e), ur, 0, 0, 0);
-            ui = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), comp4(dzc, e), ui, 0, 0, 0);
+            const float bz = xa * comp4(zc, e);
+            vr = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), bz, vr, 0, 0, 0);
+            vi = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), bz, vi, 0, 0, 0);
+          }
+        } else {
+          mfma_f4 tr = {0.f, 0.f, 0.f, 0.f}, ti = tr;
+#pragma unroll
+          for (int e = K0; e < K0 + NK; e++) {
+            tr = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), comp4(zc, e), tr, 0, 0, 0);
+            ti = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), comp4(zc, e), ti, 0, 0, 0);
+          }
+#pragma unroll
+          for (int e = K0; e < K0 + NK; e++) {
+            const float bz = xa * comp4(dzc, e);
+            vr = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), bz, vr, 0, 0, 0);
+            vi = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), bz, vi, 0, 0, 0);
+          }
+          const float sr = yb.x * tr[0] + yb.y * tr[1] + yb.z * tr[2] + yb.w * tr[3];
+          const float si = yb.x * ti[0] + yb.y * ti[1] + yb.z * ti[2] + yb.w * ti[3];
+          if (S::WF) {
+            fr += xa * sr;
+            fi += xa * si;
           }
           const float syr = dyb.x * tr[0] + dyb.y * tr[1] + dyb.z * tr[2] + dyb.w * tr[3];
           const float syi = dyb.x * ti[0] + dyb.y * ti[1] + dyb.z * ti[2] + dyb.w * ti[3];
-          const float szr = yb.x * ur[0] + yb.y * ur[1] + yb.z * ur[2] + yb.w * ur[3];
-          const float szi = yb.x * ui[0] + yb.y * ui[1] + yb.z * ui[2] + yb.w * ui[3];
           g0r += dxa * sr;
           g0i += dxa * si;
           g1r += xa * syr;
           g1i += xa * syi;
-          g2r += xa * szr;
-          g2i += xa * szi;
         }
         gr = grn;
         gi = gin;
         xa = xan;
         dxa = dxan;
+      }
+      const float vsr = yb.x * vr[0] + yb.y * vr[1] + yb.z * vr[2] + yb.w * vr[3];
+      const float vsi = yb.x * vi[0] + yb.y * vi[1] + yb.z * vi[2] + yb.w * vi[3];
+      if (!WG) {
+        fr = vsr;
+        fi = vsi;
+      } else {
+        g2r = vsr;
+        g2i = vsi;
       }
     };
     if constexpr (MT == 4 && P == 16) {
