@@ -215,9 +215,9 @@ __global__ __launch_bounds__((ColsShape<R, MT, MODE>::NT)) void scatter_cols_k(B
 // v_mfma_f32_16x16x4f32 per plane, K = 4 terms, per real and imaginary part.
 // The four waves of a block own the planes a = w + 4q (q = 0..3, interleaved so
 // that every wave meets a node's stencil on 2-3 of its planes); a plane is
-// skipped when none of the quad's four terms reaches it (sorted nodes of a tile
-// share their plane offset most of the time).  The FP32 accumulators are added
-// into FP64 totals after every batch of terms; the totals go to the tile's slot.
+// skipped for a whole batch when none of the batch's stencils reaches it
+// (sorted nodes of a tile share their plane offset most of the time).  The
+// FP32 accumulators sum over all nodes of the tile and go to the tile's slot.
 // ---------------------------------------------------------------------------
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
@@ -233,15 +233,28 @@ struct MfmaShape {
   static constexpr int NBUF = V == 1 ? 2 : 1;
 };
 
-// Window rows of the batch, 16 floats per term v, with positions swizzled per
-// row so that the rows phase, where lane q writes the same position of rows
-// q V + u, spreads its stores over the banks:
-// x rows: plane a in float4 chunk (a & 3) ^ ((v >> 1) & 3), entry a >> 2
-// (wave a & 3 reads its four planes as one float4); y and z rows: position p
-// at p ^ ((v >> 1) & 15).
-__device__ __forceinline__ int xchunk(int v, int w) { return w ^ ((v >> 1) & 3); }
-__device__ __forceinline__ int xpos(int v, int a) { return (xchunk(v, a & 3) << 2) | (a >> 2); }
-__device__ __forceinline__ int yzpos(int v, int p) { return p ^ ((v >> 1) & 15); }
+// Window rows of the batch: term v's row of axis t holds the window at the
+// cells l .. l + 2m of the node's stencil (zero elsewhere), at the odd pitch
+// kRowPitch, so that the rows phase, where lane q writes the same position of
+// rows q V + u, and the matrix-core phase, where the lane groups read four
+// consecutive rows, both spread over the banks.
+constexpr int kRowPitch = 17;
+
+// OR of v over each row of 16 lanes (DPP row shifts); lane 15 of a row holds it
+__device__ __forceinline__ unsigned row_or16(unsigned v) {
+  v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  return v;
+}
+
+// OR of v over the 64 lanes of a wave, as a wave-uniform value
+__device__ __forceinline__ unsigned wave_or(unsigned v) {
+  v = row_or16(v);
+  return (unsigned)(__builtin_amdgcn_readlane((int)v, 15) | __builtin_amdgcn_readlane((int)v, 31) |
+                    __builtin_amdgcn_readlane((int)v, 47) | __builtin_amdgcn_readlane((int)v, 63));
+}
 
 // one thread's share of a node's data for the rows phase: axis `role` (0..2)
 // or the complex weights (role 3)
@@ -267,11 +280,14 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
   // window rows of a batch; with two buffers the rows of batch b + 1 are written
   // while batch b is on the matrix cores (one barrier per batch)
   constexpr int NBUF = MS::NBUF;
-  __shared__ __align__(16) float s_x[NBUF][NV][16];  // X_v[a] at xpos(v, a)
-  __shared__ __align__(16) float s_y[NBUF][NV][16];  // Y_v[b] at yzpos(v, b), zero outside the stencil
-  __shared__ __align__(16) float s_z[NBUF][NV][16];  // Z_v[c] at yzpos(v, c)
+  constexpr int RP = kRowPitch;
+  __shared__ __align__(16) float s_x[NBUF][NV][RP];  // X_v[a], zero outside the stencil
+  __shared__ __align__(16) float s_y[NBUF][NV][RP];  // Y_v[b]
+  __shared__ __align__(16) float s_z[NBUF][NV][RP];  // Z_v[c]
   __shared__ float2 s_w[NBUF][NV];                   // W_v
   __shared__ unsigned char s_bad[NBUF][3][NB];       // node outside the tile on axis t
+  __shared__ unsigned s_pm[NBUF];                    // planes the batch's stencils reach (bit a)
+  static_assert((NV * RP) % 4 == 0, "row buffers of whole 16-byte chunks");
 
   const int tile = A.torder[blockIdx.x];
   const int t2 = tile % A.ntile[2];
@@ -321,15 +337,13 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
     }
   };
 
-  mfma_f4 cre[4], cim[4];  // plane wave + 4q: D[b = 4 lk + e][c = li]
-  double tre[4][4], tim[4][4];
+  // plane wave + 4q: G[b = 4 lk + e][c = li], summed in FP32 over the tile's
+  // nodes (the single-precision reference sums its adjoint in float as well)
+  mfma_f4 cre[4], cim[4];
 #pragma unroll
   for (int qq = 0; qq < 4; qq++)
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      cre[qq][e] = cim[qq][e] = 0.f;
-      tre[qq][e] = tim[qq][e] = 0.0;
-    }
+    for (int e = 0; e < 4; e++) cre[qq][e] = cim[qq][e] = 0.f;
 
   ScatterLoad<CM, WF, WG> L;
   {
@@ -366,24 +380,29 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
           }
         }
       }
-      float(*srow)[16] = ax == 0 ? s_x[bf] : (ax == 1 ? s_y[bf] : s_z[bf]);
+      float(*srow)[RP] = ax == 0 ? s_x[bf] : (ax == 1 ? s_y[bf] : s_z[bf]);
       // the wave's rows are zeroed with consecutive lanes on consecutive 16-byte
       // chunks; the window values below are later stores of the same wave
       const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-      for (int k = 0; k < NV * 16 / 4 / 64; k++) reinterpret_cast<float4*>(&srow[0][0])[q + 64 * k] = z4;
+      for (int k = 0; k < (NV * RP / 4 + 63) / 64; k++)
+        if (q + 64 * k < NV * RP / 4) reinterpret_cast<float4*>(&srow[0][0])[q + 64 * k] = z4;
 #pragma unroll
       for (int u = 0; u < V; u++) {
-        const int v = q * V + u;
-        float* row = srow[v];
+        float* row = srow[q * V + u];
         if (L.j >= 0 && !bad) {
           // term u differentiates axis ax when it is the gradient term of that axis
           const bool d = WG && u == (WF ? 1 : 0) + ax;
 #pragma unroll
-          for (int s = 0; s < CM; s++) row[ax == 0 ? xpos(v, l + s) : yzpos(v, l + s)] = d ? dw[s] : w[s];
+          for (int s = 0; s < CM; s++) row[l + s] = d ? dw[s] : w[s];
         }
       }
       s_bad[bf][ax][q] = bad ? 1 : 0;
+      if (role == 0) {
+        // the planes of the batch: the union of its nodes' x stencils
+        const unsigned pm = wave_or((L.j >= 0 && !bad) ? (((1u << CM) - 1u) << l) : 0u);
+        if (q == 0) s_pm[bf] = pm;
+      }
     } else {
       int v = q * V;
       if (WF) s_w[bf][v++] = L.j >= 0 ? make_float2(L.f.x, L.f.y) : make_float2(0.f, 0.f);
@@ -422,30 +441,28 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
     const int jnn = perm_raw(bnext + NB < end ? bnext + NB : base);
     const bool jnn_ok = perm_ok(bnext + NB);
     const int nq = (nb * V + 3) >> 2;
+    // this wave's planes a = wave + 4 qq that the batch reaches (bit qq)
+    const unsigned pmb = (unsigned)__builtin_amdgcn_readfirstlane((int)s_pm[bf]) >> wave;
+    const unsigned act = (pmb & 1u) | ((pmb >> 3) & 2u) | ((pmb >> 6) & 4u) | ((pmb >> 9) & 8u);
+    const float* xrow = &s_x[bf][lk][wave];
+    const float* yrow = &s_y[bf][lk][li];
+    const float* zrow = &s_z[bf][lk][li];
+    const float2* wrow = &s_w[bf][lk];
+#pragma unroll 4
     for (int q4 = 0; q4 < nq; q4++) {
-      const int v = 4 * q4 + lk;
-      const float4 x = reinterpret_cast<const float4*>(s_x[bf][v])[xchunk(v, wave)];
-      const float y = s_y[bf][v][yzpos(v, li)], z = s_z[bf][v][yzpos(v, li)];
-      const float2 wv = s_w[bf][v];
+      // term v = 4 q4 + lk of this lane group
+      const float y = yrow[4 * RP * q4], z = zrow[4 * RP * q4];
+      const float2 wv = wrow[4 * q4];
       const float bre = wv.x * z, bim = wv.y * z;
-      const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
       for (int qq = 0; qq < 4; qq++) {
-        if (__any(xs[qq] != 0.f)) {
-          const float a = xs[qq] * y;
+        if (act & (1u << qq)) {
+          const float a = xrow[4 * RP * q4 + 4 * qq] * y;
           cre[qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bre, cre[qq], 0, 0, 0);
           cim[qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bim, cim[qq], 0, 0, 0);
         }
       }
     }
-#pragma unroll
-    for (int qq = 0; qq < 4; qq++)
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        tre[qq][e] += (double)cre[qq][e];
-        tim[qq][e] += (double)cim[qq][e];
-        cre[qq][e] = cim[qq][e] = 0.f;
-      }
     __syncthreads();
     L = Ln;
     jn = jnn;
@@ -459,7 +476,7 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       const int b = 4 * lk + e;
-      if (a < P && b < P && li < P) slot[(a * P + b) * P + li] = cmk<float>((float)tre[qq][e], (float)tim[qq][e]);
+      if (a < P && b < P && li < P) slot[(a * P + b) * P + li] = cmk<float>(cre[qq][e], cim[qq][e]);
     }
   }
 }
