@@ -323,7 +323,7 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 // c = 4e + k of the e-th product of a plane from its subgrid chunk (cells in
 // the order kp below) and from the z-window row, and reads the rows b = 4k + e
 // of its accumulator with the y-window chunk k.
-__device__ __forceinline__ int sw4(int r) { return (r >> 2) & 3; }
+__device__ __forceinline__ int sw4(int r) { return (r >> 1) & 3; }
 __device__ __forceinline__ int rpos(int r, int v) { return v ^ (sw4(r) << 2); }
 
 // Position of cell c (axis 2) in a subgrid row: c = 4 e + k at 4 k + e.  The

@@ -234,11 +234,12 @@ struct MfmaShape {
 };
 
 // Window rows of the batch: term v's row of axis t holds the window at the
-// cells l .. l + 2m of the node's stencil (zero elsewhere), at the odd pitch
-// kRowPitch, so that the rows phase, where lane q writes the same position of
-// rows q V + u, and the matrix-core phase, where the lane groups read four
-// consecutive rows, both spread over the banks.
-constexpr int kRowPitch = 17;
+// cells l .. l + 2m of the node's stencil (zero elsewhere) and starts at
+// 16 v + v / 2.  The half-word step per row pair keeps both access patterns in
+// distinct banks: the rows phase, where lane q writes the same position of row
+// q (V = 1), and the matrix-core phase, where 32 lanes read two consecutive
+// rows (v even, v + 1).
+__device__ __forceinline__ constexpr int row_at(int v) { return 16 * v + (v >> 1); }
 
 // OR of v over each row of 16 lanes (DPP row shifts); lane 15 of a row holds it
 __device__ __forceinline__ unsigned row_or16(unsigned v) {
@@ -280,14 +281,14 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
   // window rows of a batch; with two buffers the rows of batch b + 1 are written
   // while batch b is on the matrix cores (one barrier per batch)
   constexpr int NBUF = MS::NBUF;
-  constexpr int RP = kRowPitch;
-  __shared__ __align__(16) float s_x[NBUF][NV][RP];  // X_v[a], zero outside the stencil
-  __shared__ __align__(16) float s_y[NBUF][NV][RP];  // Y_v[b]
-  __shared__ __align__(16) float s_z[NBUF][NV][RP];  // Z_v[c]
-  __shared__ float2 s_w[NBUF][NV];                   // W_v
-  __shared__ unsigned char s_bad[NBUF][3][NB];       // node outside the tile on axis t
-  __shared__ unsigned s_pm[NBUF];                    // planes the batch's stencils reach (bit a)
-  static_assert((NV * RP) % 4 == 0, "row buffers of whole 16-byte chunks");
+  constexpr int RS = row_at(NV);                   // floats per row buffer
+  __shared__ __align__(16) float s_x[NBUF][RS];    // X_v[a] at row_at(v) + a, zero outside the stencil
+  __shared__ __align__(16) float s_y[NBUF][RS];    // Y_v[b]
+  __shared__ __align__(16) float s_z[NBUF][RS];    // Z_v[c]
+  __shared__ float2 s_w[NBUF][NV];                 // W_v
+  __shared__ unsigned char s_bad[NBUF][3][NB];     // node outside the tile on axis t
+  __shared__ unsigned s_pm[NBUF];                  // planes the batch's stencils reach (bit a)
+  static_assert(RS % 4 == 0, "row buffers of whole 16-byte chunks");
 
   const int tile = A.torder[blockIdx.x];
   const int t2 = tile % A.ntile[2];
@@ -380,16 +381,16 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
           }
         }
       }
-      float(*srow)[RP] = ax == 0 ? s_x[bf] : (ax == 1 ? s_y[bf] : s_z[bf]);
+      float* srow = ax == 0 ? s_x[bf] : (ax == 1 ? s_y[bf] : s_z[bf]);
       // the wave's rows are zeroed with consecutive lanes on consecutive 16-byte
       // chunks; the window values below are later stores of the same wave
       const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-      for (int k = 0; k < (NV * RP / 4 + 63) / 64; k++)
-        if (q + 64 * k < NV * RP / 4) reinterpret_cast<float4*>(&srow[0][0])[q + 64 * k] = z4;
+      for (int k = 0; k < (RS / 4 + 63) / 64; k++)
+        if (q + 64 * k < RS / 4) reinterpret_cast<float4*>(srow)[q + 64 * k] = z4;
 #pragma unroll
       for (int u = 0; u < V; u++) {
-        float* row = srow[q * V + u];
+        float* row = srow + row_at(q * V + u);
         if (L.j >= 0 && !bad) {
           // term u differentiates axis ax when it is the gradient term of that axis
           const bool d = WG && u == (WF ? 1 : 0) + ax;
@@ -444,20 +445,20 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(
     // this wave's planes a = wave + 4 qq that the batch reaches (bit qq)
     const unsigned pmb = (unsigned)__builtin_amdgcn_readfirstlane((int)s_pm[bf]) >> wave;
     const unsigned act = (pmb & 1u) | ((pmb >> 3) & 2u) | ((pmb >> 6) & 4u) | ((pmb >> 9) & 8u);
-    const float* xrow = &s_x[bf][lk][wave];
-    const float* yrow = &s_y[bf][lk][li];
-    const float* zrow = &s_z[bf][lk][li];
+    // term v = 4 q4 + lk of this lane group: row_at(v) = row_at(lk) + 66 q4
+    const float* xrow = s_x[bf] + row_at(lk) + wave;
+    const float* yrow = s_y[bf] + row_at(lk) + li;
+    const float* zrow = s_z[bf] + row_at(lk) + li;
     const float2* wrow = &s_w[bf][lk];
 #pragma unroll 4
     for (int q4 = 0; q4 < nq; q4++) {
-      // term v = 4 q4 + lk of this lane group
-      const float y = yrow[4 * RP * q4], z = zrow[4 * RP * q4];
+      const float y = yrow[66 * q4], z = zrow[66 * q4];
       const float2 wv = wrow[4 * q4];
       const float bre = wv.x * z, bim = wv.y * z;
 #pragma unroll
       for (int qq = 0; qq < 4; qq++) {
         if (act & (1u << qq)) {
-          const float a = xrow[4 * RP * q4 + 4 * qq] * y;
+          const float a = xrow[66 * q4 + 4 * qq] * y;
           cre[qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bre, cre[qq], 0, 0, 0);
           cim[qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bim, cim[qq], 0, 0, 0);
         }
