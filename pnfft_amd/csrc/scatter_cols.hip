@@ -271,7 +271,8 @@ struct ScatterLoad {
 };
 
 template <int MT, int MODE>
-__global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES)) void scatter_mfma_k(BArgs<float> A) {
+__global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES + (MODE == B_F && MT <= 4 ? 1 : 0))) void scatter_mfma_k(
+    BArgs<float> A) {
   using S = ColsShape<float, MT, MODE>;
   using MS = MfmaShape<MODE>;
   constexpr int CM = S::CM, P = S::P, TS = S::TS, V = MS::V, NB = MS::NB, NV = MS::NV;
