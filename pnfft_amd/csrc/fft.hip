@@ -133,6 +133,22 @@ constexpr int cols_ept() {
 template <class R, int L, bool NARROW>
 using ColsShape = PassShape<R, L, cols_threads<R, L>(), cols_ept<R, L, NARROW>()>;
 
+// The wide 512-point FP32 pass (32 pencils, 512 threads) makes its two LDS
+// round trips in two halves of the pencil positions, so a block holds 256
+// positions (68 KB) and a CU two blocks: one block's loads and stores overlap
+// the other's LDS work.
+template <class R, int L, bool NARROW>
+constexpr bool cols_split() {
+  using S = ColsShape<R, L, NARROW>;
+  return !NARROW && sizeof(R) == 4 && L == 512 && S::C == 32 && cols_threads<R, L>() == 512 && S::EPT == 32;
+}
+
+template <class R, int L, bool NARROW>
+constexpr size_t cols_lds() {
+  using S = ColsShape<R, L, NARROW>;
+  return cols_split<R, L, NARROW>() ? S::LDS / 2 : S::LDS;
+}
+
 // radix of the stage that starts at span Ns (8 while possible, then 4 or 2)
 template <int L, int Ns>
 struct StageRadix {
@@ -389,7 +405,8 @@ __global__ __launch_bounds__(256) void fft_rows512(PassArgs<R> P) {
 // stage runs on the loaded registers and the last stage stores straight to
 // HBM, so only the middle stages touch LDS.
 template <class R, int L, bool NARROW>
-__global__ __launch_bounds__((cols_threads<R, L>())) void fft_cols(PassArgs<R> P) {
+__global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW>() ? 4 : 1)) void fft_cols(
+    PassArgs<R> P) {
   constexpr int NT = cols_threads<R, L>();
   using S = ColsShape<R, L, NARROW>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -405,8 +422,7 @@ __global__ __launch_bounds__((cols_threads<R, L>())) void fft_cols(PassArgs<R> P
   const long out_base = (long)blockIdx.y * P.out_outer;
   Cplx<R> v[S::EPT];
   // ---- first stage (span 1): butterfly (c, j) takes positions j + q L/R1
-#pragma unroll
-  for (int i = 0; i < NB1; i++) {
+  auto load_bf = [&](int i) {
     const int bf = tid + i * NT;
     const int c = bf % S::C, j = bf / S::C;
     const int col = c0 + c;
@@ -416,14 +432,112 @@ __global__ __launch_bounds__((cols_threads<R, L>())) void fft_cols(PassArgs<R> P
       v[i * R1 + q] = cmk<R>(R(0), R(0));
       if (col < P.n_inner && e >= 0) v[i * R1 + q] = P.in[in_base + col + (long)e * P.in_elem];
     }
-  }
-  if (P.backward) {
+  };
+  auto stage1_bf = [&](int i) {
+    if (P.backward) {
 #pragma unroll
-    for (int i = 0; i < S::EPT; i++) v[i] = cconj(v[i]);
-  }
+      for (int q = 0; q < R1; q++) v[i * R1 + q] = cconj(v[i * R1 + q]);
+    }
+    dft<R, R1>(v + i * R1);
+  };
+  if constexpr (!cols_split<R, L, NARROW>()) {
 #pragma unroll
-  for (int i = 0; i < NB1; i++) dft<R, R1>(v + i * R1);
-  if constexpr (NsL > 1) {
+    for (int i = 0; i < NB1; i++) load_bf(i);
+#pragma unroll
+    for (int i = 0; i < NB1; i++) stage1_bf(i);
+  }
+  if constexpr (cols_split<R, L, NARROW>()) {
+    // L = 512 = 8^3, butterflies i = 0..3 of pencil c at j = j0 + 16 i; stage
+    // outputs of butterflies 2h, 2h + 1 and stage inputs q = 4h .. 4h + 3 lie in
+    // the half h of the positions, [256 h, 256 h + 256).  Butterflies 2 and 3
+    // are loaded while half 0 goes through the LDS.
+    constexpr int LD = S::LD, HALF = L / 2;
+    const int c = tid % S::C, j0 = tid / S::C;
+    Cplx<R> u[S::EPT];
+    load_bf(0);
+    load_bf(1);
+    stage1_bf(0);
+    stage1_bf(1);
+    // stage 1 outputs (8 j + q) -> stage 2 inputs (j + 64 q)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      if (h == 1) {
+        stage1_bf(2);
+        stage1_bf(3);
+      }
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++) {
+        const int i = 2 * h + ii, j = j0 + 16 * i;
+#pragma unroll
+        for (int q = 0; q < 8; q++) sh[(8 * j + q - HALF * h) * LD + c] = v[i * 8 + q];
+      }
+      if (h == 0) {
+        load_bf(2);
+        load_bf(3);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int j = j0 + 16 * i;
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+          const int q = 4 * h + qq;
+          u[i * 8 + q] = sh[(j + 64 * q - HALF * h) * LD + c];
+        }
+      }
+      __syncthreads();
+    }
+    // stage 2 (span 8): twiddles tw[8 k q], k = j mod 8; outputs (j - k) 8 + k + 8 q
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int j = j0 + 16 * i, k = j & 7;
+      Cplx<R>* w = u + i * 8;
+#pragma unroll
+      for (int q = 1; q < 8; q++) w[q] = cmul(w[q], P.tw[8 * k * q]);
+      dft8(w);
+    }
+    // stage 2 outputs -> stage 3 inputs (j + 64 q)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++) {
+        const int i = 2 * h + ii, j = j0 + 16 * i, k = j & 7;
+#pragma unroll
+        for (int q = 0; q < 8; q++) sh[((j - k) * 8 + k + 8 * q - HALF * h) * LD + c] = u[i * 8 + q];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int j = j0 + 16 * i;
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+          const int q = 4 * h + qq;
+          v[i * 8 + q] = sh[(j + 64 * q - HALF * h) * LD + c];
+        }
+      }
+      __syncthreads();
+    }
+    // stage 3 (span 64): twiddles tw[j q], outputs j + 64 q stored per butterfly
+    // (its twiddles are loaded where they are used, not all at once)
+    const int col = c0 + c;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int j = j0 + 16 * i;
+      Cplx<R>* w = v + i * 8;
+#pragma unroll
+      for (int q = 1; q < 8; q++) w[q] = cmul(w[q], P.tw[j * q]);
+      dft8(w);
+      if (col < P.n_inner) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int e = store_index(P, j + 64 * q, L);
+          if (e >= 0) P.out[out_base + col + (long)e * P.out_elem] = P.backward ? cconj(w[q]) : w[q];
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    return;
+  } else if constexpr (NsL > 1) {
 #pragma unroll
     for (int i = 0; i < NB1; i++) {
       const int bf = tid + i * NT;
@@ -464,7 +578,8 @@ __global__ __launch_bounds__((cols_threads<R, L>())) void fft_cols(PassArgs<R> P
 #pragma unroll
     for (int q = 0; q < RO; q++) {
       const int e = store_index(P, j + q * stepO, L);
-      if (e >= 0) on_store(P, v[i * RO + q], col, e, out_base + col + (long)e * P.out_elem);
+      // the column passes carry no factors and never accumulate: conjugation only
+      if (e >= 0) P.out[out_base + col + (long)e * P.out_elem] = P.backward ? cconj(v[i * RO + q]) : v[i * RO + q];
     }
   }
 }
@@ -498,8 +613,8 @@ static PassArgs<R> make_args(const FftAxis<R>& ax) {
 template <class R, int L>
 static int pass_attr() {
   constexpr size_t lr = PassShape<R, L>::LDS;
-  constexpr size_t lc = ColsShape<R, L, false>::LDS;
-  constexpr size_t ln = ColsShape<R, L, true>::LDS;
+  constexpr size_t lc = cols_lds<R, L, false>();
+  constexpr size_t ln = cols_lds<R, L, true>();
   if (lr > 64 * 1024 &&
       hipFuncSetAttribute((const void*)fft_rows<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr) !=
           hipSuccess)
@@ -528,10 +643,10 @@ static void pass_launch(bool rows, dim3 grid, const PassArgs<R>& P, hipStream_t 
     constexpr int nt = cols_threads<R, L>();
     // the narrow variant is the one whose pencil count the pass was planned with
     if (ColsShape<R, L, true>::C != ColsShape<R, L, false>::C && P.C == ColsShape<R, L, true>::C) {
-      constexpr size_t lds = ColsShape<R, L, true>::LDS;
+      constexpr size_t lds = cols_lds<R, L, true>();
       hipLaunchKernelGGL((fft_cols<R, L, true>), grid, dim3(nt), lds, s, P);
     } else {
-      constexpr size_t lds = ColsShape<R, L, false>::LDS;
+      constexpr size_t lds = cols_lds<R, L, false>();
       hipLaunchKernelGGL((fft_cols<R, L, false>), grid, dim3(nt), lds, s, P);
     }
   }
