@@ -173,6 +173,12 @@ extern "C" {
    * node_bins: bins the nodes for the plan's grid (if not binned yet) and copies              \
    * the node processing order (node ids sorted stably by bin key, int[M]) and                \
    * the sorted keys (unsigned[M]) to host or device arrays; returns 0 on success.            \
+   * A node's key is tile * T0 T1 T2 + cell: per axis its stencil start                        \
+   * s_t = (floor(n_t x_t) - m) mod n_t gives tile index s_t / T_t (tiles                      \
+   * row-major, axis 2 fastest) and offset l_t = s_t mod T_t; with q = min(T2, 4)              \
+   * the cell is ((l2 / q) T0 T1 + l0 T1 + l1) q + l2 mod q.  T_t is the plan's                \
+   * tile size (8 for m <= 4 and for FP64 m <= 6, else 4; halved where it                     \
+   * does not divide n_t).                                                                     \
    * kernel_times: per-kernel device time (ms, from HIP events on the plan's                   \
    * stream) and launch counts accumulated since the plan was made or                         \
    * reset_timer was called, for the kernel classes PNFFT_KT_* below; copies                  \
