@@ -184,8 +184,9 @@ extern "C" {
    * reset_timer was called, for the kernel classes PNFFT_KT_* below; copies                  \
    * min(n, PNFFT_KT_COUNT) entries and returns PNFFT_KT_COUNT.                               \
    * oversampled_grid: device pointer to the plan's oversampled grid, n0*n1*n2                \
-   * complex values in natural DFT order (index l_t mod n_t, row-major); after                \
-   * trafo it holds F D f_hat (F f_hat with PNFFT_OMIT_DECONV), after adj B^T f.              \
+   * complex values (reals for c2r plans) in natural DFT order (index l_t mod                 \
+   * n_t, row-major); after trafo it holds F D f_hat (F f_hat with                            \
+   * PNFFT_OMIT_DECONV), after adj B^T f.                                                     \
    * The plan's D factors and window values carry reciprocal powers of two per               \
    * axis (2^e and 2^-e, e = exponent of psi(0)), so the grid is 2^e times the               \
    * reference's g2 (exact scaling; B F D and, with PNFFT_OMIT_DECONV, B F are                \

@@ -559,10 +559,10 @@ static PlanT<R>* plan_create(int d, const ptrdiff_t* N, const ptrdiff_t* n, cons
 
   bool ok = hipStreamCreate(&P->stream) == hipSuccess;
   int Ni[3] = {(int)N[0], (int)N[1], (int)N[2]}, ni[3] = {(int)n[0], (int)n[1], (int)n[2]};
-  if (ok) ok = P->fft.init(Ni, ni) == kOk;
+  if (ok) ok = P->fft.init(Ni, ni, c2r) == kOk;
   if (ok) {
     P->d_fhat = dalloc<Cplx<R>>(P->N_total);
-    P->d_grid = dalloc<Cplx<R>>(P->n_total);
+    P->d_grid = dalloc<Cplx<R>>(P->fft.grid_elems());  // c2r plans: n_total reals
     P->d_buf1 = dalloc<Cplx<R>>(P->fft.buf1_elems());
     P->d_buf2 = dalloc<Cplx<R>>(P->fft.buf2_elems());
     P->d_tab = dalloc<double>(N[0] + N[1] + N[2]);
@@ -1207,7 +1207,7 @@ static bool adj_passes(PlanT<R>* P, NodesT<R>* D, unsigned cf, R* dx, Cplx<R>* d
             return false;
           }
         } else {
-          fill_zero<R>(P->d_grid, P->n_total, s);
+          fill_zero<R>(P->d_grid, (long)P->fft.grid_elems(), s);
         }
       }
       if (back && fft3d_backward<R>(P->fft, P->d_grid, nullptr, P->d_buf1, P->d_buf2, dfh, 1, s, P->c2r, nullptr,
@@ -1300,7 +1300,7 @@ static void adj_impl(PlanT<R>* P, NodesT<R>* D, unsigned cf) {
         P->evt.rec(5, s);
         had_b = true;
       } else {
-        fill_zero<R>(P->d_grid, P->n_total, s);
+        fill_zero<R>(P->d_grid, (long)P->fft.grid_elems(), s);
       }
     }
     // F^H and D^T (fused), f_hat += D^T F^H grid

@@ -116,6 +116,13 @@ __device__ inline void shift_x(NodeWin<R, CM>& w) {
 // Separable gather sum f = sum_a px sum_b py sum_c pz G (and the three
 // gradient sums with one factor replaced by its derivative), static width CM.
 // ROW(a, b) returns a pointer p with p[IDX(c)] the grid value at stencil (a,b,c).
+// a grid cell read through a row pointer: complex grids, or the real grid of a
+// c2r plan (imaginary part zero)
+template <class R>
+__device__ __forceinline__ Cplx<R> grid_cell(const Cplx<R>* row, int i) { return row[i]; }
+template <class R>
+__device__ __forceinline__ Cplx<R> grid_cell(const R* row, int i) { return cmk<R>(row[i], R(0)); }
+
 template <class R, int CM, int MODE, class RowFn, class IdxFn>
 __device__ inline void gather_sum(NodeWin<R, CM> w, RowFn row_of, IdxFn idx, Cplx<R>* out) {
   constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD || MODE == B_VJP);
@@ -125,11 +132,11 @@ __device__ inline void gather_sum(NodeWin<R, CM> w, RowFn row_of, IdxFn idx, Cpl
     Cplx<R> ha = cmk<R>(0, 0), hay = cmk<R>(0, 0), haz = cmk<R>(0, 0);
 #pragma unroll
     for (int b = 0; b < CM; b++) {
-      const Cplx<R>* row = row_of(a, b);
+      const auto row = row_of(a, b);
       Cplx<R> hb = cmk<R>(0, 0), hbz = cmk<R>(0, 0);
 #pragma unroll
       for (int s = 0; s < CM; s++) {
-        const Cplx<R> gv = row[idx(s)];
+        const Cplx<R> gv = grid_cell<R>(row, idx(s));
         hb.x += w.pz[s] * gv.x;
         hb.y += w.pz[s] * gv.y;
         if (WG) {
@@ -172,10 +179,10 @@ __device__ inline void gather_sum_rt(const NodeWin<R, CM>& w, int c, RowFn row_o
   for (int a = 0; a < c; a++) {
     Cplx<R> ha = cmk<R>(0, 0), hay = cmk<R>(0, 0), haz = cmk<R>(0, 0);
     for (int b = 0; b < c; b++) {
-      const Cplx<R>* row = row_of(a, b);
+      const auto row = row_of(a, b);
       Cplx<R> hb = cmk<R>(0, 0), hbz = cmk<R>(0, 0);
       for (int s = 0; s < c; s++) {
-        const Cplx<R> gv = row[idx(s)];
+        const Cplx<R> gv = grid_cell<R>(row, idx(s));
         hb.x += w.pz[s] * gv.x;
         hb.y += w.pz[s] * gv.y;
         if (WG) {
@@ -274,20 +281,30 @@ __device__ inline void gather_node_global(const BArgs<R>& A, long j, const NodeW
     const int v = w.start[2] + s;
     i2s[s] = v >= n2 ? v - n2 : v;
   }
-  const Cplx<R>* grid = A.grid;
-  auto row_of = [&](int a, int b) {
+  auto row_off = [&](int a, int b) {
     int i0 = w.start[0] + a;
     if (i0 >= n0) i0 -= n0;
     int i1 = w.start[1] + b;
     if (i1 >= n1) i1 -= n1;
-    return grid + ((long)i0 * n1 + i1) * n2;
+    return ((long)i0 * n1 + i1) * n2;
   };
   auto idx = [&](int s) { return i2s[s]; };
   Cplx<R> v[4];
-  if constexpr (ST)
-    gather_sum<R, CM, MODE>(w, row_of, idx, v);
-  else
-    gather_sum_rt<R, CM, MODE>(w, c, row_of, idx, v);
+  if (A.c2r) {  // real grid
+    const R* grid = reinterpret_cast<const R*>(A.grid);
+    auto row_of = [&](int a, int b) { return grid + row_off(a, b); };
+    if constexpr (ST)
+      gather_sum<R, CM, MODE>(w, row_of, idx, v);
+    else
+      gather_sum_rt<R, CM, MODE>(w, c, row_of, idx, v);
+  } else {
+    const Cplx<R>* grid = A.grid;
+    auto row_of = [&](int a, int b) { return grid + row_off(a, b); };
+    if constexpr (ST)
+      gather_sum<R, CM, MODE>(w, row_of, idx, v);
+    else
+      gather_sum_rt<R, CM, MODE>(w, c, row_of, idx, v);
+  }
   gather_store<R, MODE>(A, j, v);
 }
 
@@ -308,6 +325,18 @@ __device__ inline Cplx<R> scatter_value(R pxy, R pdxy, R pxdy, R pz, R dz, Cplx<
     vi += q0 * gv0.y + q1 * gv1.y + q2 * gv2.y;
   }
   return cmk<R>(vr, vi);
+}
+
+// adds v at cell row + i of the grid (global atomics): both parts, or the real
+// part of a c2r plan's real grid (real node values leave no imaginary part)
+template <class R>
+__device__ __forceinline__ void add_cell(const BArgs<R>& A, long row, int i, Cplx<R> v) {
+  if (A.c2r) {
+    atomicAdd(reinterpret_cast<R*>(A.grid) + row + i, v.x);
+  } else {
+    atomicAdd(&A.grid[row + i].x, v.x);
+    atomicAdd(&A.grid[row + i].y, v.y);
+  }
 }
 
 template <class R, int CM, int MODE, bool ST>
@@ -337,15 +366,14 @@ __device__ inline void scatter_node_global(const BArgs<R>& A, long j, NodeWin<R,
       for (int b = 0; b < CM; b++) {
         int i1 = w.start[1] + b;
         if (i1 >= n1) i1 -= n1;
-        Cplx<R>* row = A.grid + ((long)i0 * n1 + i1) * n2;
+        const long row = ((long)i0 * n1 + i1) * n2;
         const R pxy = w.px[0] * w.py[b];
         const R pdxy = WG ? w.dx[0] * w.py[b] : R(0);
         const R pxdy = WG ? w.px[0] * w.dy[b] : R(0);
 #pragma unroll
         for (int s = 0; s < CM; s++) {
           const Cplx<R> v = scatter_value<R, WF, WG>(pxy, pdxy, pxdy, w.pz[s], WG ? w.dz[s] : R(0), fv, gv0, gv1, gv2);
-          atomicAdd(&row[i2s[s]].x, v.x);
-          atomicAdd(&row[i2s[s]].y, v.y);
+          add_cell<R>(A, row, i2s[s], v);
         }
       }
       shift_x<R, CM, WG>(w);
@@ -357,7 +385,7 @@ __device__ inline void scatter_node_global(const BArgs<R>& A, long j, NodeWin<R,
       for (int b = 0; b < c; b++) {
         int i1 = w.start[1] + b;
         if (i1 >= n1) i1 -= n1;
-        Cplx<R>* row = A.grid + ((long)i0 * n1 + i1) * n2;
+        const long row = ((long)i0 * n1 + i1) * n2;
         const R pxy = w.px[a] * w.py[b];
         const R pdxy = WG ? w.dx[a] * w.py[b] : R(0);
         const R pxdy = WG ? w.px[a] * w.dy[b] : R(0);
@@ -365,8 +393,7 @@ __device__ inline void scatter_node_global(const BArgs<R>& A, long j, NodeWin<R,
           int i2 = w.start[2] + s;
           if (i2 >= n2) i2 -= n2;
           const Cplx<R> v = scatter_value<R, WF, WG>(pxy, pdxy, pxdy, w.pz[s], WG ? w.dz[s] : R(0), fv, gv0, gv1, gv2);
-          atomicAdd(&row[i2].x, v.x);
-          atomicAdd(&row[i2].y, v.y);
+          add_cell<R>(A, row, i2, v);
         }
       }
     }

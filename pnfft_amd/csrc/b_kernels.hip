@@ -121,8 +121,10 @@ __device__ inline void stage_subgrid_static(const BArgs<R>& A, const TileItem& i
   constexpr int H = P / 2, RP = P + 1, PP = P * RP + 1, NPAIR = P * P * H;
   constexpr int K = (NPAIR + kTileThreads - 1) / kTileThreads;
   using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;  // two cells
+  using V2 = typename std::conditional<sizeof(R) == 4, float2, double2>::type;  // two cells of a real grid
   const unsigned n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
   const V* src = reinterpret_cast<const V*>(A.grid);
+  const V2* src_r = reinterpret_cast<const V2*>(A.grid);
   // all loads first, then the LDS stores: the K round trips overlap
   V v[K];
 #pragma unroll
@@ -132,7 +134,15 @@ __device__ inline void stage_subgrid_static(const BArgs<R>& A, const TileItem& i
       const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
       const unsigned g0 = (unsigned)(it.o[0] + cx) & (n0 - 1), g1 = (unsigned)(it.o[1] + cy) & (n1 - 1);
       const unsigned g2 = (unsigned)(it.o[2] + 2 * hz) & (n2 - 1);
-      v[k] = src[((g0 * n1 + g1) * n2 + g2) >> 1];
+      if (A.c2r) {  // real grid (c2r plans): imaginary parts zero
+        const V2 r = src_r[((g0 * n1 + g1) * n2 + g2) >> 1];
+        v[k].x = r.x;
+        v[k].y = R(0);
+        v[k].z = r.y;
+        v[k].w = R(0);
+      } else {
+        v[k] = src[((g0 * n1 + g1) * n2 + g2) >> 1];
+      }
     }
   }
 #pragma unroll
@@ -175,7 +185,8 @@ __global__ __launch_bounds__(kTileThreads, (gather_waves<R, MODE>())) void gathe
     for (int idx = threadIdx.x; idx < vol; idx += kTileThreads) {
       const int cz = idx % P2, cy = (idx / P2) % P1, cx = idx / (P1 * P2);
       const int g0 = wrapi(it.o[0] + cx, n0), g1 = wrapi(it.o[1] + cy, n1), g2 = wrapi(it.o[2] + cz, n2);
-      tile[cx * sp.PP + cy * sp.RP + cz] = A.grid[((long)g0 * n1 + g1) * n2 + g2];
+      const long gi = ((long)g0 * n1 + g1) * n2 + g2;
+      tile[cx * sp.PP + cy * sp.RP + cz] = A.c2r ? cmk<R>(reinterpret_cast<const R*>(A.grid)[gi], R(0)) : A.grid[gi];
     }
   }
   __syncthreads();
@@ -287,7 +298,12 @@ __global__ __launch_bounds__(kTileThreads) void scatter_tiled_k(BArgs<R> A) {
     const double vr = acc[2 * li], vi = acc[2 * li + 1];
     if (vr == 0.0 && vi == 0.0) continue;
     const int g0 = wrapi(it.o[0] + cx, n0), g1 = wrapi(it.o[1] + cy, n1), g2 = wrapi(it.o[2] + cz, n2);
-    Cplx<R>* cell = A.grid + ((long)g0 * n1 + g1) * n2 + g2;
+    const long gi = ((long)g0 * n1 + g1) * n2 + g2;
+    if (A.c2r) {  // real grid: the real part (the imaginary one is zero for real node values)
+      atomicAdd(reinterpret_cast<R*>(A.grid) + gi, (R)vr);
+      continue;
+    }
+    Cplx<R>* cell = A.grid + gi;
     atomicAdd(&cell->x, (R)vr);
     atomicAdd(&cell->y, (R)vi);
   }
@@ -417,6 +433,7 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
   {
     constexpr int H = P / 2, NPAIR = P * P * H, K = (NPAIR + NT - 1) / NT;
     const float4* src = reinterpret_cast<const float4*>(A.grid);
+    const float2* src_r = reinterpret_cast<const float2*>(A.grid);  // real grid of a c2r plan
     float4 v[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -425,7 +442,12 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
         const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
         const unsigned g0 = (unsigned)(o0 + cx) & (n0 - 1), g1 = (unsigned)(o1 + cy) & (n1 - 1);
         const unsigned g2 = (unsigned)(o2 + 2 * hz) & (n2 - 1);
-        v[k] = src[((g0 * n1 + g1) * n2 + g2) >> 1];
+        if (A.c2r) {
+          const float2 r = src_r[((g0 * n1 + g1) * n2 + g2) >> 1];
+          v[k] = make_float4(r.x, 0.f, r.y, 0.f);
+        } else {
+          v[k] = src[((g0 * n1 + g1) * n2 + g2) >> 1];
+        }
       }
     }
 #pragma unroll

@@ -55,11 +55,14 @@ struct PassArgs {
   int N1;
   int ctab;                  // tab0..2 hold complex factors (re, im pairs) per frequency index
   int accumulate;            // backward store: out += value (D^T onto f_hat)
+  int colfac;                // columns pass over f_hat's axis 1 (c2r plans): factors at (plane, position, column)
   // frequency <-> index map of the pruned side: index e holds frequency k = e - h;
-  // forward loads take k in [-h, pos); the c2r half spectrum is h = N2/2, pos = 1
-  // (k2 in [-N2/2, 0]) and its k2 < 0 entries enter the pencil twice (weight 2)
+  // forward loads take k in [-h, pos)
   int h, pos;
-  int half;
+  // c2r / r2c row passes: twiddles tw2[p] = e^{-i pi p / L} of the length-2L real
+  // transform and N2 / 2 (the stored half spectrum k2 in [-N2/2, 0], N2/2 + 1 values)
+  const Cplx<R>* tw2;
+  int N2h;
 };
 
 template <class R>
@@ -137,16 +140,17 @@ using ColsShape = PassShape<R, L, cols_threads<R, L>(), cols_ept<R, L, NARROW>()
 // round trips in two halves of the pencil positions, so a block holds 256
 // positions (68 KB) and a CU two blocks: one block's loads and stores overlap
 // the other's LDS work.
-template <class R, int L, bool NARROW>
+template <class R, int L, bool NARROW, bool FAC = false>
 constexpr bool cols_split() {
   using S = ColsShape<R, L, NARROW>;
-  return !NARROW && sizeof(R) == 4 && L == 512 && S::C == 32 && cols_threads<R, L>() == 512 && S::EPT == 32;
+  return !FAC && !NARROW && sizeof(R) == 4 && L == 512 && S::C == 32 && cols_threads<R, L>() == 512 &&
+         S::EPT == 32;
 }
 
-template <class R, int L, bool NARROW>
+template <class R, int L, bool NARROW, bool FAC = false>
 constexpr size_t cols_lds() {
   using S = ColsShape<R, L, NARROW>;
-  return cols_split<R, L, NARROW>() ? S::LDS / 2 : S::LDS;
+  return cols_split<R, L, NARROW, FAC>() ? S::LDS / 2 : S::LDS;
 }
 
 // radix of the stage that starts at span Ns (8 while possible, then 4 or 2)
@@ -250,8 +254,7 @@ __device__ inline Cplx<R> apply_factors(const PassArgs<R>& P, Cplx<R> w, int i0,
   return cmk<R>(R(wr * fr - wi * fi), R(wr * fi + wi * fr));
 }
 
-// value as it enters the transform: D factor and c2r weight (forward rows),
-// conjugation (backward)
+// value as it enters the transform: D factor (forward rows), conjugation (backward)
 template <class R>
 __device__ inline Cplx<R> on_load(const PassArgs<R>& P, Cplx<R> w, int col, int e) {
   const bool fwd = !P.backward;
@@ -259,7 +262,6 @@ __device__ inline Cplx<R> on_load(const PassArgs<R>& P, Cplx<R> w, int col, int 
     // row = i0*N1 + i1 (rows pass over f_hat), e = i2
     w = apply_factors(P, w, col / P.N1, col % P.N1, e);
   }
-  if (P.half && fwd && e < P.h) w = cmk<R>(R(2) * w.x, R(2) * w.y);
   if (!fwd) w = cconj(w);
   return w;
 }
@@ -328,6 +330,99 @@ __global__ __launch_bounds__(kThreads) void fft_rows(PassArgs<R> P) {
     const int col = c0 + c;
     const int e = store_index(P, p, L);
     if (col < P.n_inner && e >= 0) on_store(P, v[i], col, e, (long)col * P.out_outer + e);
+  }
+}
+
+// Coefficient S[k] (k = 0..L) of the Hermitian spectrum of a real row of length
+// 2L whose stored half u holds the frequencies k2 = e - Nh in [-Nh, 0]: the row is
+// Re(u[Nh] + 2 sum_{k2 < 0} u[k2 + Nh] e^{-2 pi i k2 l / 2L}), i.e. S[0] = Re u[Nh],
+// S[k] = u[Nh - k] for 0 < k < L (zero beyond Nh), and at k = L (only when
+// Nh = L) the Nyquist value 2 Re u[0].
+template <class R>
+__device__ inline Cplx<R> half_coef(const Cplx<R>* u, int k, int Nh, int L) {
+  if (k == 0) return cmk<R>(u[Nh].x, R(0));
+  if (k == L) return Nh == L ? cmk<R>(R(2) * u[0].x, R(0)) : cmk<R>(R(0), R(0));
+  return k <= Nh ? u[Nh - k] : cmk<R>(R(0), R(0));
+}
+
+// c2r rows pass (axis 2 of a c2r plan, last forward pass): the real row x of
+// length 2L from the half spectrum u (Nh + 1 values per row) as one complex
+// transform of length L: z = IDFT_L(E + i O) with E[k] = S[k] + conj S[L - k] and
+// O[k] = (S[k] - conj S[L - k]) e^{i pi k / L} gives z[j] = x[2j] + i x[2j + 1],
+// so the complex output row is the real row in memory.  The inverse transform
+// is conj(FFT(conj)).
+template <class R, int L>
+__global__ __launch_bounds__(kThreads) void fft_rows_c2r(PassArgs<R> P) {
+  using S = PassShape<R, L>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<R>* sh = reinterpret_cast<Cplx<R>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * S::C;
+#pragma unroll
+  for (int i = 0; i < S::EPT; i++) {
+    const int idx = tid + i * kThreads;
+    const int c = idx / L, p = idx % L;
+    const int row = c0 + c;
+    Cplx<R> z = cmk<R>(R(0), R(0));
+    if (row < P.n_inner) {
+      const Cplx<R>* u = P.in + (long)row * P.in_outer;
+      const Cplx<R> a = half_coef<R>(u, p, P.N2h, L), b = cconj(half_coef<R>(u, L - p, P.N2h, L));
+      const Cplx<R> E = cadd(a, b), O = cmul(csub(a, b), cconj(P.tw2[p]));
+      z = cmk<R>(E.x - O.y, E.y + O.x);
+    }
+    sh[p * S::LD + c] = cconj(z);
+  }
+  __syncthreads();
+  stages_lds<R, L, 1, L, kThreads, S::EPT>(sh, P.tw);
+#pragma unroll
+  for (int i = 0; i < S::EPT; i++) {
+    const int idx = tid + i * kThreads;
+    const int c = idx / L, p = idx % L;
+    const int row = c0 + c;
+    if (row < P.n_inner) P.out[(long)row * P.out_outer + p] = cconj(sh[p * S::LD + c]);
+  }
+}
+
+// r2c rows pass (axis 2 of a c2r plan, first backward pass): for the real row x
+// of length 2L read as z[j] = x[2j] + i x[2j + 1], Z = DFT+_L(z) (the adjoint's
+// sign, conj(FFT(conj))) gives G[k] = sum_l x[l] e^{+2 pi i k l / 2L} =
+// (Z[k] + conj Z[L - k]) / 2 + e^{i pi k / L} (Z[k] - conj Z[L - k]) / 2i, and the
+// stored half k2 = -k in [-Nh, 0] is conj G[k].
+template <class R, int L>
+__global__ __launch_bounds__(kThreads) void fft_rows_r2c(PassArgs<R> P) {
+  using S = PassShape<R, L>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<R>* sh = reinterpret_cast<Cplx<R>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * S::C;
+#pragma unroll
+  for (int i = 0; i < S::EPT; i++) {
+    const int idx = tid + i * kThreads;
+    const int c = idx / L, p = idx % L;
+    const int row = c0 + c;
+    Cplx<R> z = cmk<R>(R(0), R(0));
+    if (row < P.n_inner) z = P.in[(long)row * P.in_outer + p];
+    sh[p * S::LD + c] = cconj(z);
+  }
+  __syncthreads();
+  stages_lds<R, L, 1, L, kThreads, S::EPT>(sh, P.tw);
+  const int Nh = P.N2h;
+#pragma unroll
+  for (int i = 0; i < S::EPT; i++) {
+    const int idx = tid + i * kThreads;
+    const int c = idx / L, p = idx % L;
+    const int row = c0 + c;
+    if (row < P.n_inner && p <= Nh) {
+      const Cplx<R> zk = cconj(sh[p * S::LD + c]);
+      const Cplx<R> zm = sh[((L - p) & (L - 1)) * S::LD + c];  // conj Z[L - k]
+      const Cplx<R> ge = cmk<R>(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y + zm.y));
+      const Cplx<R> d = csub(zk, zm);
+      const Cplx<R> go = cmk<R>(R(0.5) * d.y, R(-0.5) * d.x);  // d / 2i
+      const Cplx<R> G = cadd(ge, cmul(cconj(P.tw2[p]), go));
+      Cplx<R>* o = P.out + (long)row * P.out_outer;
+      o[Nh - p] = cconj(G);
+      if (p == 0 && Nh == L) o[0] = cmk<R>(zk.x - zk.y, R(0));  // k = L: G = Re Z[0] - Im Z[0]
+    }
   }
 }
 
@@ -404,8 +499,11 @@ __global__ __launch_bounds__(256) void fft_rows512(PassArgs<R> P) {
 // plane blockIdx.y, consecutive lanes take consecutive columns.  The first
 // stage runs on the loaded registers and the last stage stores straight to
 // HBM, so only the middle stages touch LDS.
-template <class R, int L, bool NARROW>
-__global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW>() ? 4 : 1)) void fft_cols(
+// FAC: the pass over f_hat's axis 1 of a c2r plan, which takes the per-axis
+// factors (D, or the complex ones) of element (plane, position, column) on its
+// loads (forward) or its stores (backward, with accumulation onto f_hat)
+template <class R, int L, bool NARROW, bool FAC>
+__global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW, FAC>() ? 4 : 1)) void fft_cols(
     PassArgs<R> P) {
   constexpr int NT = cols_threads<R, L>();
   using S = ColsShape<R, L, NARROW>;
@@ -430,7 +528,12 @@ __global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW>()
     for (int q = 0; q < R1; q++) {
       const int e = load_index(P, j + q * nbf1, L);
       v[i * R1 + q] = cmk<R>(R(0), R(0));
-      if (col < P.n_inner && e >= 0) v[i * R1 + q] = P.in[in_base + col + (long)e * P.in_elem];
+      if (col < P.n_inner && e >= 0) {
+        v[i * R1 + q] = P.in[in_base + col + (long)e * P.in_elem];
+        if constexpr (FAC) {
+          if (!P.backward && P.tab0) v[i * R1 + q] = apply_factors(P, v[i * R1 + q], (int)blockIdx.y, e, col);
+        }
+      }
     }
   };
   auto stage1_bf = [&](int i) {
@@ -440,13 +543,13 @@ __global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW>()
     }
     dft<R, R1>(v + i * R1);
   };
-  if constexpr (!cols_split<R, L, NARROW>()) {
+  if constexpr (!cols_split<R, L, NARROW, FAC>()) {
 #pragma unroll
     for (int i = 0; i < NB1; i++) load_bf(i);
 #pragma unroll
     for (int i = 0; i < NB1; i++) stage1_bf(i);
   }
-  if constexpr (cols_split<R, L, NARROW>()) {
+  if constexpr (cols_split<R, L, NARROW, FAC>()) {
     // L = 512 = 8^3, butterflies i = 0..3 of pencil c at j = j0 + 16 i; stage
     // outputs of butterflies 2h, 2h + 1 and stage inputs q = 4h .. 4h + 3 lie in
     // the half h of the positions, [256 h, 256 h + 256).  Butterflies 2 and 3
@@ -578,8 +681,16 @@ __global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW>()
 #pragma unroll
     for (int q = 0; q < RO; q++) {
       const int e = store_index(P, j + q * stepO, L);
-      // the column passes carry no factors and never accumulate: conjugation only
-      if (e >= 0) P.out[out_base + col + (long)e * P.out_elem] = P.backward ? cconj(v[i * RO + q]) : v[i * RO + q];
+      if (e >= 0) {
+        const long addr = out_base + col + (long)e * P.out_elem;
+        Cplx<R> w = P.backward ? cconj(v[i * RO + q]) : v[i * RO + q];
+        if constexpr (FAC) {
+          // the c2r plan's last backward pass: D^T (or the complex factors) and accumulation onto f_hat
+          if (P.backward && P.tab0) w = apply_factors(P, w, (int)blockIdx.y, e, col);
+          if (P.accumulate) w = cadd(P.out[addr], w);
+        }
+        P.out[addr] = w;
+      }
     }
   }
 }
@@ -598,7 +709,9 @@ bool fft_length_supported(int L, bool is_double) {
 template <class R>
 void FftAxis<R>::release() {
   if (tw) (void)hipFree(tw);
+  if (tw2) (void)hipFree(tw2);
   tw = nullptr;
+  tw2 = nullptr;
 }
 
 template <class R>
@@ -615,39 +728,46 @@ static int pass_attr() {
   constexpr size_t lr = PassShape<R, L>::LDS;
   constexpr size_t lc = cols_lds<R, L, false>();
   constexpr size_t ln = cols_lds<R, L, true>();
-  if (lr > 64 * 1024 &&
-      hipFuncSetAttribute((const void*)fft_rows<R, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr) !=
-          hipSuccess)
-    return kHipError;
-  if (lc > 64 * 1024 &&
-      hipFuncSetAttribute((const void*)fft_cols<R, L, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)lc) != hipSuccess)
-    return kHipError;
-  if (ln > 64 * 1024 &&
-      hipFuncSetAttribute((const void*)fft_cols<R, L, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)ln) != hipSuccess)
+  auto attr = [](const void* f, size_t lds) {
+    return lds <= 64 * 1024 ||
+           hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
+  };
+  if (!attr((const void*)fft_rows<R, L>, lr) || !attr((const void*)fft_rows_c2r<R, L>, lr) ||
+      !attr((const void*)fft_rows_r2c<R, L>, lr) || !attr((const void*)fft_cols<R, L, false, false>, lc) ||
+      !attr((const void*)fft_cols<R, L, true, false>, ln) || !attr((const void*)fft_cols<R, L, true, true>, ln))
     return kHipError;
   return kOk;
 }
 
+enum PassKind { kPassRows = 0, kPassCols = 1, kPassRowsC2r = 2, kPassRowsR2c = 3 };
+
 template <class R, int L>
-static void pass_launch(bool rows, dim3 grid, const PassArgs<R>& P, hipStream_t s) {
-  if (rows) {
+static void pass_launch(int kind, dim3 grid, const PassArgs<R>& P, hipStream_t s) {
+  if (kind == kPassRows) {
     if constexpr (L == 512) {
       hipLaunchKernelGGL((fft_rows512<R>), grid, dim3(256), 0, s, P);
     } else {
       constexpr size_t lds = PassShape<R, L>::LDS;
       hipLaunchKernelGGL((fft_rows<R, L>), grid, dim3(kThreads), lds, s, P);
     }
+  } else if (kind == kPassRowsC2r) {
+    constexpr size_t lds = PassShape<R, L>::LDS;
+    hipLaunchKernelGGL((fft_rows_c2r<R, L>), grid, dim3(kThreads), lds, s, P);
+  } else if (kind == kPassRowsR2c) {
+    constexpr size_t lds = PassShape<R, L>::LDS;
+    hipLaunchKernelGGL((fft_rows_r2c<R, L>), grid, dim3(kThreads), lds, s, P);
   } else {
     constexpr int nt = cols_threads<R, L>();
-    // the narrow variant is the one whose pencil count the pass was planned with
-    if (ColsShape<R, L, true>::C != ColsShape<R, L, false>::C && P.C == ColsShape<R, L, true>::C) {
-      constexpr size_t lds = cols_lds<R, L, true>();
-      hipLaunchKernelGGL((fft_cols<R, L, true>), grid, dim3(nt), lds, s, P);
+    constexpr size_t lf = cols_lds<R, L, true, true>();
+    constexpr size_t ln = cols_lds<R, L, true>();
+    constexpr size_t lw = cols_lds<R, L, false>();
+    if (P.colfac) {  // planned with the narrow pencil count
+      hipLaunchKernelGGL((fft_cols<R, L, true, true>), grid, dim3(nt), lf, s, P);
+    } else if (ColsShape<R, L, true>::C != ColsShape<R, L, false>::C && P.C == ColsShape<R, L, true>::C) {
+      // the narrow variant is the one whose pencil count the pass was planned with
+      hipLaunchKernelGGL((fft_cols<R, L, true, false>), grid, dim3(nt), ln, s, P);
     } else {
-      constexpr size_t lds = cols_lds<R, L, false>();
-      hipLaunchKernelGGL((fft_cols<R, L, false>), grid, dim3(nt), lds, s, P);
+      hipLaunchKernelGGL((fft_cols<R, L, false, false>), grid, dim3(nt), lw, s, P);
     }
   }
 }
@@ -681,7 +801,7 @@ static int with_length(int L, F&& f) {
 }
 
 template <class R>
-int FftAxis<R>::init(int L_) {
+int FftAxis<R>::init(int L_, bool half_angles) {
   L = L_;
   if (!fft_length_supported(L, sizeof(R) == 8)) return kUnsupported;
   C = L == 512 ? kRows512Pencils : FftEpt<R>::value * kThreads / L;
@@ -701,13 +821,22 @@ int FftAxis<R>::init(int L_) {
   }
   if (hipMalloc(&tw, sizeof(Cplx<R>) * L) != hipSuccess) return kHipError;
   if (hipMemcpy(tw, h.data(), sizeof(Cplx<R>) * L, hipMemcpyHostToDevice) != hipSuccess) return kHipError;
+  if (half_angles) {
+    for (int i = 0; i < L; i++) {
+      // angle -pi i / L in FP64
+      const double a = -3.14159265358979323846 * (double)i / (double)L;
+      h[i] = cmk<R>(R(cos(a)), R(sin(a)));
+    }
+    if (hipMalloc(&tw2, sizeof(Cplx<R>) * L) != hipSuccess) return kHipError;
+    if (hipMemcpy(tw2, h.data(), sizeof(Cplx<R>) * L, hipMemcpyHostToDevice) != hipSuccess) return kHipError;
+  }
   return kOk;
 }
 
 template <class R>
-static void launch_pass(bool rows, dim3 grid, const PassArgs<R>& P, hipStream_t s) {
+static void launch_pass(int kind, dim3 grid, const PassArgs<R>& P, hipStream_t s) {
   (void)with_length<R>(P.L, [&](auto Lc) {
-    pass_launch<R, decltype(Lc)::value>(rows, grid, P, s);
+    pass_launch<R, decltype(Lc)::value>(kind, grid, P, s);
     return kOk;
   });
 }
@@ -735,21 +864,66 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
                   const double* ctab) {
   const int N0 = F.N[0], N1 = F.N[1], N2 = F.N[2];
   const int n0 = F.n[0], n1 = F.n[1], n2 = F.n[2];
-  const int N2in = half ? N2 / 2 + 1 : N2;  // stored length of f_hat rows
+  if (half != F.c2r) return kUnsupported;
+  if (half) {
+    const int H = N2 / 2 + 1;
+    // axis 1: f_hat[i0][i1][k2] (D fused on the loads) -> buf1[i0][p1][k2]
+    {
+      PassArgs<R> P = make_args(F.ax[1]);
+      P.in = fhat; P.out = buf1;
+      P.in_outer = (long)N1 * H; P.in_elem = H;
+      P.out_outer = (long)n1 * H; P.out_elem = H;
+      P.n_inner = H;
+      P.C = F.ax[1].Ccols_narrow;
+      P.Lin = N1; P.Lout = n1;
+      P.h = N1 / 2; P.pos = N1 - N1 / 2;
+      P.colfac = 1;
+      set_factors(P, tab, ctab, N0, N1);
+      dim3 grid_dim((H + P.C - 1) / P.C, N0);
+      launch_pass<R>(kPassCols, grid_dim, P, s);
+      if (mark1) (void)hipEventRecord(mark1, s);
+    }
+    // axis 0: pencils (p1, k2) stride n1*H -> buf2[p0][p1][k2]
+    {
+      PassArgs<R> P = make_args(F.ax[0]);
+      P.in = buf1; P.out = buf2;
+      P.in_outer = 0; P.in_elem = (long)n1 * H;
+      P.out_outer = 0; P.out_elem = (long)n1 * H;
+      P.n_inner = n1 * H;
+      P.C = F.ax[0].Ccols_narrow;
+      P.Lin = N0; P.Lout = n0;
+      P.h = N0 / 2; P.pos = N0 - N0 / 2;
+      dim3 grid_dim((n1 * H + P.C - 1) / P.C, 1);
+      launch_pass<R>(kPassCols, grid_dim, P, s);
+      if (mark2) (void)hipEventRecord(mark2, s);
+    }
+    // axis 2: c2r rows buf2[p0][p1][:] -> the real grid rows (n2 reals = n2/2 complex)
+    {
+      PassArgs<R> P = make_args(F.ax[2]);
+      P.in = buf2; P.out = grid;
+      P.in_outer = H; P.out_outer = n2 / 2;
+      P.n_inner = n0 * n1;
+      P.C = FftEpt<R>::value * kThreads / F.ax[2].L;
+      P.tw2 = F.ax[2].tw2;
+      P.N2h = N2 / 2;
+      dim3 grid_dim((P.n_inner + P.C - 1) / P.C, 1);
+      launch_pass<R>(kPassRowsC2r, grid_dim, P, s);
+    }
+    return hipGetLastError() == hipSuccess ? kOk : kHipError;
+  }
   // axis 2: rows (i0,i1) of f_hat -> buf1[i0][i1][p2]
   {
     PassArgs<R> P = make_args(F.ax[2]);
     P.in = fhat; P.out = buf1;
-    P.in_outer = N2in; P.in_elem = 1;
+    P.in_outer = N2; P.in_elem = 1;
     P.out_outer = n2; P.out_elem = 1;
     P.n_inner = N0 * N1;
-    P.Lin = N2in; P.Lout = n2;
+    P.Lin = N2; P.Lout = n2;
     P.h = N2 / 2;
-    P.pos = half ? 1 : N2 - N2 / 2;
-    P.half = half ? 1 : 0;
+    P.pos = N2 - N2 / 2;
     set_factors(P, tab, ctab, N0, N1);
     dim3 grid_dim((P.n_inner + P.C - 1) / P.C, 1);
-    launch_pass<R>(true, grid_dim, P, s);
+    launch_pass<R>(kPassRows, grid_dim, P, s);
     if (mark1) (void)hipEventRecord(mark1, s);
   }
   // axis 1: pencils (i0, p2) stride n2 -> buf2[i0][p1][p2]
@@ -763,7 +937,7 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.Lin = N1; P.Lout = n1;
     P.h = N1 / 2; P.pos = N1 - N1 / 2;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);
-    launch_pass<R>(false, grid_dim, P, s);
+    launch_pass<R>(kPassCols, grid_dim, P, s);
     if (mark2) (void)hipEventRecord(mark2, s);
   }
   // axis 0: pencils (p1, p2) stride n1*n2 -> grid[p0][p1][p2]
@@ -777,7 +951,7 @@ int fft3d_forward(const Fft3d<R>& F, const Cplx<R>* fhat, const double* tab, Cpl
     P.Lin = N0; P.Lout = n0;
     P.h = N0 / 2; P.pos = N0 - N0 / 2;
     dim3 grid_dim((n2 + P.C - 1) / P.C, n1);
-    launch_pass<R>(false, grid_dim, P, s);
+    launch_pass<R>(kPassCols, grid_dim, P, s);
   }
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
@@ -788,7 +962,56 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
                    hipEvent_t mark2, const double* ctab) {
   const int N0 = F.N[0], N1 = F.N[1], N2 = F.N[2];
   const int n0 = F.n[0], n1 = F.n[1], n2 = F.n[2];
-  const int N2out = half ? N2 / 2 + 1 : N2;
+  if (half != F.c2r) return kUnsupported;
+  if (half) {
+    const int H = N2 / 2 + 1;
+    // axis 2: r2c rows of the real grid -> buf2[p0][p1][k2], k2 in [-N2/2, 0]
+    {
+      PassArgs<R> P = make_args(F.ax[2]);
+      P.in = grid; P.out = buf2;
+      P.in_outer = n2 / 2; P.out_outer = H;
+      P.n_inner = n0 * n1;
+      P.C = FftEpt<R>::value * kThreads / F.ax[2].L;
+      P.tw2 = F.ax[2].tw2;
+      P.N2h = N2 / 2;
+      P.backward = 1;
+      dim3 grid_dim((P.n_inner + P.C - 1) / P.C, 1);
+      launch_pass<R>(kPassRowsR2c, grid_dim, P, s);
+      if (mark1) (void)hipEventRecord(mark1, s);
+    }
+    // axis 0: buf2 pencils (p1, k2) -> buf1[i0][p1][k2]
+    {
+      PassArgs<R> P = make_args(F.ax[0]);
+      P.in = buf2; P.out = buf1;
+      P.in_outer = 0; P.in_elem = (long)n1 * H;
+      P.out_outer = 0; P.out_elem = (long)n1 * H;
+      P.n_inner = n1 * H;
+      P.C = F.ax[0].Ccols_narrow;
+      P.Lin = n0; P.Lout = N0; P.h = N0 / 2;
+      P.backward = 1;
+      dim3 grid_dim((n1 * H + P.C - 1) / P.C, 1);
+      launch_pass<R>(kPassCols, grid_dim, P, s);
+      if (mark2) (void)hipEventRecord(mark2, s);
+    }
+    // axis 1: buf1[i0][p1][k2] -> f_hat[i0][i1][k2] (D^T fused on the stores, accumulated)
+    {
+      PassArgs<R> P = make_args(F.ax[1]);
+      P.in = buf1; P.out = fhat;
+      P.in_outer = (long)n1 * H; P.in_elem = H;
+      P.out_outer = (long)N1 * H; P.out_elem = H;
+      P.n_inner = H;
+      P.C = F.ax[1].Ccols_narrow;
+      P.Lin = n1; P.Lout = N1; P.h = N1 / 2;
+      P.backward = 1;
+      P.accumulate = accumulate;
+      P.colfac = 1;
+      set_factors(P, tab, ctab, N0, N1);
+      dim3 grid_dim((H + P.C - 1) / P.C, N0);
+      launch_pass<R>(kPassCols, grid_dim, P, s);
+    }
+    return hipGetLastError() == hipSuccess ? kOk : kHipError;
+  }
+  const int N2out = N2;
   // axis 0: grid pencils (p1,p2) -> buf2[i0][p1][p2]
   {
     PassArgs<R> P = make_args(F.ax[0]);
@@ -800,7 +1023,7 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.Lin = n0; P.Lout = N0; P.h = N0 / 2;
     P.backward = 1;
     dim3 grid_dim((n2 + P.C - 1) / P.C, n1);
-    launch_pass<R>(false, grid_dim, P, s);
+    launch_pass<R>(kPassCols, grid_dim, P, s);
     if (mark1) (void)hipEventRecord(mark1, s);
   }
   // axis 1: buf2[i0][p1][p2] -> buf1[i0][i1][p2]
@@ -814,10 +1037,10 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.Lin = n1; P.Lout = N1; P.h = N1 / 2;
     P.backward = 1;
     dim3 grid_dim((n2 + P.C - 1) / P.C, N0);
-    launch_pass<R>(false, grid_dim, P, s);
+    launch_pass<R>(kPassCols, grid_dim, P, s);
     if (mark2) (void)hipEventRecord(mark2, s);
   }
-  // axis 2: rows buf1[i0][i1][:] -> f_hat[i0][i1][:] (D^T fused; c2r keeps k2 in [-N2/2, 0])
+  // axis 2: rows buf1[i0][i1][:] -> f_hat[i0][i1][:] (D^T fused)
   {
     PassArgs<R> P = make_args(F.ax[2]);
     P.in = buf1; P.out = fhat;
@@ -829,18 +1052,22 @@ int fft3d_backward(const Fft3d<R>& F, const Cplx<R>* grid, const double* tab, Cp
     P.accumulate = accumulate;
     set_factors(P, tab, ctab, N0, N1);
     dim3 grid_dim((P.n_inner + P.C - 1) / P.C, 1);
-    launch_pass<R>(true, grid_dim, P, s);
+    launch_pass<R>(kPassRows, grid_dim, P, s);
   }
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
 
 template <class R>
-int Fft3d<R>::init(const int* N_, const int* n_) {
+int Fft3d<R>::init(const int* N_, const int* n_, bool c2r_) {
+  c2r = c2r_;
   for (int t = 0; t < 3; t++) {
     N[t] = N_[t];
     n[t] = n_[t];
     if (N[t] > n[t]) return kUnsupported;
-    int st = ax[t].init(n[t]);
+    // c2r: the real axis-2 transform of length n2 is a complex one of length n2 / 2
+    const bool real_axis = c2r && t == 2;
+    if (real_axis && n[t] < 4) return kUnsupported;
+    int st = ax[t].init(real_axis ? n[t] / 2 : n[t], real_axis);
     if (st != kOk) return st;
   }
   for (int t = 0; t < 3; t++) {

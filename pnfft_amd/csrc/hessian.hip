@@ -88,12 +88,14 @@ __global__ __launch_bounds__(kHT) void hess_gather_k(BArgs<R> A, Cplx<R>* __rest
     for (int b = 0; b < CM; b++) {
       int i1 = s1 + b;
       if (i1 >= n1) i1 -= n1;
-      const Cplx<R>* row = A.grid + ((long)i0 * n1 + i1) * n2;
+      const long row = ((long)i0 * n1 + i1) * n2;
       R a0r = 0, a0i = 0, a1r = 0, a1i = 0, a2r = 0, a2i = 0;  // sum_c z G, z' G, z'' G
 #pragma unroll
       for (int s = 0; s < CM; s++) {
-        const Cplx<R> gv = row[i2s[s]];
-        const R gr = gv.x, gi = A.c2r ? R(0) : gv.y;
+        // c2r plans: the grid is real
+        const Cplx<R> gv = A.c2r ? cmk<R>(reinterpret_cast<const R*>(A.grid)[row + i2s[s]], R(0))
+                                 : A.grid[row + i2s[s]];
+        const R gr = gv.x, gi = gv.y;
         a0r += z0[s] * gr;
         a0i += z0[s] * gi;
         a1r += z1[s] * gr;

@@ -497,7 +497,8 @@ int scatter(const BArgs<R>& A, int mode, hipStream_t s) {
     if (st != kUnsupported) return st;  // kUnsupported: nothing was launched, take the LDS-atomic path
   }
   // LDS-atomic path: accumulates into the cleared grid
-  const long ntot = (long)A.g.n[0] * A.g.n[1] * A.g.n[2];
+  const long ncell = (long)A.g.n[0] * A.g.n[1] * A.g.n[2];
+  const long ntot = A.c2r ? (ncell + 1) / 2 : ncell;  // complex elements of the grid's storage (c2r: real grid)
   hipLaunchKernelGGL((fill_zero_k<R>), dim3(grid1d(ntot, 256)), dim3(256), 0, s, A.grid, (size_t)ntot);
   switch (mode) {
     case B_F: return b_launch<R, 1, B_F>(A, s);

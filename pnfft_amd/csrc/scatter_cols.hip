@@ -519,7 +519,10 @@ __global__ __launch_bounds__(TS* TS* TS) void fold_k(BArgs<R> A) {
   }
   const long n1 = A.g.n[1], n2 = A.g.n[2];
   const long x0 = (long)T0 * TS + q0, x1 = (long)T1 * TS + q1, x2 = (long)T2 * TS + q2;
-  A.grid[(x0 * n1 + x1) * n2 + x2] = cmk<R>((R)sr, (R)si);
+  if (A.c2r)  // real grid of a c2r plan
+    reinterpret_cast<R*>(A.grid)[(x0 * n1 + x1) * n2 + x2] = (R)sr;
+  else
+    A.grid[(x0 * n1 + x1) * n2 + x2] = cmk<R>((R)sr, (R)si);
 }
 
 // fold_k for tiles of 8^3 cells: 128 threads per grid tile, four consecutive
@@ -582,6 +585,12 @@ __global__ __launch_bounds__(128) void fold8_k(BArgs<R> A) {
   }
   const long n1 = A.g.n[1], n2 = A.g.n[2];
   const long x0 = (long)T0 * TS + q0, x1 = (long)T1 * TS + q1, x2 = (long)T2 * TS + q2;
+  if (A.c2r) {  // real grid of a c2r plan
+    R* dr = reinterpret_cast<R*>(A.grid) + (x0 * n1 + x1) * n2 + x2;
+#pragma unroll
+    for (int k = 0; k < 4; k++) dr[k] = (R)sr[k];
+    return;
+  }
   Cplx<R>* dst = A.grid + (x0 * n1 + x1) * n2 + x2;
   if constexpr (sizeof(R) == 4) {
     reinterpret_cast<float4*>(dst)[0] = make_float4((float)sr[0], (float)si[0], (float)sr[1], (float)si[1]);
@@ -652,6 +661,12 @@ __global__ __launch_bounds__(256) void fold8b_k(BArgs<R> A) {
   }
   const long n1 = A.g.n[1], n2 = A.g.n[2];
   const long x0 = (long)T0 * TS + q0, x1 = (long)T1 * TS + q1, x2 = (long)T2 * TS + c;
+  if (A.c2r) {  // real grid of a c2r plan
+    R* dr = reinterpret_cast<R*>(A.grid) + (x0 * n1 + x1) * n2 + x2;
+#pragma unroll
+    for (int k = 0; k < 4; k++) dr[k] = (R)sr[k];
+    return;
+  }
   Cplx<R>* dst = A.grid + (x0 * n1 + x1) * n2 + x2;
   if constexpr (sizeof(R) == 4) {
     reinterpret_cast<float4*>(dst)[0] = make_float4((float)sr[0], (float)si[0], (float)sr[1], (float)si[1]);
@@ -720,6 +735,12 @@ __global__ __launch_bounds__(128) void fold4_k(BArgs<R> A) {
   }
   const long n1 = A.g.n[1], n2 = A.g.n[2];
   const long x0 = (long)T0 * TS + q0, x1 = (long)T1 * TS + q1, x2 = (long)T2 * TS + c;
+  if (A.c2r) {  // real grid of a c2r plan
+    R* dr = reinterpret_cast<R*>(A.grid) + (x0 * n1 + x1) * n2 + x2;
+    dr[0] = (R)sr[0];
+    dr[1] = (R)sr[1];
+    return;
+  }
   Cplx<R>* dst = A.grid + (x0 * n1 + x1) * n2 + x2;
   if constexpr (sizeof(R) == 4) {
     *reinterpret_cast<float4*>(dst) = make_float4((float)sr[0], (float)si[0], (float)sr[1], (float)si[1]);

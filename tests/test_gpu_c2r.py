@@ -153,3 +153,68 @@ def test_c2r_particle_mesh_workload(gpu, oracle):
     assert rel(g["f"], f_o) < 1e-12
     assert rel(g["grad"], g_o) < 1e-12
     assert rel(g["adj"], P.adj_c2r(w["x"], f=q)) < 1e-12
+
+
+@pytest.mark.parametrize("precision,N,n,m", [("double", (8, 64, 256), (16, 128, 512), 4),
+                                              ("float", (8, 64, 256), (16, 128, 512), 4),
+                                              ("double", (8, 8, 16), (16, 16, 16), 3)])
+def test_c2r_real_axis_transforms(gpu, oracle, precision, N, n, m):
+    """The real grid of a c2r plan (axis 2 as a complex transform of half length): a 512-point
+    axis (256-point half transform) and an unpadded axis 2 (n2 = N2, whose k2 = -N2/2 plane is the
+    Nyquist plane of the real transform), trafo and adjoint against the oracle.  The unpadded grid
+    runs in FP64 only: without oversampling the deconvolution factors near the band edge are large
+    enough to lift FP32 rounding above 1e-5 (9e-5 measured for the adjoint)."""
+    rng = SplitMix64(93)
+    M = 1200
+    x = rng.uniform(3 * M).reshape(M, 3) - 0.5
+    if precision == "float":
+        x = x.astype(np.float32).astype(np.float64)
+    fh = oracle.semirandom_hermitian(N, half=True) * 1e-3
+    fv, gv = rng.normal(M), rng.normal(3 * M)
+    out = _run(N, n, m, 0, x, fh, fv, gv, precision=precision)
+    P = oracle.Plan(N, n, m, 0, c2r=True)
+    f_o, g_o = P.trafo_c2r(x, fh, compute_grad=True)
+    tol = 1e-12 if precision == "double" else 1e-5
+    errs = dict(f=rel(out["f"], f_o), g=rel(out["grad"], g_o), a=rel(out["adj"], P.adj_c2r(x, f=fv)),
+                ag=rel(out["adj_grad"], P.adj_c2r(x, grad=gv)))
+    assert all(v < tol for v in errs.values()), errs
+
+
+def test_c2r_grid_is_the_real_transform(gpu, oracle):
+    """The oversampled grid of a c2r plan holds n0 n1 n2 reals: with PNFFT_OMIT_DECONV it equals
+    Re(fftn) of the half spectrum with its k2 < 0 coefficients counted twice (numpy, FP64), and the
+    adjoint's F^H of that real grid equals numpy's inverse transform at the stored frequencies."""
+    import ctypes
+    import torch
+    N, n, m = (16, 8, 32), (32, 16, 64), 4
+    H = N[2] // 2 + 1
+    fh = oracle.semirandom_hermitian(N, half=True).reshape(N[0], N[1], H)
+    dp = DevicePlan(N, n, m, 0, 0, precision="double", c2r=True)
+    dp.set_f_hat(fh)
+    dp.api.trafo(dp.plan, None, PA.PNFFT_OMIT_DECONV | PA.PNFFT_OMIT_CONV)
+    assert dp.api.last_error() is None
+    ptr = dp.api.oversampled_grid(dp.plan)
+    grid = torch.empty(tuple(n), dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipMemcpy(ctypes.c_void_p(grid.data_ptr()), ctypes.c_void_p(ptr), grid.numel() * 8, 3) == 0
+    g = grid.cpu().numpy()
+    e = sum(int(np.frexp(abs(dp.api.psi(dp.plan, t, 0.0)))[1]) - 1 for t in range(3))
+    A = np.zeros(n, np.complex128)
+    i0 = np.arange(-(N[0] // 2), N[0] // 2) % n[0]
+    i1 = np.arange(-(N[1] // 2), N[1] // 2) % n[1]
+    k2 = np.arange(-(N[2] // 2), 1)
+    A[np.ix_(i0, i1, k2 % n[2])] = fh * np.where(k2 < 0, 2.0, 1.0)[None, None, :]
+    ref = np.fft.fftn(A).real * 2.0 ** e
+    assert rel(g, ref) < 1e-12
+    # backward: f_hat = D^T F^H grid over the stored half, with the plan's own D factors
+    dp.api.adj(dp.plan, None, PA.PNFFT_OMIT_CONV)
+    assert dp.api.last_error() is None
+    out = dp.f_hat.cpu().numpy().reshape(N[0], N[1], H)
+    back = (np.fft.ifftn(g) * float(np.prod(n)))[np.ix_(i0, i1, k2 % n[2])]
+    d = [np.array([dp.api.inv_phi_hat(dp.plan, t, k) for k in range(-(N[t] // 2), N[t] - N[t] // 2)])
+         for t in range(3)]
+    D = d[0][:, None, None] * d[1][None, :, None] * d[2][None, None, :H]
+    dp.close()
+    assert rel(out, back * D * 2.0 ** e) < 1e-12
