@@ -402,7 +402,9 @@ struct AxisLoad {
 };
 
 // blocks take the tiles torder[first + blockIdx.x]
-template <int MT, int MODE, int NW>
+// RG: the real grid of a c2r plan (imaginary parts zero: staged as zeros, and
+// the products that would multiply them are left out)
+template <int MT, int MODE, int NW, bool RG>
 __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) void gather_mfma_k(BArgs<float> A,
                                                                                                    int first) {
   using S = GatherMfmaShape<MT, MODE, NW>;
@@ -442,7 +444,7 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
         const int hz = q % H, cy = (q / H) % P, cx = q / (H * P);
         const unsigned g0 = (unsigned)(o0 + cx) & (n0 - 1), g1 = (unsigned)(o1 + cy) & (n1 - 1);
         const unsigned g2 = (unsigned)(o2 + 2 * hz) & (n2 - 1);
-        if (A.c2r) {
+        if constexpr (RG) {
           const float2 r = src_r[((g0 * n1 + g1) * n2 + g2) >> 1];
           v[k] = make_float4(r.x, 0.f, r.y, 0.f);
         } else {
@@ -609,20 +611,20 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
           for (int e = K0; e < K0 + NK; e++) {
             const float bz = xa * comp4(zc, e);
             vr = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), bz, vr, 0, 0, 0);
-            vi = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), bz, vi, 0, 0, 0);
+            if constexpr (!RG) vi = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), bz, vi, 0, 0, 0);
           }
         } else {
           mfma_f4 tr = {0.f, 0.f, 0.f, 0.f}, ti = tr;
 #pragma unroll
           for (int e = K0; e < K0 + NK; e++) {
             tr = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), comp4(zc, e), tr, 0, 0, 0);
-            ti = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), comp4(zc, e), ti, 0, 0, 0);
+            if constexpr (!RG) ti = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), comp4(zc, e), ti, 0, 0, 0);
           }
 #pragma unroll
           for (int e = K0; e < K0 + NK; e++) {
             const float bz = xa * comp4(dzc, e);
             vr = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gr, e), bz, vr, 0, 0, 0);
-            vi = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), bz, vi, 0, 0, 0);
+            if constexpr (!RG) vi = __builtin_amdgcn_mfma_f32_16x16x4f32(comp4(gi, e), bz, vi, 0, 0, 0);
           }
           const float sr = yb.x * tr[0] + yb.y * tr[1] + yb.z * tr[2] + yb.w * tr[3];
           const float si = yb.x * ti[0] + yb.y * ti[1] + yb.z * ti[2] + yb.w * ti[3];
@@ -736,17 +738,24 @@ void launch_gather_mfma(const BArgs<R>& A, hipStream_t s) {
     using S2 = GatherMfmaShape<MT, MODE, 2>;
     const int nh = min(max(A.n_heavy, 0), A.n_torder);
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, A.stale_count);
-    if (nh > 0) {
-      (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)S4::LDS);
-      hipLaunchKernelGGL((gather_mfma_k<MT, MODE, 4>), dim3((unsigned)nh), dim3(S4::NT), S4::LDS, s, A, 0);
-    }
-    if (A.n_torder > nh) {  // tiles of at most 31 nodes: two groups, two waves
-      (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)S2::LDS);
-      hipLaunchKernelGGL((gather_mfma_k<MT, MODE, 2>), dim3((unsigned)(A.n_torder - nh)), dim3(S2::NT), S2::LDS, s,
-                         A, nh);
-    }
+    auto go = [&](auto rgc) {
+      constexpr bool RG = decltype(rgc)::value;
+      if (nh > 0) {
+        (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE, 4, RG>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)S4::LDS);
+        hipLaunchKernelGGL((gather_mfma_k<MT, MODE, 4, RG>), dim3((unsigned)nh), dim3(S4::NT), S4::LDS, s, A, 0);
+      }
+      if (A.n_torder > nh) {  // tiles of at most 31 nodes: two groups, two waves
+        (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE, 2, RG>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)S2::LDS);
+        hipLaunchKernelGGL((gather_mfma_k<MT, MODE, 2, RG>), dim3((unsigned)(A.n_torder - nh)), dim3(S2::NT),
+                           S2::LDS, s, A, nh);
+      }
+    };
+    if (A.c2r)
+      go(std::true_type{});
+    else
+      go(std::false_type{});
     if (A.mid) (void)hipEventRecord(A.mid, s);
     if (A.mid2) (void)hipEventRecord(A.mid2, s);
     hipLaunchKernelGGL((gather_stale_k<MT, MODE>), dim3(256), dim3(256), 0, s, A);

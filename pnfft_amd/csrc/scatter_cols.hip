@@ -270,7 +270,8 @@ struct ScatterLoad {
   Cplx<float> g[WG ? 3 : 1];
 };
 
-template <int MT, int MODE>
+// RG: the real grid of a c2r plan (real node values: the imaginary products are left out)
+template <int MT, int MODE, bool RG>
 __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES + (MODE == B_F && MT <= 4 ? 1 : 0))) void scatter_mfma_k(
     BArgs<float> A) {
   using S = ColsShape<float, MT, MODE>;
@@ -461,7 +462,7 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES + (MODE == B_F && MT <
         if (act & (1u << qq)) {
           const float a = xrow[66 * q4 + 4 * qq] * y;
           cre[qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bre, cre[qq], 0, 0, 0);
-          cim[qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bim, cim[qq], 0, 0, 0);
+          if constexpr (!RG) cim[qq] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bim, cim[qq], 0, 0, 0);
         }
       }
     }
@@ -780,7 +781,10 @@ void launch_m(const BArgs<R>& A, hipStream_t s) {
   bool done = false;
   if constexpr (sizeof(R) == 4 && S::P <= 16) {
     if (!A.no_mfma) {
-      hipLaunchKernelGGL((scatter_mfma_k<MT, MODE>), dim3((unsigned)A.n_torder), dim3(256), 0, s, A);
+      if (A.c2r)
+        hipLaunchKernelGGL((scatter_mfma_k<MT, MODE, true>), dim3((unsigned)A.n_torder), dim3(256), 0, s, A);
+      else
+        hipLaunchKernelGGL((scatter_mfma_k<MT, MODE, false>), dim3((unsigned)A.n_torder), dim3(256), 0, s, A);
       done = true;
     }
   }
