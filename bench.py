@@ -317,7 +317,11 @@ def copy_probe():
 
 def bw_probe(reps=20):
     """Achievable HBM bandwidth, unprofiled: torch's device copy, sum (read only) and fill (write
-    only) of a 1 GiB buffer, one warm-up call each, then `reps` calls timed with events."""
+    only) of a 1 GiB buffer, and the hand-written stream kernels of pnfft_amd/libpnfft_probe.so
+    (copy, nontemporal copy, read, write, and copies with the access patterns of the axis-0
+    column pass and of the 512-point rows pass); one warm-up call each, then `reps` calls timed
+    with events."""
+    import ctypes
     import torch
 
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -338,6 +342,17 @@ def bw_probe(reps=20):
         ms = e0.elapsed_time(e1) / reps
         out[f"{name}_GBps"] = round(mult * (1 << 30) / (ms * 1e-3) / 1e9, 1)
         out[f"{name}_ms"] = round(ms, 4)
+    del src, dst
+    torch.cuda.empty_cache()
+    lib = ctypes.CDLL(os.path.join(ROOT, "pnfft_amd", "libpnfft_probe.so"))
+    lib.pnfft_amd_hbm_probe.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    lib.pnfft_amd_hbm_probe.restype = ctypes.c_int
+    res = (ctypes.c_double * 6)()
+    if lib.pnfft_amd_hbm_probe(res, reps) != 0:
+        raise RuntimeError("HBM probe kernels failed")
+    for k, name in enumerate(("kernel_copy", "kernel_copy_nt", "kernel_read", "kernel_write",
+                              "kernel_copy_cols0_pattern", "kernel_copy_rows_pattern")):
+        out[f"{name}_GBps"] = round(res[k], 1)
     print(json.dumps(out), flush=True)
 
 

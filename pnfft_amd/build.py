@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpnfft_amd.so")
+# the HBM bandwidth probe of bench.py --bw-probe, a library of its own
+PROBE_LIB = os.path.join(HERE, "libpnfft_probe.so")
 SOURCES = ["fft.hip", "nfft_b.hip", "ndft.hip", "api.cpp"]
 # b_kernels.hip is compiled once per (precision, kind, mode): (name suffix, defines)
 B_VARIANTS = [
@@ -75,8 +77,14 @@ def build(verbose=True, force=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    ptmp = PROBE_LIB + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-shared", "-x", "hip", os.path.join(CSRC, "hbm_probe.hip"), "-o", ptmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for hbm_probe.hip:\n{r.stderr[-6000:]}")
+    os.replace(ptmp, PROBE_LIB)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {PROBE_LIB}")
     return LIB
 
 

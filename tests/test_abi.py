@@ -43,6 +43,14 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_probe_library_exports_the_bandwidth_probe():
+    """bench.py --bw-probe loads the stream kernels from their own library (loaded, not called here)."""
+    import ctypes
+    path = os.path.join(os.path.dirname(pnfft_amd.__file__), "libpnfft_probe.so")
+    assert os.path.exists(path), "build() makes libpnfft_probe.so"
+    assert hasattr(ctypes.CDLL(path), "pnfft_amd_hbm_probe")
+
+
 def test_reference_entry_points_present():
     names = set(pnfft_amd.header_symbols())
     for base in ["init_3d", "init_guru", "init_nodes", "get_x", "get_f", "get_f_hat", "get_grad_f", "trafo", "adj",
