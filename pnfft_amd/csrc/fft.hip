@@ -433,22 +433,30 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Rows pass for L = 512 = 8^3: one wave per pencil (four per block), no block
-// barrier.  Lane j holds butterfly j of every stage: the first stage runs on
-// the loaded registers (positions j + 64 q, consecutive lanes on consecutive
-// elements), the middle stage goes through the wave's LDS pencil (element p at
-// p + p / 8, which keeps the stage's reads and writes within two-way bank
-// conflicts), and the last stage's outputs j + 64 q go straight to HBM.  Same
-// Stockham stages, twiddles and per-element hooks as fft_rows.
+// Rows pass for L = 512 = 8^3: one wave per pencil (four per block).  Lane j
+// holds butterfly j of every stage: the first stage runs on the loaded
+// registers (positions j + 64 q, consecutive lanes on consecutive elements),
+// the middle stage goes through the wave's LDS pencil (element p at p + p / 8,
+// which keeps the stage's reads and writes within two-way bank conflicts), and
+// the last stage's outputs j + 64 q go straight to HBM.  Same Stockham stages,
+// twiddles and per-element hooks as fft_rows.  Everything a wave reads from
+// global memory is requested before its first stage: in FP32 the block stages
+// the twiddles in LDS (one barrier), and a backward pass with real factors
+// loads its D^T factors and, when it accumulates, the f_hat values it adds to
+// up front, so a pencil costs one memory round trip instead of three or four.
 constexpr int kRows512Pencils = 4;
 
-template <class R>
+// PRE: a backward pass with real factors or accumulation (the launch checks it)
+template <class R, bool PRE>
 __global__ __launch_bounds__(256) void fft_rows512(PassArgs<R> P) {
   constexpr int L = 512, NP = L + L / 8;
+  constexpr bool TW_LDS = sizeof(R) == 4;
+  constexpr int NTW = 448;  // twiddles used: tw[8 k q] (k, q < 8) and tw[j q] (j < 64, q < 8), all below 448
   __shared__ Cplx<R> sh[kRows512Pencils][NP];
+  __shared__ Cplx<R> stw[TW_LDS ? NTW : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = blockIdx.x * kRows512Pencils + wave;
-  if (col >= P.n_inner) return;  // whole wave; no barrier follows
+  const bool live = col < P.n_inner;
   Cplx<R>* pen = sh[wave];
   auto pad = [](int p) { return p + (p >> 3); };
   Cplx<R> v[8];
@@ -456,8 +464,28 @@ __global__ __launch_bounds__(256) void fft_rows512(PassArgs<R> P) {
   for (int q = 0; q < 8; q++) {
     const int e = load_index(P, lane + 64 * q, L);
     v[q] = cmk<R>(R(0), R(0));
-    if (e >= 0) v[q] = P.in[(long)col * P.in_outer + e];
+    if (live && e >= 0) v[q] = P.in[(long)col * P.in_outer + e];
   }
+  // backward pass with real factors: D^T factors and the accumulated values, loaded now
+  double fac[PRE ? 8 : 1];
+  Cplx<R> acc[PRE ? 8 : 1];
+  if constexpr (PRE) {
+    const double f01 = P.tab0 ? P.tab0[col / P.N1] * P.tab1[col % P.N1] : 1.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = store_index(P, lane + 64 * q, L);
+      fac[q] = (P.tab0 && e >= 0) ? f01 * P.tab2[e] : 1.0;
+      acc[q] = cmk<R>(R(0), R(0));
+      if (live && P.accumulate && e >= 0) acc[q] = P.out[(long)col * P.out_outer + e];
+    }
+  }
+  const Cplx<R>* tw = P.tw;
+  if constexpr (TW_LDS) {
+    for (int i = threadIdx.x; i < NTW; i += 256) stw[i] = P.tw[i];
+    __syncthreads();
+    tw = stw;
+  }
+  if (!live) return;  // whole wave; no barrier follows
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int e = load_index(P, lane + 64 * q, L);
@@ -474,7 +502,7 @@ __global__ __launch_bounds__(256) void fft_rows512(PassArgs<R> P) {
 #pragma unroll
     for (int q = 0; q < 8; q++) v[q] = pen[pad(lane + 64 * q)];
 #pragma unroll
-    for (int q = 1; q < 8; q++) v[q] = cmul(v[q], P.tw[8 * k * q]);
+    for (int q = 1; q < 8; q++) v[q] = cmul(v[q], tw[8 * k * q]);
     dft8(v);
     wave_sync();  // every read of the pencil precedes the writes
     const int base = (lane - k) * 8 + k;
@@ -486,12 +514,22 @@ __global__ __launch_bounds__(256) void fft_rows512(PassArgs<R> P) {
 #pragma unroll
   for (int q = 0; q < 8; q++) v[q] = pen[pad(lane + 64 * q)];
 #pragma unroll
-  for (int q = 1; q < 8; q++) v[q] = cmul(v[q], P.tw[lane * q]);
+  for (int q = 1; q < 8; q++) v[q] = cmul(v[q], tw[lane * q]);
   dft8(v);
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int e = store_index(P, lane + 64 * q, L);
-    if (e >= 0) on_store(P, v[q], col, e, (long)col * P.out_outer + e);
+    if (e < 0) continue;
+    const long addr = (long)col * P.out_outer + e;
+    if constexpr (PRE) {
+      // on_store with the factors and accumulated values loaded above
+      Cplx<R> w = cconj(v[q]);
+      w = cmk<R>(R((double)w.x * fac[q]), R((double)w.y * fac[q]));
+      if (P.accumulate) w = cadd(acc[q], w);
+      P.out[addr] = w;
+    } else {
+      on_store(P, v[q], col, e, addr);
+    }
   }
 }
 
@@ -745,7 +783,10 @@ template <class R, int L>
 static void pass_launch(int kind, dim3 grid, const PassArgs<R>& P, hipStream_t s) {
   if (kind == kPassRows) {
     if constexpr (L == 512) {
-      hipLaunchKernelGGL((fft_rows512<R>), grid, dim3(256), 0, s, P);
+      if (P.backward && (P.tab0 != nullptr || P.accumulate) && !P.ctab)
+        hipLaunchKernelGGL((fft_rows512<R, true>), grid, dim3(256), 0, s, P);
+      else
+        hipLaunchKernelGGL((fft_rows512<R, false>), grid, dim3(256), 0, s, P);
     } else {
       constexpr size_t lds = PassShape<R, L>::LDS;
       hipLaunchKernelGGL((fft_rows<R, L>), grid, dim3(kThreads), lds, s, P);
