@@ -554,6 +554,15 @@ __global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW, F
   constexpr int NBL = S::EPT / RL, nbfL = L / RL;
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * S::C;
+  // FP32 512-point passes read their twiddles from LDS (staged here; the first
+  // block barrier below precedes every use)
+  constexpr bool TW_LDS = sizeof(R) == 4 && L == 512;
+  __shared__ Cplx<R> stw[TW_LDS ? L : 1];
+  const Cplx<R>* tw = P.tw;
+  if constexpr (TW_LDS) {
+    for (int i = tid; i < L; i += NT) stw[i] = P.tw[i];
+    tw = stw;
+  }
   const long in_base = (long)blockIdx.y * P.in_outer;
   const long out_base = (long)blockIdx.y * P.out_outer;
   Cplx<R> v[S::EPT];
@@ -634,7 +643,7 @@ __global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW, F
       const int j = j0 + 16 * i, k = j & 7;
       Cplx<R>* w = u + i * 8;
 #pragma unroll
-      for (int q = 1; q < 8; q++) w[q] = cmul(w[q], P.tw[8 * k * q]);
+      for (int q = 1; q < 8; q++) w[q] = cmul(w[q], tw[8 * k * q]);
       dft8(w);
     }
     // stage 2 outputs -> stage 3 inputs (j + 64 q)
@@ -666,7 +675,7 @@ __global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW, F
       const int j = j0 + 16 * i;
       Cplx<R>* w = v + i * 8;
 #pragma unroll
-      for (int q = 1; q < 8; q++) w[q] = cmul(w[q], P.tw[j * q]);
+      for (int q = 1; q < 8; q++) w[q] = cmul(w[q], tw[j * q]);
       dft8(w);
       if (col < P.n_inner) {
 #pragma unroll
@@ -687,7 +696,7 @@ __global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW, F
       for (int q = 0; q < R1; q++) sh[(j * R1 + q) * S::LD + c] = v[i * R1 + q];
     }
     __syncthreads();
-    stages_lds<R, L, R1, NsL, NT, S::EPT>(sh, P.tw);
+    stages_lds<R, L, R1, NsL, NT, S::EPT>(sh, tw);
     // ---- last stage (span NsL): outputs at positions j + q NsL
 #pragma unroll
     for (int i = 0; i < NBL; i++) {
@@ -702,7 +711,7 @@ __global__ __launch_bounds__((cols_threads<R, L>()), (cols_split<R, L, NARROW, F
       const int bf = tid + i * NT;
       const int j = bf / S::C;
 #pragma unroll
-      for (int q = 1; q < RL; q++) v[i * RL + q] = cmul(v[i * RL + q], P.tw[j * q * twstep]);
+      for (int q = 1; q < RL; q++) v[i * RL + q] = cmul(v[i * RL + q], tw[j * q * twstep]);
       dft<R, RL>(v + i * RL);
     }
   }
