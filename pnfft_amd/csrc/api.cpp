@@ -294,6 +294,10 @@ struct NodesT {
   size_t slots_cap = 0;  // complex elements
   int n_torder = 0;
   int n_heavy = 0;       // torder[0, n_heavy): tiles with 32 nodes or more
+  // offsets of each tile's nodes per axis (build_tile_boxes), by tile id and by position in torder
+  unsigned* d_box_tile = nullptr;
+  unsigned* d_box_ord = nullptr;
+  bool boxes = false;
   int* d_stale = nullptr;            // [M] sorted positions of nodes outside their tile
   unsigned* d_stale_count = nullptr;
   // window tables of precompute_psi (sorted order, see precompute_tables)
@@ -618,6 +622,9 @@ static void nodes_release_device(NodesT<R>* D) {
   dfree(D->d_torder);
   dfree(D->d_tile_slot);
   dfree(D->d_twork);
+  dfree(D->d_box_tile);
+  dfree(D->d_box_ord);
+  D->boxes = false;
   dfree(D->d_slots);
   D->slots_cap = 0;
   D->n_torder = 0;
@@ -729,6 +736,8 @@ static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
     dfree(D->d_torder);
     dfree(D->d_tile_slot);
     dfree(D->d_twork);
+    dfree(D->d_box_tile);
+    dfree(D->d_box_ord);
     D->d_tile_start = dalloc<int>(T.ntiles() + 1);
     D->d_chunks = dalloc<unsigned>(2 * T.ntiles());
     D->d_tpartial = dalloc<unsigned>((T.ntiles() + 4095) / 4096 + 1);
@@ -737,8 +746,10 @@ static bool ensure_sort_capacity(NodesT<R>* D, const TileLayout& T) {
     D->d_torder = dalloc<int>(T.ntiles());
     D->d_tile_slot = dalloc<int>(T.ntiles());
     D->d_twork = dalloc<unsigned>(65);
+    D->d_box_tile = dalloc<unsigned>(T.ntiles());
+    D->d_box_ord = dalloc<unsigned>(T.ntiles());
     if (!D->d_tile_start || !D->d_chunks || !D->d_tpartial || !D->d_dense || !D->d_dense_s || !D->d_torder ||
-        !D->d_tile_slot || !D->d_twork)
+        !D->d_tile_slot || !D->d_twork || !D->d_box_tile || !D->d_box_ord)
       return false;
     D->tile_cap = T.ntiles() + 1;
   }
@@ -808,6 +819,10 @@ static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false)
     A.slots = D->d_slots;
     A.tile_slot = D->d_tile_slot;
   }
+  if (D->n_torder > 0 && D->boxes) {
+    A.box_tile = D->d_box_tile;
+    A.box_ord = D->d_box_ord;
+  }
   for (int t = 0; t < 3; t++) {
     A.ts[t] = D->bin_tiles.ts[t];
     A.ntile[t] = D->bin_tiles.ntile[t];
@@ -865,6 +880,8 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx, const GridGeom& g) {
     // without slot memory the scatter takes the LDS-atomic path
     D->n_torder = D->d_slots ? nt : 0;
   }
+  D->boxes = D->n_torder > 0 && build_tile_boxes(D->d_keys, D->M, P->tiles, D->d_torder, D->n_torder,
+                                                 D->d_box_tile, D->d_box_ord, P->stream) == kOk;
   D->bin_geom = g;
   D->bin_tiles = P->tiles;
   D->bins_valid = true;

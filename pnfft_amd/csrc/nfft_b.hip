@@ -452,6 +452,53 @@ int cols_pad(int m, bool is_double, const int* ts) {
   return TS + 2 * m;
 }
 
+// Offsets of the nodes of every tile, per axis, as bit masks: the in-tile cell
+// index of a sorted key (the layout of bin_keys_k: runs of q = min(TS2, 4)
+// cells along axis 2 outermost, then axis 0, axis 1 and the cell within the
+// run) is decoded and OR-ed into box[tile] (bits 8 t + l for offset l on axis
+// t; tile sizes of at most 8).
+// The keys are sorted, so most waves see one tile: they OR their lanes' masks
+// first and make one atomic instead of 64 on the same word.
+__global__ __launch_bounds__(kB) void tile_box_k(const unsigned* __restrict__ keys, long M, TileLayout T,
+                                                 unsigned* box) {
+  const long i = (long)blockIdx.x * kB + threadIdx.x;
+  const bool live = i < M;
+  const unsigned vol = (unsigned)(T.ts[0] * T.ts[1] * T.ts[2]);
+  const unsigned key = live ? keys[i] : 0u, tile = key / vol, in = key % vol;
+  const unsigned q = T.ts[2] < 4 ? (unsigned)T.ts[2] : 4u;
+  const unsigned r = in % q, rest = in / q;  // rest = (l2 / q) * TS0 * TS1 + l0 * TS1 + l1
+  const unsigned l1 = rest % (unsigned)T.ts[1], l0 = (rest / (unsigned)T.ts[1]) % (unsigned)T.ts[0];
+  const unsigned l2 = (rest / (unsigned)(T.ts[0] * T.ts[1])) * q + r;
+  unsigned m = live ? (1u << l0) | (1u << (8 + l1)) | (1u << (16 + l2)) : 0u;
+  const unsigned t0 = (unsigned)__builtin_amdgcn_readfirstlane((int)tile);
+  const bool one = __ballot(live && tile != t0) == 0ull;  // every live lane holds tile t0
+  if (one) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) m |= __shfl_xor(m, d);
+    if ((threadIdx.x & 63) == 0 && m) atomicOr(box + t0, m);
+  } else if (live) {
+    atomicOr(box + tile, m);
+  }
+}
+
+// box_ord[p] = box_tile[order[p]]: the masks by position in the tile order
+__global__ __launch_bounds__(kB) void box_order_k(const int* __restrict__ order, int n, const unsigned* __restrict__ box_tile,
+                                                  unsigned* box_ord) {
+  const int p = blockIdx.x * kB + threadIdx.x;
+  if (p < n) box_ord[p] = box_tile[order[p]];
+}
+
+int build_tile_boxes(const unsigned* keys, long M, const TileLayout& T, const int* order, int n_order,
+                     unsigned* box_tile, unsigned* box_ord, hipStream_t s) {
+  if (T.ts[0] > 8 || T.ts[1] > 8 || T.ts[2] > 8) return kUnsupported;
+  const long ntiles = T.ntiles();
+  hipLaunchKernelGGL(fill_uint_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, box_tile, ntiles, 0u);
+  if (M > 0) hipLaunchKernelGGL(tile_box_k, dim3(grid1d(M, kB)), dim3(kB), 0, s, keys, M, T, box_tile);
+  if (n_order > 0)
+    hipLaunchKernelGGL(box_order_k, dim3(grid1d(n_order, kB)), dim3(kB), 0, s, order, n_order, box_tile, box_ord);
+  return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
 int build_tile_order(const int* tile_start, long ntiles, unsigned* work, unsigned* light, unsigned* partial,
                      int* order, int* tile_slot, int* n_heavy, hipStream_t s) {
   hipLaunchKernelGGL(fill_uint_k, dim3(1), dim3(128), 0, s, work, 65L, 0u);

@@ -41,6 +41,11 @@ struct BArgs {
   // column scatter (scatter_cols.hip): when slots is set, scatter() runs it instead of the LDS-atomic path
   Cplx<R>* slots;          // [n_torder][P^3] padded subgrid of every nonempty tile
   const int* tile_slot;    // [ntiles] slot of each tile, -1 if empty
+  // offsets of each tile's nodes at binning time, per axis (bits 8 t + l for
+  // offset l on axis t), by tile id and by position in torder (or null)
+  const unsigned* box_tile;
+  const unsigned* box_ord;
+  int box_slots;           // scatter: slots hold only the cells of their box (the fold reads box_tile)
   int no_mfma;             // 1: vector kernels also where the matrix-core gather / scatter apply
   // window tables of pnfft_precompute_psi in the sorted order, [3 CM][M] each (or null)
   const R* pre_psi;
@@ -96,6 +101,16 @@ struct TileLayout {
   long ntiles() const { return (long)ntile[0] * ntile[1] * ntile[2]; }
   long tile_vol() const { return (long)ts[0] * ts[1] * ts[2]; }
 };
+
+// Per-tile offset masks of the nodes whose sorted bin keys are keys[0, M)
+// (tile layout T, tile sizes of at most 8): box_tile[ntiles] by tile id and
+// box_ord[n_order] by position in order.  The slots of the matrix-core column
+// scatter (with the brick fold) hold only the box these masks span (from the
+// smallest offset to the largest plus 2m per axis); a node whose stencil
+// leaves that box (moved since binning) takes the stale path.  Returns kOk,
+// kUnsupported for larger tiles, or kHipError.
+int build_tile_boxes(const unsigned* keys, long M, const TileLayout& T, const int* order, int n_order,
+                     unsigned* box_tile, unsigned* box_ord, hipStream_t s);
 
 // Sorting scratch (device): keys/values double-buffered, histograms, scan partials.
 struct SortWork {

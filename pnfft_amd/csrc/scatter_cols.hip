@@ -311,6 +311,13 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES + (MODE == B_F && MT <
   const float wst = (float)(ax == 0 ? A.g.ws[0] : (ax == 1 ? A.g.ws[1] : A.g.ws[2]));
   const long M = A.M;
   const bool tab = A.pre_psi != nullptr && (!WG || A.pre_dpsi != nullptr);
+  // With box slots the slot holds only the box the tile's nodes reached at
+  // binning time (per axis, the smallest offset to the largest plus 2m); a node
+  // whose offset now lies outside it takes the stale path like a node outside
+  // its tile.  bml: the offsets of this thread's axis.
+  const unsigned bm = A.box_slots ? A.box_ord[blockIdx.x] : 0x00FFFFFFu;
+  const unsigned bml = (unsigned)__builtin_amdgcn_readfirstlane((int)((bm >> (8 * ax)) & 0xFFu));  // one axis per wave
+  const int blo = bml ? __builtin_ctz(bml) : 0, bhi = bml ? 31 - __builtin_clz(bml) : -1;
 
   // The loads below are unconditional, from clamped positions (their values
   // are discarded where no node exists), so that no register copy of a loaded
@@ -366,7 +373,7 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES + (MODE == B_F && MT <
       if (L.j >= 0) {
         const NodeAxis<float> na = node_axis<float>(L.x, nt, MT, A.g.shift);
         l = local_off(na.start, ot, TS, nt);
-        bad = l < 0;
+        bad = l < blo || l > bhi;  // outside the tile (l < 0) or outside the slot's box
         if (tab && L.x == L.px) {
 #pragma unroll
           for (int s = 0; s < CM; s++) {
@@ -473,13 +480,21 @@ __global__ __launch_bounds__(256, (MfmaShape<MODE>::WAVES + (MODE == B_F && MT <
   }
 
   Cplx<float>* slot = A.slots + (size_t)blockIdx.x * (size_t)(P * P * P);
+  // cells of the box per axis (all cells without box slots)
+  auto cells = [](unsigned m) {  // offsets m -> cells lo .. hi + 2m
+    if (!m) return 0u;
+    const int lo = __builtin_ctz(m), hi = 31 - __builtin_clz(m) + CM - 1;
+    return ((2u << hi) - 1u) & ~((1u << lo) - 1u);
+  };
+  const unsigned cx = cells(bm & 0xFFu), cy = cells((bm >> 8) & 0xFFu), cz = cells((bm >> 16) & 0xFFu);
 #pragma unroll
   for (int qq = 0; qq < 4; qq++) {
     const int a = wave + 4 * qq;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       const int b = 4 * lk + e;
-      if (a < P && b < P && li < P) slot[(a * P + b) * P + li] = cmk<float>(cre[qq][e], cim[qq][e]);
+      if (a < P && b < P && li < P && ((cx >> a) & (cy >> b) & (cz >> li) & 1u))
+        slot[(a * P + b) * P + li] = cmk<float>(cre[qq][e], cim[qq][e]);
     }
   }
 }
@@ -621,6 +636,11 @@ __global__ __launch_bounds__(256) void fold8b_k(BArgs<R> A) {
   const int T2 = (B % nb2) * 2, T1 = (B / nb2) % nt1, T0 = B / (nb2 * nt1);
   const int h = threadIdx.x & 3, q1 = (threadIdx.x >> 2) & 7, q0 = threadIdx.x >> 5;
   const int c = 4 * h;  // first of the thread's four cells in the brick
+  auto cells = [](unsigned m) {  // offsets m -> cells lo .. hi + 2m of a slot
+    if (!m) return 0u;
+    const int lo = __builtin_ctz(m), hi = 31 - __builtin_clz(m) + P - TS;
+    return ((2u << hi) - 1u) & ~((1u << lo) - 1u);
+  };
   double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int d0 = 0; d0 < D; d0++) {
@@ -634,13 +654,29 @@ __global__ __launch_bounds__(256) void fold8b_k(BArgs<R> A) {
 #pragma unroll
       for (int j = 0; j < 3; j++) {
         const int s2 = ((T2 - 1 + j) % nt2 + nt2) % nt2;
-        const int slot = A.tile_slot[((long)s0 * nt1 + s1) * nt2 + s2];
+        const long tix = ((long)s0 * nt1 + s1) * nt2 + s2;
+        const int slot = A.tile_slot[tix];
         if (slot < 0) continue;
         const int p2 = c + 8 - 8 * j;
         if (row_ok && p2 >= 0 && p2 < P) {
+          // box slots: only the cells of the slot's box were written; zq marks
+          // which of the four cells lie in it
+          unsigned zq = 0xFu;
+          if (A.box_slots) {
+            const unsigned bmb = A.box_tile[tix];
+            if (!((cells(bmb & 0xFFu) >> p0) & (cells((bmb >> 8) & 0xFFu) >> p1) & 1u)) continue;
+            zq = (cells((bmb >> 16) & 0xFFu) >> p2) & 0xFu;
+            if (!zq) continue;
+          }
           const Cplx<R>* src = A.slots + (size_t)slot * VOL + ((size_t)p0 * P + p1) * P + p2;
           if constexpr (sizeof(R) == 4) {  // two aligned 16-byte loads (P and p2 multiples of 4)
-            const float4 a = reinterpret_cast<const float4*>(src)[0], b = reinterpret_cast<const float4*>(src)[1];
+            float4 a = reinterpret_cast<const float4*>(src)[0], b = reinterpret_cast<const float4*>(src)[1];
+            if (zq != 0xFu) {
+              if (!(zq & 1u)) a.x = a.y = 0.f;
+              if (!(zq & 2u)) a.z = a.w = 0.f;
+              if (!(zq & 4u)) b.x = b.y = 0.f;
+              if (!(zq & 8u)) b.z = b.w = 0.f;
+            }
             sr[0] += (double)a.x;
             si[0] += (double)a.y;
             sr[1] += (double)a.z;
@@ -652,6 +688,7 @@ __global__ __launch_bounds__(256) void fold8b_k(BArgs<R> A) {
           } else {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
+              if (!((zq >> k) & 1u)) continue;
               sr[k] += (double)src[k].x;
               si[k] += (double)src[k].y;
             }
@@ -775,8 +812,13 @@ __global__ void zero_count_k(unsigned* c) {
 inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 template <class R, int MT, int MODE>
-void launch_m(const BArgs<R>& A, hipStream_t s) {
+void launch_m(const BArgs<R>& A0, hipStream_t s) {
   using S = ColsShape<R, MT, MODE>;
+  // box slots where the matrix-core scatter writes the slots and fold8b_k reads them
+  BArgs<R> A = A0;
+  A.box_slots = 0;
+  if constexpr (sizeof(R) == 4 && S::TS == 8 && S::P <= 16 && S::P % 4 == 0)
+    A.box_slots = (A.box_tile && A.box_ord && !A.no_mfma && A.ts[2] == 8 && A.ntile[2] % 2 == 0) ? 1 : 0;
   hipLaunchKernelGGL(zero_count_k, dim3(1), dim3(64), 0, s, A.stale_count);
   bool done = false;
   if constexpr (sizeof(R) == 4 && S::P <= 16) {
