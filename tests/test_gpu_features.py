@@ -200,6 +200,42 @@ def test_omit_flags(gpu, oracle):
     assert rel(f2, f_ref) < 1e-12
 
 
+def test_set_b_changes_the_window_shape(gpu, oracle):
+    """set_b (api/api-basic.c:587-596) replaces the Kaiser-Bessel shape parameter per axis: the
+    plan's D factors, window scale and windows follow it, so trafo (f, grad) and adj match the
+    oracle run with the same b (its D tables rebuilt from inv_phi_hat with that b), get_b
+    returns it, and the outputs differ from those of the default b."""
+    import ctypes
+    N, n, M, m = (8, 16, 8), (16, 32, 16), 200, 4
+    x, fh, fv, _ = _inputs(41, N, M, False, oracle)
+    P = oracle.Plan(N, n, m, 0)
+    f_def, _ = P.trafo(x, fh)
+    b_new = np.array([0.75 * P.b[0], 1.1 * P.b[1], 0.7 * P.b[2]])
+    P.b = b_new
+    P.tab = np.concatenate([[oracle.inv_phi_hat(P.window, i - N[t] // 2, n[t], b_new[t], m) for i in range(N[t])]
+                            for t in range(3)])
+    f_o, g_o = P.trafo(x, fh, compute_grad=True)
+    a_o = P.adj(x, f=fv)
+    assert rel(f_o, f_def) > 1e-6  # the new shape changes the transform beyond the tolerance below
+    say("case set_b", b_new)
+    dp = DevicePlan(N, n, m, 0, M)
+    dp.api.set_b(float(b_new[0]), float(b_new[1]), float(b_new[2]), dp.plan)
+    got = [ctypes.c_double() for _ in range(3)]
+    dp.api.get_b(dp.plan, *[ctypes.byref(v) for v in got])
+    assert [v.value for v in got] == list(b_new)
+    out = _run(dp, x, fh, fv, None, F | G, want_adj=False)
+    f, g = out["f"], out["g"]
+    dp.set_f(fv)
+    dp.adj(F)
+    a = dp.f_hat.cpu().numpy().reshape(-1).copy()
+    err = dp.api.last_error()
+    dp.close()
+    assert err is None, err
+    assert rel(f, f_o) < 1e-12
+    assert rel(g, g_o) < 1e-12
+    assert rel(a, a_o) < 1e-12
+
+
 def test_interlaced_vjp_is_rejected(gpu):
     dp = DevicePlan((8, 8, 8), None, 3, IL, 16)
     w = torch.zeros(16, dtype=torch.complex128, device="cuda:0")
