@@ -141,6 +141,47 @@ __host__ __device__ inline R kb_psi(R u, R b, int m) {
   return (d > R(0)) ? m_sinh(b * r) / (pi * r) : b / pi;
 }
 
+// kb_psi for |u| <= m (d >= 0): the sinh branch and the edge value only.  For
+// u = (s - m) - frac with s >= 1 and frac in [0, 1], |u| <= m holds in floating
+// point as well (the exact value does not exceed m, a representable number),
+// so a stencil's points s >= 1 never take the sin branch.
+template <class R>
+__host__ __device__ inline R kb_psi_in(R u, R b, int m) {
+  const R d = R(m) * R(m) - u * u;
+  const R r = m_sqrt(d);
+  const R pi = pi_v(R(0));
+  return (d > R(0)) ? m_sinh(b * r) / (pi * r) : b / pi;
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// FP32 Kaiser-Bessel at a stencil point with |u| <= m (the points s >= 1, see
+// kb_psi_in), psi and psi' from one exponential.  With r = sqrt(m^2 - u^2) and
+// z = b r:  psi = (b / pi) S(z), S(z) = sinh z / z, and psi' = n xn b^3 H(z) / pi,
+// H(z) = (sinh z - z cosh z) / z^3, xn = -u.  For z >= 1, E = e^z gives
+// S = (E - 1/E) / (2 z) and H = ((E - 1/E) - z (E + 1/E)) / (2 z^3); below 1
+// their Taylor series S = sum_k z^2k / (2k+1)!, H = -sum_{k>=1} 2k z^(2k-2) / (2k+1)!
+// (truncated after z^10, remainder < 2e-10).  The d = 0 edge takes the reference's
+// value -n m b^3 / (3 pi) for psi'.
+__device__ inline void kb_in_f32(float u, float b, int m, float nR, float& psi, float* dpsi) {
+  const float d = float(m) * float(m) - u * u;
+  const float r = __builtin_amdgcn_sqrtf(d);
+  const float z = b * r, z2 = z * z;
+  const float E = expf(z), Ei = __builtin_amdgcn_rcpf(E), iz = __builtin_amdgcn_rcpf(z);
+  const bool small = z < 1.f;
+  const float Ss = 1.f + z2 * (1.f / 6.f + z2 * (1.f / 120.f + z2 * (1.f / 5040.f + z2 * (1.f / 362880.f +
+                                                                                            z2 * (1.f / 39916800.f)))));
+  const float S = small ? Ss : 0.5f * (E - Ei) * iz;
+  const float bpi = b * 0.318309886183790671538f;  // b / pi
+  psi = bpi * S;
+  if (dpsi) {
+    const float Hs = -(2.f / 6.f + z2 * (4.f / 120.f + z2 * (6.f / 5040.f + z2 * (8.f / 362880.f +
+                                                                                    z2 * (10.f / 39916800.f + z2 * (12.f / 6227020800.f))))));
+    const float H = small ? Hs : 0.5f * ((E - Ei) - z * (E + Ei)) * (iz * iz * iz);
+    *dpsi = (d == 0.f) ? -nR * float(m) * b * b * b / (3.f * 3.14159265358979323846f) : nR * (-u) * b * b * bpi * H;
+  }
+}
+#endif
+
 // Kaiser-Bessel psi' at xn = n x - l (kaiser_bessel_derivative_1d, :2256-2269).
 // FP64 keeps the reference's formula; FP32 uses the cancellation-free form
 // n xn b^3 h(b r)/pi.  The d == 0 value is the reference's -n m b^3/(3 pi) on
@@ -158,6 +199,17 @@ __host__ __device__ inline R kb_dpsi(R xn, R nR, R b, int m, R psi) {
   const R z = b * r;
   if (d > R(0)) return nR * xn * b * b * b * kb_h_sinh(z) / pi;
   return -nR * xn * b * b * b * kb_h_sin(z) / pi;
+}
+
+// kb_dpsi for |xn| <= m (d >= 0), as kb_psi_in: xn = frac + (m - s) with s >= 1
+template <class R>
+__host__ __device__ inline R kb_dpsi_in(R xn, R nR, R b, int m, R psi) {
+  const R d = R(m) * R(m) - xn * xn;
+  const R r = m_sqrt(d);
+  const R pi = pi_v(R(0));
+  if (d == R(0)) return -nR * R(m) * b * b * b / (R(3) * pi);
+  if (sizeof(R) == sizeof(double)) return nR * xn / d * (psi - b * m_cosh(b * r) / pi);
+  return nR * xn * b * b * b * kb_h_sinh(b * r) / pi;
 }
 
 // Window values of one axis.  frac in [0,1), nR = n as R.  dpsi may be null.
@@ -316,12 +368,21 @@ __host__ __device__ inline void window_axis_s(int win, R frac, R nR, R b, R (&ps
       }
       break;
     }
-    default: {  // Kaiser-Bessel
+    default: {  // Kaiser-Bessel: only the point s = 0 can lie outside |u| <= m
+      psi[0] = kb_psi<R>(R(-m) - frac, b, m);
+      if (WD) dpsi[0] = kb_dpsi<R>(frac + R(m), nR, b, m, psi[0]);
+#if defined(__HIP_DEVICE_COMPILE__)
+      if constexpr (sizeof(R) == 4) {
 #pragma unroll
-      for (int s = 0; s < CM; s++) psi[s] = kb_psi<R>(R(s - m) - frac, b, m);
+        for (int s = 1; s < CM; s++) kb_in_f32(R(s - m) - frac, b, m, nR, psi[s], WD ? &dpsi[s] : nullptr);
+        break;
+      }
+#endif
+#pragma unroll
+      for (int s = 1; s < CM; s++) psi[s] = kb_psi_in<R>(R(s - m) - frac, b, m);
       if (WD) {
 #pragma unroll
-        for (int s = 0; s < CM; s++) dpsi[s] = kb_dpsi<R>(frac + R(m - s), nR, b, m, psi[s]);
+        for (int s = 1; s < CM; s++) dpsi[s] = kb_dpsi_in<R>(frac + R(m - s), nR, b, m, psi[s]);
       }
       break;
     }
