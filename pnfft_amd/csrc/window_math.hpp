@@ -299,19 +299,24 @@ __host__ __device__ inline R sincpow_c2(R y) {
 
 // bspline_values with the order K fixed at compile time: same recursion and
 // operation order, every index static so V stays in registers.
+// one level r of the recursion below: the r values of order r from the r - 1
+// values of order r - 1 in V[0, r - 1), in place
+template <class R>
+__host__ __device__ inline void bspline_level(R u, int r, R* V) {
+  const R inv = R(1) / R(r - 1);
+  V[r - 1] = R(0);
+#pragma unroll
+  for (int d = r - 1; d >= 0; d--) {
+    const R left = (d > 0) ? V[d - 1] : R(0);
+    V[d] = ((u + R(d)) * V[d] + (R(r) - u - R(d)) * left) * inv;
+  }
+}
+
 template <int K, class R>
 __host__ __device__ inline void bspline_values_s(R u, R (&V)[K]) {
   V[0] = R(1);
 #pragma unroll
-  for (int r = 2; r <= K; r++) {
-    const R inv = R(1) / R(r - 1);
-    V[r - 1] = R(0);
-#pragma unroll
-    for (int d = r - 1; d >= 0; d--) {
-      const R left = (d > 0) ? V[d - 1] : R(0);
-      V[d] = ((u + R(d)) * V[d] + (R(r) - u - R(d)) * left) * inv;
-    }
-  }
+  for (int r = 2; r <= K; r++) bspline_level(u, r, V);
 }
 
 // window_axis for a stencil width CM = 2m+1 known at compile time (the device
@@ -337,13 +342,21 @@ __host__ __device__ inline void window_axis_s(int win, R frac, R nR, R b, R (&ps
     case WIN_BSPLINE: {
       R V[2 * m];
       const R u = R(1) - frac;
-      bspline_values_s<2 * m, R>(u, V);
+      R W[2 * m - 1];
+      if (WD) {
+        // the order 2m - 1 values of the derivative are the recursion's last
+        // level but one: one more level gives the window values
+        bspline_values_s<2 * m - 1, R>(u, W);
+#pragma unroll
+        for (int d = 0; d < 2 * m - 1; d++) V[d] = W[d];
+        bspline_level(u, 2 * m, V);
+      } else {
+        bspline_values_s<2 * m, R>(u, V);
+      }
       psi[0] = R(0);
 #pragma unroll
       for (int s = 1; s < CM; s++) psi[s] = V[s - 1];
       if (WD) {
-        R W[2 * m - 1];
-        bspline_values_s<2 * m - 1, R>(u, W);
 #pragma unroll
         for (int s = 0; s < CM; s++) {
           const R a = (s >= 1 && s - 1 < 2 * m - 1) ? W[s - 1] : R(0);
