@@ -385,8 +385,15 @@ struct GatherMfmaShape {
   // scalars), y rows of 20 floats (read as float4 chunks), at X 0, Y YOFF,
   // Z ZOFF; the derivative rows follow at ROWSZ / 2.  The odd and the 20-float
   // pitches keep the rows' stores and reads in distinct banks without a swizzle.
-  static constexpr int YOFF = 16 * 17, ZOFF = YOFF + 16 * 20, ROWSZ = (WG ? 2 : 1) * (ZOFF + 16 * 17);
-  static constexpr int WAVES = WG ? 2 : 3;  // waves per SIMD the kernel is compiled for
+  // With gradients in four-wave blocks (SYZ) the z rows take the y rows' place
+  // once those are read (both are read once per group), so such a block fits
+  // three times per CU (two-wave blocks fit three times either way).
+  static constexpr bool SYZ = WG && NW == 4;
+  static constexpr int YOFF = 16 * 17, ZOFF = SYZ ? YOFF : YOFF + 16 * 20;
+  static constexpr int ROWSZ = (WG ? 2 : 1) * (SYZ ? YOFF + 16 * 20 : ZOFF + 16 * 17);
+  // waves per SIMD the kernel is compiled for (the gradient kernels of m = 5, 6
+  // need more than the 168 VGPRs of three)
+  static constexpr int WAVES = WG && (MT >= 5 || !SYZ) ? 2 : 3;
   static constexpr int NT = 64 * NW;        // threads per block
   static constexpr size_t LDS = sizeof(float) * (2 * P * 256 + NW * ROWSZ);
 };
@@ -528,12 +535,22 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
     // the row zero, so it adds nothing; it is listed for the stale kernel below)
     int l = 0;
     bool bad = false;
+    // window values of this lane's axis (the z lanes of SYZ write theirs after the y rows are read)
+    float zw[S::SYZ ? CM : 1], zdw[S::SYZ ? CM : 1];
+    const bool zlate = S::SYZ && lk == 2;
     if (lk < 3 && jg >= 0) {
       const NodeAxis<float> na = node_axis<float>(L.x, nt, MT, A.g.shift);
       l = local_off(na.start, ot, TS, nt);
       bad = l < 0;
       if (tab && L.x == L.px) {
-        if (!bad) {
+        if constexpr (S::SYZ) {
+#pragma unroll
+          for (int s = 0; s < CM; s++) {
+            zw[s] = L.psi[s];
+            zdw[s] = L.dpsi[s];
+          }
+        }
+        if (!bad && !zlate) {
 #pragma unroll
           for (int s = 0; s < CM; s++) {
             r0[l + s] = L.psi[s];
@@ -544,7 +561,14 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
         float w[CM], dw[CM];
         // inlined: an out-of-line call would keep w and dw in scratch memory
         [[clang::always_inline]] window_axis_s<float, CM, WG>(A.g.win, na.frac, (float)nt, bt, w, dw);
-        if (!bad) {
+        if constexpr (S::SYZ) {
+#pragma unroll
+          for (int s = 0; s < CM; s++) {
+            zw[s] = w[s] * wst;
+            zdw[s] = dw[s] * wst;
+          }
+        }
+        if (!bad && !zlate) {
 #pragma unroll
           for (int s = 0; s < CM; s++) {
             r0[l + s] = w[s] * wst;
@@ -576,14 +600,34 @@ __global__ __launch_bounds__(64 * NW, (GatherMfmaShape<MT, MODE, NW>::WAVES)) vo
     const float* dxr = xr + S::ROWSZ / 2;
     const int ch = 4 * (lk ^ sw4(li));
     const float4 yb = *reinterpret_cast<const float4*>(rows + S::YOFF + li * 20 + 4 * lk);  // Y_j[4 lk + e]
+    float4 dyb = yb;
+    if (WG) dyb = *reinterpret_cast<const float4*>(rows + S::ROWSZ / 2 + S::YOFF + li * 20 + 4 * lk);
+    if constexpr (S::SYZ) {
+      // the y rows are read: clear their area (values and derivatives) and write the z rows there
+      wave_sync();
+      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+      constexpr int NQ = 16 * 20 / 4;  // 16-byte chunks of one area
+#pragma unroll
+      for (int k = 0; k < (2 * NQ + 63) / 64; k++) {
+        const int qq = lane + 64 * k;
+        if (qq < 2 * NQ)
+          reinterpret_cast<float4*>(rows + S::YOFF + (qq >= NQ ? S::ROWSZ / 2 - NQ * 4 : 0))[qq] = z4;
+      }
+      if (zlate && jg >= 0 && !bad) {
+        float* const rz = rows + S::ZOFF + li * 17;
+#pragma unroll
+        for (int s = 0; s < CM; s++) {
+          rz[l + s] = zw[s];
+          rz[S::ROWSZ / 2 + l + s] = zdw[s];
+        }
+      }
+      wave_sync();
+    }
     // Z_j[4 e + lk] in component e: the K-slice of this lane group in the e-th product
     auto zread = [&](const float* row) { return make_float4(row[lk], row[4 + lk], row[8 + lk], row[12 + lk]); };
     const float4 zc = zread(rows + S::ZOFF + li * 17);
-    float4 dyb = yb, dzc = zc;
-    if (WG) {
-      dyb = *reinterpret_cast<const float4*>(rows + S::ROWSZ / 2 + S::YOFF + li * 20 + 4 * lk);
-      dzc = zread(rows + S::ROWSZ / 2 + S::ZOFF + li * 17);
-    }
+    float4 dzc = zc;
+    if (WG) dzc = zread(rows + S::ROWSZ / 2 + S::ZOFF + li * 17);
     float fr = 0.f, fi = 0.f, g0r = 0.f, g0i = 0.f, g1r = 0.f, g1i = 0.f, g2r = 0.f, g2i = 0.f;
     // plane a + 1's operands are read while plane a is on the matrix cores;
     // products e = K0 .. K0 + NK - 1 (the K-blocks the group's stencils reach)
