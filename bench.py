@@ -392,8 +392,8 @@ def main():
         print(json.dumps(t, indent=1))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
-              "described in DESIGN.md but not built", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the slab decomposition "
+              "over several GPUs is not built", file=sys.stderr)
         sys.exit(2)
 
     import torch
