@@ -294,6 +294,7 @@ struct NodesT {
   size_t slots_cap = 0;  // complex elements
   int n_torder = 0;
   int n_heavy = 0;       // torder[0, n_heavy): tiles with 32 nodes or more
+  int n_single = 0;      // the last n_single tiles of torder: one node each
   // offsets of each tile's nodes per axis (build_tile_boxes), by tile id and by position in torder
   unsigned* d_box_tile = nullptr;
   unsigned* d_box_ord = nullptr;
@@ -812,6 +813,7 @@ static void set_tiles(BArgs<R>& A, const NodesT<R>* D, bool for_scatter = false)
     A.torder = D->d_torder;
     A.n_torder = D->n_torder;
     A.n_heavy = D->n_heavy;
+    A.n_single = D->n_single;
     A.stale_count = D->d_stale_count;
     A.stale_list = D->d_stale;
   }
@@ -863,7 +865,7 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx, const GridGeom& g) {
   const int pad = cols_enabled() ? cols_pad(P->m, sizeof(R) == 8, P->tiles.ts) : 0;
   if (pad > 0 && D->M > 0) {
     const int nt = build_tile_order(D->d_tile_start, P->tiles.ntiles(), D->d_twork, D->d_chunks, D->d_tpartial,
-                                    D->d_torder, D->d_tile_slot, &D->n_heavy,
+                                    D->d_torder, D->d_tile_slot, &D->n_heavy, &D->n_single,
                                     P->stream);
     if (nt < 0) {
       set_error("building the tile order failed");

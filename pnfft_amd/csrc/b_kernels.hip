@@ -759,6 +759,101 @@ __global__ __launch_bounds__(256) void gather_stale_k(BArgs<float> A) {
   }
 }
 
+// Tiles of one node (the last n_single of the tile order): one wave per node,
+// gathered straight from the grid (the matrix-core kernels would stage a whole
+// subgrid for it).  Every lane forms the node's windows (the tables when they
+// hold its coordinates), lane r takes the stencil rows (a, b) = (r / CM, r % CM)
+// and r + 64, and the wave adds the lanes' partial sums.
+template <int MT, int MODE>
+__global__ __launch_bounds__(256) void gather_single_k(BArgs<float> A, int first) {
+  constexpr int CM = 2 * MT + 1, NR = CM * CM;
+  constexpr bool WG = (MODE == B_GRAD || MODE == B_FGRAD);
+  const int lane = threadIdx.x & 63;
+  const int p = first + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= A.n_torder) return;  // whole wave; no barrier follows
+  const int pos = A.tile_start[A.torder[p]];  // the tile's node, as a sorted position
+  const long j = A.perm[pos];
+  NodeWin<float, CM> w;
+  node_windows<float, MT, CM, WG>(A, pos, j, MT, w);
+  const int n0 = A.g.n[0], n1 = A.g.n[1], n2 = A.g.n[2];
+  int i2s[CM];
+#pragma unroll
+  for (int s = 0; s < CM; s++) {
+    const int v = w.start[2] + s;
+    i2s[s] = v >= n2 ? v - n2 : v;
+  }
+  float fr = 0.f, fi = 0.f, g0r = 0.f, g0i = 0.f, g1r = 0.f, g1i = 0.f, g2r = 0.f, g2i = 0.f;
+#pragma unroll
+  for (int it = 0; it < (NR + 63) / 64; it++) {
+    const int r = lane + 64 * it;
+    const bool live = r < NR;
+    const int a = live ? r / CM : 0, b = live ? r - (r / CM) * CM : 0;
+    int i0 = w.start[0] + a;
+    if (i0 >= n0) i0 -= n0;
+    int i1 = w.start[1] + b;
+    if (i1 >= n1) i1 -= n1;
+    const long off = ((long)i0 * n1 + i1) * n2;
+    float hr = 0.f, hi = 0.f, hzr = 0.f, hzi = 0.f;
+    if (A.c2r) {  // real grid of a c2r plan
+      const float* row = reinterpret_cast<const float*>(A.grid) + off;
+#pragma unroll
+      for (int s = 0; s < CM; s++) {
+        const float gv = row[i2s[s]];
+        hr += w.pz[s] * gv;
+        if (WG) hzr += w.dz[s] * gv;
+      }
+    } else {
+      const Cplx<float>* row = A.grid + off;
+#pragma unroll
+      for (int s = 0; s < CM; s++) {
+        const Cplx<float> gv = row[i2s[s]];
+        hr += w.pz[s] * gv.x;
+        hi += w.pz[s] * gv.y;
+        if (WG) {
+          hzr += w.dz[s] * gv.x;
+          hzi += w.dz[s] * gv.y;
+        }
+      }
+    }
+    // the windows of plane a and row b (lane-dependent indices, picked by selects)
+    float xa = w.px[0], yb = w.py[0], dxa = WG ? w.dx[0] : 0.f, dyb = WG ? w.dy[0] : 0.f;
+#pragma unroll
+    for (int k = 1; k < CM; k++) {
+      xa = a == k ? w.px[k] : xa;
+      yb = b == k ? w.py[k] : yb;
+      if (WG) {
+        dxa = a == k ? w.dx[k] : dxa;
+        dyb = b == k ? w.dy[k] : dyb;
+      }
+    }
+    const float xy = live ? xa * yb : 0.f;
+    fr += xy * hr;
+    fi += xy * hi;
+    if (WG) {
+      const float dxy = live ? dxa * yb : 0.f, xdy = live ? xa * dyb : 0.f;
+      g0r += dxy * hr;
+      g0i += dxy * hi;
+      g1r += xdy * hr;
+      g1i += xdy * hi;
+      g2r += xy * hzr;
+      g2i += xy * hzi;
+    }
+  }
+  auto red = [](float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+  };
+  Cplx<float> v[4] = {cmk<float>(red(fr), red(fi)), cmk<float>(0.f, 0.f), cmk<float>(0.f, 0.f),
+                      cmk<float>(0.f, 0.f)};
+  if (WG) {
+    v[1] = cmk<float>(red(g0r), red(g0i));
+    v[2] = cmk<float>(red(g1r), red(g1i));
+    v[3] = cmk<float>(red(g2r), red(g2i));
+  }
+  if (lane == 0) gather_store<float, MODE>(A, j, v);
+}
+
 inline unsigned grid1d(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 // the matrix-core gather applies to FP32 f / grad / f + grad with m <= 6 on the
@@ -781,6 +876,7 @@ void launch_gather_mfma(const BArgs<R>& A, hipStream_t s) {
     using S4 = GatherMfmaShape<MT, MODE, 4>;
     using S2 = GatherMfmaShape<MT, MODE, 2>;
     const int nh = min(max(A.n_heavy, 0), A.n_torder);
+    const int nl = max(nh, A.n_torder - max(A.n_single, 0));  // [nl, n_torder): tiles of one node
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, A.stale_count);
     auto go = [&](auto rgc) {
       constexpr bool RG = decltype(rgc)::value;
@@ -789,10 +885,10 @@ void launch_gather_mfma(const BArgs<R>& A, hipStream_t s) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)S4::LDS);
         hipLaunchKernelGGL((gather_mfma_k<MT, MODE, 4, RG>), dim3((unsigned)nh), dim3(S4::NT), S4::LDS, s, A, 0);
       }
-      if (A.n_torder > nh) {  // tiles of at most 31 nodes: two groups, two waves
+      if (nl > nh) {  // tiles of 2..31 nodes: two groups, two waves
         (void)hipFuncSetAttribute((const void*)gather_mfma_k<MT, MODE, 2, RG>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)S2::LDS);
-        hipLaunchKernelGGL((gather_mfma_k<MT, MODE, 2, RG>), dim3((unsigned)(A.n_torder - nh)), dim3(S2::NT),
+        hipLaunchKernelGGL((gather_mfma_k<MT, MODE, 2, RG>), dim3((unsigned)(nl - nh)), dim3(S2::NT),
                            S2::LDS, s, A, nh);
       }
     };
@@ -800,6 +896,8 @@ void launch_gather_mfma(const BArgs<R>& A, hipStream_t s) {
       go(std::true_type{});
     else
       go(std::false_type{});
+    if (A.n_torder > nl)
+      hipLaunchKernelGGL((gather_single_k<MT, MODE>), dim3(grid1d(A.n_torder - nl, 4)), dim3(256), 0, s, A, nl);
     if (A.mid) (void)hipEventRecord(A.mid, s);
     if (A.mid2) (void)hipEventRecord(A.mid2, s);
     hipLaunchKernelGGL((gather_stale_k<MT, MODE>), dim3(256), dim3(256), 0, s, A);

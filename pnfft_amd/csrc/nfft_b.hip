@@ -268,26 +268,34 @@ __global__ void tile_class_scan_k(unsigned* work) {
   work[64] = acc;
 }
 
-// tiles of 1..31 nodes (classes 27..31): flag 1, else 0
+// tiles of 2..31 nodes (classes 27..30): flag[t] = 1; tiles of one node (class
+// 31): flag[ntiles + t] = 1
 __global__ __launch_bounds__(kB) void tile_light_k(const int* __restrict__ tile_start, long ntiles, unsigned* flag) {
   const long t = (long)blockIdx.x * kB + threadIdx.x;
   if (t >= ntiles) return;
   const int cnt = tile_start[t + 1] - tile_start[t];
-  flag[t] = (cnt > 0 && cnt < 32) ? 1u : 0u;
+  flag[t] = (cnt > 1 && cnt < 32) ? 1u : 0u;
+  flag[ntiles + t] = cnt == 1 ? 1u : 0u;
 }
 
 // Tiles of 32 nodes or more are placed class by class (most nodes first); the
-// tiles of 1..31 nodes follow in tile order (light_off = exclusive scan of
+// tiles of 2..31 nodes follow in tile order (light_off = exclusive scan of
 // their flags), so that neighbouring light tiles, whose padded subgrids
-// overlap, are taken by neighbouring blocks.
+// overlap, are taken by neighbouring blocks, and the tiles of one node come
+// last, in tile order as well (single_off).
 __global__ __launch_bounds__(kB) void tile_place_k(const int* __restrict__ tile_start, long ntiles, unsigned* work,
-                                                   const unsigned* __restrict__ light_off, int* order,
+                                                   const unsigned* __restrict__ light_off,
+                                                   const unsigned* __restrict__ single_off, int* order,
                                                    int* tile_slot) {
   const long t = (long)blockIdx.x * kB + threadIdx.x;
   const int cnt = t < ntiles ? tile_start[t + 1] - tile_start[t] : 0;
   const unsigned pos = wave_class_add(cnt >= 32, cnt > 0 ? __clz(cnt) : 0, work + 32);
   if (cnt > 0) {
-    const unsigned p = cnt >= 32 ? pos : work[32 + 27] + light_off[t];  // class 27 starts after the heavy tiles
+    // class 27 starts after the heavy tiles; work[27..30] are the sizes of classes 27..30
+    const unsigned light0 = work[32 + 27];
+    const unsigned p = cnt >= 32 ? pos
+                       : cnt > 1 ? light0 + light_off[t]
+                                 : light0 + work[27] + work[28] + work[29] + work[30] + single_off[t];
     order[p] = (int)t;
     tile_slot[t] = (int)p;
   }
@@ -500,20 +508,23 @@ int build_tile_boxes(const unsigned* keys, long M, const TileLayout& T, const in
 }
 
 int build_tile_order(const int* tile_start, long ntiles, unsigned* work, unsigned* light, unsigned* partial,
-                     int* order, int* tile_slot, int* n_heavy, hipStream_t s) {
+                     int* order, int* tile_slot, int* n_heavy, int* n_single, hipStream_t s) {
   hipLaunchKernelGGL(fill_uint_k, dim3(1), dim3(128), 0, s, work, 65L, 0u);
   hipLaunchKernelGGL(tile_class_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, tile_slot);
   hipLaunchKernelGGL(tile_class_scan_k, dim3(1), dim3(64), 0, s, work);
   hipLaunchKernelGGL(tile_light_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, light);
   scan_exclusive(light, ntiles, partial, s);
-  hipLaunchKernelGGL(tile_place_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, light, order,
-                     tile_slot);
-  // after the placement work[32 + k] is the end of class k; classes k <= 26 hold 32 nodes or more
-  unsigned ends[33] = {0};
-  if (hipMemcpyAsync(ends, work + 32, 33 * sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  scan_exclusive(light + ntiles, ntiles, partial, s);
+  hipLaunchKernelGGL(tile_place_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, light,
+                     light + ntiles, order, tile_slot);
+  // work[k] is the size of class k; after the placement work[32 + k] is the end
+  // of class k for k <= 26 (32 nodes or more) and work[64] the total
+  unsigned w[65] = {0};
+  if (hipMemcpyAsync(w, work, 65 * sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  if (n_heavy) *n_heavy = (int)ends[26];
-  return (int)ends[32];
+  if (n_heavy) *n_heavy = (int)w[32 + 26];
+  if (n_single) *n_single = (int)w[31];
+  return (int)w[64];
 }
 
 template <class R>

@@ -38,6 +38,7 @@ struct BArgs {
   const int* torder;       // [n_torder]
   int n_torder;
   int n_heavy;             // tiles torder[0, n_heavy) hold 32 nodes or more, the rest at most 31
+  int n_single;            // the last n_single tiles of torder hold one node each
   // column scatter (scatter_cols.hip): when slots is set, scatter() runs it instead of the LDS-atomic path
   Cplx<R>* slots;          // [n_torder][P^3] padded subgrid of every nonempty tile
   const int* tile_slot;    // [ntiles] slot of each tile, -1 if empty
@@ -72,13 +73,14 @@ constexpr int kTileNodes = 512;    // nodes per work item
 int cols_pad(int m, bool is_double, const int* ts);
 
 // Nonempty tiles: those of 32 nodes or more by decreasing node count
-// (power-of-two classes, most nodes first), then those of 1..31 nodes in tile
-// order; tile_slot[t] = position of tile t (-1 if empty).  work holds 65
-// unsigned, light ntiles unsigned, partial the scan partials of ntiles values.
+// (power-of-two classes, most nodes first), then those of 2..31 nodes and then
+// those of one node, both in tile order; tile_slot[t] = position of tile t (-1
+// if empty).  work holds 65 unsigned, partial the scan partials of ntiles values.
 // Returns the number of nonempty tiles (host value; synchronizes s), or -1;
-// *n_heavy = the number of them with 32 nodes or more.
+// *n_heavy = the number of them with 32 nodes or more; the tiles of one node
+// come last, *n_single of them.  light holds 2 ntiles unsigned.
 int build_tile_order(const int* tile_start, long ntiles, unsigned* work, unsigned* light, unsigned* partial,
-                     int* order, int* tile_slot, int* n_heavy, hipStream_t s);
+                     int* order, int* tile_slot, int* n_heavy, int* n_single, hipStream_t s);
 
 // Launch of the column scatter for MODE (B_F, B_GRAD, B_FGRAD): column kernel
 // into the slots, fold of the slots into the grid (which it overwrites), then
