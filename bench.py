@@ -119,7 +119,7 @@ def roofline_from_kernels(kt, ab, af, steps, precision, m=4):
 
 # kernels of each class in a rocprofv3 counter collection
 TRAFFIC_CLASSES = {
-    "gather": ("gather_mfma_k", "gather_tiled_k", "gather_direct_k", "gather_stale_k"),
+    "gather": ("gather_mfma_k", "gather_single_k", "gather_tiled_k", "gather_direct_k", "gather_stale_k"),
     "scatter": ("scatter_mfma_k", "scatter_cols_k", "fold_k", "fold8_k", "fold8b_k", "fold4_k", "scatter_stale_k",
                 "scatter_tiled_k", "scatter_direct_k"),
     "fft_rows": ("fft_rows", "fft_rows512"),
