@@ -69,6 +69,15 @@ def algorithmic_flops(w):
     return {"gather_f": 4 * c3 * M, "gather_grad": 8 * c3 * M, "scatter_f": 4 * c3 * M}
 
 
+def window_name(flags):
+    """The window a plan's flags select (include/pnfft.h window bits; none set: Kaiser-Bessel)."""
+    names = ((1 << 17, "gaussian_t"), (1 << 13, "gaussian"), (1 << 14, "bspline"), (1 << 15, "sinc_power"), (1 << 16, "bessel_i0"))
+    for bit, name in names:
+        if flags & bit:
+            return name
+    return "kaiser_bessel"
+
+
 # MI355X peaks: HBM3E 8 TB/s; dense FP32 matrix-core rate 157.3 TFLOP/s (FP64 78.6)
 PEAK_HBM_GBPS = 8000.0
 PEAK_MFMA_TFLOPS = {"float": 157.3, "double": 78.6}
@@ -463,7 +472,7 @@ def main():
         "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if precision == "float" else "f64", "data": "synthetic",
         "config": {"workload": w["name"], "N": list(w["N"]), "n": list(w["n"]), "m": w["m"], "M": w["M"],
-                   "window": "kaiser_bessel" if w["flags"] == 0 else w["flags"], "setup_ms": round(res["setup_ms"], 2),
+                   "window": window_name(w["flags"]), "setup_ms": round(res["setup_ms"], 2),
                    "setup_first_ms": round(res["setup_first_ms"], 2)},
         "accuracy_rel_l2_vs_fp64": acc,
         "accuracy_rel_l2_vs_oracle": acc_oracle,

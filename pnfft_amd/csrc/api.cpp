@@ -882,8 +882,14 @@ static int bin(PlanT<R>* P, NodesT<R>* D, const R* dx, const GridGeom& g) {
     // without slot memory the scatter takes the LDS-atomic path
     D->n_torder = D->d_slots ? nt : 0;
   }
-  D->boxes = D->n_torder > 0 && build_tile_boxes(D->d_keys, D->M, P->tiles, D->d_torder, D->n_torder,
-                                                 D->d_box_tile, D->d_box_ord, P->stream) == kOk;
+  // the boxes serve only the FP32 matrix-core scatter and its fold on 8^3 tiles
+  // whose padded subgrids are a multiple of four cells wide (m = 2, 4)
+  const int pb = 8 + 2 * P->m;
+  const bool want_boxes = sizeof(R) == 4 && P->tiles.ts[0] == 8 && P->tiles.ts[1] == 8 && P->tiles.ts[2] == 8 &&
+                          pb <= 16 && pb % 4 == 0;
+  D->boxes = want_boxes && D->n_torder > 0 &&
+             build_tile_boxes(D->d_keys, D->M, P->tiles, D->d_torder, D->n_torder, D->d_box_tile, D->d_box_ord,
+                              P->stream) == kOk;
   D->bin_geom = g;
   D->bin_tiles = P->tiles;
   D->bins_valid = true;
