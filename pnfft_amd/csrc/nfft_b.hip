@@ -140,18 +140,34 @@ __global__ __launch_bounds__(kSortT) void scan_apply_k(unsigned* d, long n, cons
   }
 }
 
-// Stable scatter of one digit pass: items of a block are ranked in index order.
+// Stable scatter of one digit pass: items of a block are ranked in index order
+// and placed by digit in LDS first, so that the stores to the output are runs
+// of consecutive positions (one per digit present in the block) rather than
+// one scattered 4-byte store per item.  offs: the exclusive scan of the
+// per-(digit, block) counts in digit-major order, so that a count is the
+// difference of neighbouring entries.
 __global__ __launch_bounds__(kSortT) void sort_scatter_k(const unsigned* __restrict__ kin, const int* __restrict__ vin,
                                                          unsigned* kout, int* vout, long M, int shift,
                                                          const unsigned* __restrict__ offs, int nblk) {
   __shared__ unsigned run[256];
   __shared__ unsigned wcnt[4][256];
+  __shared__ unsigned loc[256];   // first LDS position of digit d
+  __shared__ unsigned gdst[256];  // output position of LDS position 0 for digit d
+  __shared__ unsigned tmp[256];
+  __shared__ unsigned sk[kSortIPB];
+  __shared__ int sv[kSortIPB];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  run[tid] = 0;
-  const unsigned boff = 0;
-  (void)boff;
   const long base = (long)blockIdx.x * kSortIPB;
+  {
+    const long e = (long)tid * nblk + blockIdx.x;  // entry (digit tid, this block)
+    const unsigned o = offs[e];
+    const unsigned c = (e + 1 < 256L * nblk ? offs[e + 1] : (unsigned)M) - o;
+    const unsigned l = block_exscan256(c, tmp, nullptr);
+    loc[tid] = l;
+    gdst[tid] = o - l;
+    run[tid] = 0;
+  }
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int r = 0; r < kSortIPB / kSortT; r++) {
     const long idx = base + (long)r * kSortT + tid;
@@ -180,13 +196,20 @@ __global__ __launch_bounds__(kSortT) void sort_scatter_k(const unsigned* __restr
     if (valid) {
       unsigned pre = run[d] + rank;
       for (int w = 0; w < wave; w++) pre += wcnt[w][d];
-      const unsigned pos = offs[(long)d * nblk + blockIdx.x] + pre;
-      kout[pos] = key;
-      vout[pos] = val;
+      const unsigned at = loc[d] + pre;  // < the block's item count: the counts of all digits sum to it
+      sk[at] = key;
+      sv[at] = val;
     }
     __syncthreads();
     run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
     __syncthreads();
+  }
+  const int nv = (int)min((long)kSortIPB, M - base);
+  for (int i = tid; i < nv; i += kSortT) {
+    const unsigned key = sk[i];
+    const unsigned pos = gdst[(key >> shift) & 255u] + (unsigned)i;
+    kout[pos] = key;
+    vout[pos] = sv[i];
   }
 }
 
@@ -247,14 +270,26 @@ __device__ inline unsigned wave_class_add(bool act, int cls, unsigned* ctr) {
   return pos;
 }
 
+// Tiles per block of tile_class_k and tile_place_k: the class counts are
+// gathered in LDS over this many tiles, so that a block adds each class to the
+// global counters once (per-wave global atomics on the 32 counters serialize).
+constexpr int kClassTiles = 8 * kB;
+
 // nonempty tiles by node-count class (leading zeros of the count: more nodes,
 // smaller class), then placed class by class: most nodes first
 __global__ __launch_bounds__(kB) void tile_class_k(const int* __restrict__ tile_start, long ntiles, unsigned* work,
                                                    int* tile_slot) {
-  const long t = (long)blockIdx.x * kB + threadIdx.x;
-  const int cnt = t < ntiles ? tile_start[t + 1] - tile_start[t] : 0;
-  if (t < ntiles) tile_slot[t] = -1;
-  (void)wave_class_add(cnt > 0, cnt > 0 ? __clz(cnt) : 0, work);
+  __shared__ unsigned lc[32];
+  if (threadIdx.x < 32) lc[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = 0; i < kClassTiles / kB; i++) {
+    const long t = (long)blockIdx.x * kClassTiles + (long)i * kB + threadIdx.x;
+    const int cnt = t < ntiles ? tile_start[t + 1] - tile_start[t] : 0;
+    if (t < ntiles) tile_slot[t] = -1;
+    (void)wave_class_add(cnt > 0, cnt > 0 ? __clz(cnt) : 0, lc);
+  }
+  __syncthreads();
+  if (threadIdx.x < 32 && lc[threadIdx.x]) atomicAdd(&work[threadIdx.x], lc[threadIdx.x]);
 }
 
 // work[0..31] class sizes -> work[32..63] class offsets, work[64] total
@@ -278,7 +313,8 @@ __global__ __launch_bounds__(kB) void tile_light_k(const int* __restrict__ tile_
   flag[ntiles + t] = cnt == 1 ? 1u : 0u;
 }
 
-// Tiles of 32 nodes or more are placed class by class (most nodes first); the
+// Tiles of 32 nodes or more are placed class by class (most nodes first; in a
+// class, block by block, a block's tiles in the order of their LDS ranks); the
 // tiles of 2..31 nodes follow in tile order (light_off = exclusive scan of
 // their flags), so that neighbouring light tiles, whose padded subgrids
 // overlap, are taken by neighbouring blocks, and the tiles of one node come
@@ -287,17 +323,32 @@ __global__ __launch_bounds__(kB) void tile_place_k(const int* __restrict__ tile_
                                                    const unsigned* __restrict__ light_off,
                                                    const unsigned* __restrict__ single_off, int* order,
                                                    int* tile_slot) {
-  const long t = (long)blockIdx.x * kB + threadIdx.x;
-  const int cnt = t < ntiles ? tile_start[t + 1] - tile_start[t] : 0;
-  const unsigned pos = wave_class_add(cnt >= 32, cnt > 0 ? __clz(cnt) : 0, work + 32);
-  if (cnt > 0) {
-    // class 27 starts after the heavy tiles; work[27..30] are the sizes of classes 27..30
-    const unsigned light0 = work[32 + 27];
-    const unsigned p = cnt >= 32 ? pos
-                       : cnt > 1 ? light0 + light_off[t]
-                                 : light0 + work[27] + work[28] + work[29] + work[30] + single_off[t];
-    order[p] = (int)t;
-    tile_slot[t] = (int)p;
+  constexpr int NI = kClassTiles / kB;
+  __shared__ unsigned lc[32], gb[32];
+  if (threadIdx.x < 32) lc[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned lpos[NI];
+#pragma unroll
+  for (int i = 0; i < NI; i++) {
+    const long t = (long)blockIdx.x * kClassTiles + (long)i * kB + threadIdx.x;
+    const int cnt = t < ntiles ? tile_start[t + 1] - tile_start[t] : 0;
+    lpos[i] = wave_class_add(cnt >= 32, cnt > 0 ? __clz(cnt) : 0, lc);
+  }
+  __syncthreads();
+  if (threadIdx.x < 32 && lc[threadIdx.x]) gb[threadIdx.x] = atomicAdd(&work[32 + threadIdx.x], lc[threadIdx.x]);
+  __syncthreads();
+  // class 27 starts after the heavy tiles; work[27..30] are the sizes of classes 27..30
+  const unsigned light0 = work[32 + 27];
+  const unsigned single0 = light0 + work[27] + work[28] + work[29] + work[30];
+#pragma unroll
+  for (int i = 0; i < NI; i++) {
+    const long t = (long)blockIdx.x * kClassTiles + (long)i * kB + threadIdx.x;
+    const int cnt = t < ntiles ? tile_start[t + 1] - tile_start[t] : 0;
+    if (cnt > 0) {
+      const unsigned p = cnt >= 32 ? gb[__clz(cnt)] + lpos[i] : cnt > 1 ? light0 + light_off[t] : single0 + single_off[t];
+      order[p] = (int)t;
+      tile_slot[t] = (int)p;
+    }
   }
 }
 
@@ -510,13 +561,14 @@ int build_tile_boxes(const unsigned* keys, long M, const TileLayout& T, const in
 int build_tile_order(const int* tile_start, long ntiles, unsigned* work, unsigned* light, unsigned* partial,
                      int* order, int* tile_slot, int* n_heavy, int* n_single, hipStream_t s) {
   hipLaunchKernelGGL(fill_uint_k, dim3(1), dim3(128), 0, s, work, 65L, 0u);
-  hipLaunchKernelGGL(tile_class_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, tile_slot);
+  hipLaunchKernelGGL(tile_class_k, dim3(grid1d(ntiles, kClassTiles)), dim3(kB), 0, s, tile_start, ntiles, work,
+                     tile_slot);
   hipLaunchKernelGGL(tile_class_scan_k, dim3(1), dim3(64), 0, s, work);
   hipLaunchKernelGGL(tile_light_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, light);
   scan_exclusive(light, ntiles, partial, s);
   scan_exclusive(light + ntiles, ntiles, partial, s);
-  hipLaunchKernelGGL(tile_place_k, dim3(grid1d(ntiles, kB)), dim3(kB), 0, s, tile_start, ntiles, work, light,
-                     light + ntiles, order, tile_slot);
+  hipLaunchKernelGGL(tile_place_k, dim3(grid1d(ntiles, kClassTiles)), dim3(kB), 0, s, tile_start, ntiles, work,
+                     light, light + ntiles, order, tile_slot);
   // work[k] is the size of class k; after the placement work[32 + k] is the end
   // of class k for k <= 26 (32 nodes or more) and work[64] the total
   unsigned w[65] = {0};
