@@ -516,28 +516,31 @@ int cols_pad(int m, bool is_double, const int* ts) {
 // cells along axis 2 outermost, then axis 0, axis 1 and the cell within the
 // run) is decoded and OR-ed into box[tile] (bits 8 t + l for offset l on axis
 // t; tile sizes of at most 8).
-// The keys are sorted, so most waves see one tile: they OR their lanes' masks
-// first and make one atomic instead of 64 on the same word.
+// The keys are sorted, so the lanes of a wave that share a tile are
+// consecutive: a segmented OR scan over each such run leaves the run's OR in
+// its last lane, which makes the one atomic for that tile (instead of one per
+// lane on the same word).
 __global__ __launch_bounds__(kB) void tile_box_k(const unsigned* __restrict__ keys, long M, TileLayout T,
                                                  unsigned* box) {
   const long i = (long)blockIdx.x * kB + threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const bool live = i < M;
   const unsigned vol = (unsigned)(T.ts[0] * T.ts[1] * T.ts[2]);
-  const unsigned key = live ? keys[i] : 0u, tile = key / vol, in = key % vol;
+  const unsigned key = live ? keys[i] : 0u, in = key % vol;
+  const unsigned tile = live ? key / vol : 0xFFFFFFFFu;  // no tile id reaches the sentinel
   const unsigned q = T.ts[2] < 4 ? (unsigned)T.ts[2] : 4u;
   const unsigned r = in % q, rest = in / q;  // rest = (l2 / q) * TS0 * TS1 + l0 * TS1 + l1
   const unsigned l1 = rest % (unsigned)T.ts[1], l0 = (rest / (unsigned)T.ts[1]) % (unsigned)T.ts[0];
   const unsigned l2 = (rest / (unsigned)(T.ts[0] * T.ts[1])) * q + r;
   unsigned m = live ? (1u << l0) | (1u << (8 + l1)) | (1u << (16 + l2)) : 0u;
-  const unsigned t0 = (unsigned)__builtin_amdgcn_readfirstlane((int)tile);
-  const bool one = __ballot(live && tile != t0) == 0ull;  // every live lane holds tile t0
-  if (one) {
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) m |= __shfl_xor(m, d);
-    if ((threadIdx.x & 63) == 0 && m) atomicOr(box + t0, m);
-  } else if (live) {
-    atomicOr(box + tile, m);
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned o = (unsigned)__shfl_up((int)m, d);
+    const unsigned to = (unsigned)__shfl_up((int)tile, d);
+    if (lane >= d && to == tile) m |= o;
   }
+  const unsigned tn = (unsigned)__shfl_down((int)tile, 1);
+  if (live && (lane == 63 || tn != tile)) atomicOr(box + tile, m);
 }
 
 // box_ord[p] = box_tile[order[p]]: the masks by position in the tile order
