@@ -391,19 +391,24 @@ static int sort_bins(const R* x, long M, const GridGeom& g, const TileLayout& T,
   const int passes = sort_passes(key_limit);
   const int nblk = (int)((M + kSortIPB - 1) / kSortIPB);
   hipLaunchKernelGGL((bin_keys_k<R>), dim3(grid1d(M, kB)), dim3(kB), 0, s, x, M, g, T, W.keys[0], W.vals[0]);
+  // the last pass stores straight into the outputs (separate from the scratch)
   int cur = 0;
   for (int ps = 0; ps < passes; ps++) {
     const int shift = 8 * ps;
+    const bool last = ps == passes - 1;
     hipLaunchKernelGGL(sort_hist_k, dim3(nblk), dim3(kSortT), 0, s, W.keys[cur], M, shift, W.hist, nblk);
     scan_exclusive(W.hist, 256L * nblk, W.partial, s);
-    hipLaunchKernelGGL(sort_scatter_k, dim3(nblk), dim3(kSortT), 0, s, W.keys[cur], W.vals[cur], W.keys[1 - cur],
-                       W.vals[1 - cur], M, shift, W.hist, nblk);
+    hipLaunchKernelGGL(sort_scatter_k, dim3(nblk), dim3(kSortT), 0, s, W.keys[cur], W.vals[cur],
+                       last ? keys_out : W.keys[1 - cur], last ? perm_out : W.vals[1 - cur], M, shift, W.hist,
+                       nblk);
     cur = 1 - cur;
   }
-  if (hipMemcpyAsync(perm_out, W.vals[cur], sizeof(int) * M, hipMemcpyDeviceToDevice, s) != hipSuccess)
-    return kHipError;
-  if (hipMemcpyAsync(keys_out, W.keys[cur], sizeof(unsigned) * M, hipMemcpyDeviceToDevice, s) != hipSuccess)
-    return kHipError;
+  if (passes == 0) {  // keys below 2: already in order
+    if (hipMemcpyAsync(perm_out, W.vals[0], sizeof(int) * M, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return kHipError;
+    if (hipMemcpyAsync(keys_out, W.keys[0], sizeof(unsigned) * M, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return kHipError;
+  }
   return hipGetLastError() == hipSuccess ? kOk : kHipError;
 }
 
